@@ -1,0 +1,238 @@
+// The extern "C" boundary (include/trt_capi.h): argument checking, error
+// capture and the host side of every entry point.
+#include "trt_internal.h"
+
+#include <cstring>
+
+using trt::Ctx;
+using trt::Error;
+
+struct trt_ctx {
+    Ctx c;
+};
+
+namespace {
+
+template <typename F>
+int guarded(trt_ctx *ctx, F &&f)
+{
+    if (!ctx) return -1;
+    try {
+        f(ctx->c);
+        return 0;
+    } catch (const Error &e) {
+        ctx->c.err = e.msg;
+    } catch (const std::exception &e) {
+        ctx->c.err = e.what();
+    }
+    return -2;
+}
+
+thread_local std::string g_create_error;
+
+void validate(const trt_problem *p)
+{
+    if (!p) throw Error("null problem");
+    if (p->nx < 2 || p->ny < 2) throw Error("N_x and N_y must be at least 2 (QD_SOLVERS.f08 couples neighbouring cells)");
+    if (p->nm < 4 || p->nm % 4) throw Error("N_m must be a positive multiple of 4");
+    if (p->ng < 1) throw Error("N_g must be positive");
+    if (p->nmat < 1) throw Error("need at least one material");
+    if (!p->dx || !p->dy || !p->ox || !p->oy || !p->w || !p->nu || !p->mat || !p->kmult)
+        throw Error("null problem vector");
+    for (int k = 0; k < 4; ++k)
+        if (p->bc_type[k] != 0) throw Error("only incoming-Planckian boundaries (BC_Type 0) are supported");
+    if (p->precision != 0 && p->precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 sweep)");
+    if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
+    for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
+        if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int trt_ctx_create(trt_ctx **out, const trt_problem *p)
+{
+    if (!out) return -1;
+    *out = nullptr;
+    trt_ctx *ctx = nullptr;
+    try {
+        validate(p);
+        ctx = new trt_ctx();
+        Ctx &c = ctx->c;
+        c.p = *p;
+        c.nx = p->nx; c.ny = p->ny; c.nm = p->nm; c.ng = p->ng; c.nmat = p->nmat;
+        c.precision = p->precision;
+        c.lin_rtol = p->lin_rtol > 0 ? p->lin_rtol : 1e-13;
+        c.lin_maxit = p->lin_maxit > 0 ? p->lin_maxit : 20000;
+        TRT_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        TRT_CHECK(hipEventCreate(&c.ev0));
+        TRT_CHECK(hipEventCreate(&c.ev1));
+        trt::init_state(c);
+        // the problem's host vectors are not kept
+        c.p.dx = c.p.dy = c.p.ox = c.p.oy = c.p.w = c.p.nu = c.p.kmult = nullptr;
+        c.p.mat = nullptr;
+        *out = ctx;
+        return 0;
+    } catch (const Error &e) {
+        g_create_error = e.msg;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+    }
+    if (ctx) trt_ctx_destroy(ctx);
+    return -2;
+}
+
+int trt_ctx_destroy(trt_ctx *ctx)
+{
+    if (!ctx) return -1;
+    Ctx &c = ctx->c;
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (void *p : c.owned) (void)hipFree(p);
+    if (c.hbuf) (void)hipHostFree(c.hbuf);
+    if (c.ev0) (void)hipEventDestroy(c.ev0);
+    if (c.ev1) (void)hipEventDestroy(c.ev1);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return 0;
+}
+
+const char *trt_last_error(const trt_ctx *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->c.err.c_str();
+}
+
+size_t trt_ctx_bytes(const trt_problem *p)
+{
+    if (!p) return 0;
+    return trt::sweep_bytes(*p, nullptr);
+}
+
+size_t trt_array_size(const trt_ctx *ctx, const char *name)
+{
+    if (!ctx || !name) return 0;
+    const Ctx &c = ctx->c;
+    if (!strcmp(name, "I_crn") || !strcmp(name, "I_crn_old")) return (size_t)4 * c.nx * c.ny * c.nm * c.ng;
+    auto it = c.arrays.find(name);
+    return it == c.arrays.end() ? 0 : it->second.size();
+}
+
+int trt_push(trt_ctx *ctx, const char *name, const double *host, size_t n)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!name || !host) throw Error("null argument");
+        if (n != trt_array_size(ctx, name)) throw Error(std::string("size mismatch for ") + name);
+        if (!strcmp(name, "I_crn")) { trt::push_icrn(c, c.icur, host); return; }
+        if (!strcmp(name, "I_crn_old")) {
+            // a pushed I_crn_old gets its own buffer when it shares one with I_crn
+            if (c.iold == c.icur) c.iold = 1 - c.icur;
+            trt::push_icrn(c, c.iold, host);
+            return;
+        }
+        TRT_CHECK(hipMemcpyAsync(c.arr(name), host, n * 8, hipMemcpyHostToDevice, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int trt_pull(trt_ctx *ctx, const char *name, double *host, size_t n)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!name || !host) throw Error("null argument");
+        if (n != trt_array_size(ctx, name)) throw Error(std::string("size mismatch for ") + name);
+        if (!strcmp(name, "I_crn")) { trt::pull_icrn(c, c.icur, host); return; }
+        if (!strcmp(name, "I_crn_old")) { trt::pull_icrn(c, c.iold, host); return; }
+        TRT_CHECK(hipMemcpyAsync(host, c.arr(name), n * 8, hipMemcpyDeviceToHost, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+#define SIMPLE(fn, call)                                                          \
+    int fn(trt_ctx *ctx)                                                          \
+    {                                                                             \
+        return guarded(ctx, [&](Ctx &c) { call(c); TRT_CHECK(hipStreamSynchronize(c.stream)); }); \
+    }
+
+SIMPLE(trt_material_update, trt::material_update)
+SIMPLE(trt_collapse_intensities, trt::collapse_intensities)
+SIMPLE(trt_fg_calc, trt::fg_calc)
+SIMPLE(trt_cg_calc, trt::cg_calc)
+SIMPLE(trt_mgqd_in_calc, trt::mgqd_in_calc)
+SIMPLE(trt_gqd_in_calc, trt::gqd_in_calc)
+SIMPLE(trt_old_mgqd_coefs, trt::old_mgqd_coefs)
+SIMPLE(trt_old_grey_coefs, trt::old_grey_coefs)
+SIMPLE(trt_collapse_mg_ef, trt::collapse_mg_ef)
+SIMPLE(trt_cbar_calc, trt::cbar_calc)
+SIMPLE(trt_grey_coefs, trt::grey_coefs)
+
+int trt_transport_scb(trt_ctx *ctx)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        TRT_CHECK(hipEventRecord(c.ev0, c.stream));
+        trt::transport_sweep(c);
+        TRT_CHECK(hipEventRecord(c.ev1, c.stream));
+        TRT_CHECK(hipEventSynchronize(c.ev1));
+        float ms = 0.f;
+        TRT_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        c.sweeps += 1;
+        c.sweep_ms += ms;
+    });
+}
+
+int trt_mloqd_fv(trt_ctx *ctx, int *kits)
+{
+    return guarded(ctx, [&](Ctx &c) { trt::mloqd_fv(c, kits); TRT_CHECK(hipStreamSynchronize(c.stream)); });
+}
+
+int trt_egp_fv_newt(trt_ctx *ctx, int mgqd_it, int *its)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        const int n = trt::egp_fv_newt(c, mgqd_it);
+        if (its) *its = n;
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *x_old2, int conv_type, double eps,
+                    int its, double *norm, double *rho, int *converged)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        const size_t n = trt_array_size(ctx, x);
+        if (n == 0 || n != trt_array_size(ctx, x_old) || n != trt_array_size(ctx, x_old2))
+            throw Error("convergence arrays must exist and have equal sizes");
+        trt::convergence(c, c.arr(x), c.arr(x_old), c.arr(x_old2), (int)n, conv_type, eps, its, norm, rho, converged);
+    });
+}
+
+int trt_step(trt_ctx *ctx, int its_out[4])
+{
+    return guarded(ctx, [&](Ctx &c) { trt::step(c, its_out); });
+}
+
+double trt_time(const trt_ctx *ctx) { return ctx ? ctx->c.time : 0.0; }
+
+int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (nsweeps < 1) throw Error("nsweeps must be positive");
+        TRT_CHECK(hipEventRecord(c.ev0, c.stream));
+        for (int k = 0; k < nsweeps; ++k) trt::transport_sweep(c);
+        TRT_CHECK(hipEventRecord(c.ev1, c.stream));
+        TRT_CHECK(hipEventSynchronize(c.ev1));
+        float t = 0.f;
+        TRT_CHECK(hipEventElapsedTime(&t, c.ev0, c.ev1));
+        c.sweeps += nsweeps;
+        c.sweep_ms += t;
+        if (ms) *ms = t;
+    });
+}
+
+int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms)
+{
+    if (!ctx) return -1;
+    if (sweeps) *sweeps = ctx->c.sweeps;
+    if (ms) *ms = ctx->c.sweep_ms;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Internal declarations shared by the HIP translation units of the TRT library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/trt_capi.h"
+
+#define TRT_CHECK(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            throw trt::Error(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+        }                                                                                  \
+    } while (0)
+
+namespace trt {
+
+struct Error {
+    std::string msg;
+    explicit Error(std::string m) : msg(std::move(m)) {}
+};
+
+constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double H_PLANCK = 6.62613e-19;   // erg*sh (MainSrc/main.f08:18)
+constexpr double ERG = 6.24150934e11;      // eV/erg (MainSrc/main.f08:21)
+constexpr double C_CONST = 2.99792458e2;   // cm/sh, as hard-coded in several reference routines
+
+// A device array registered under its reference name with its Fortran shape.
+struct DevArray {
+    double *p = nullptr;
+    int d[4] = {1, 1, 1, 1};
+    size_t size() const { return (size_t)d[0] * d[1] * d[2] * d[3]; }
+};
+
+// Sweep launch geometry.
+struct SweepGeom {
+    int K = 0;        // directions per quadrant (N_m/4)
+    int gblk = 0;     // groups per block (gblk*K <= 256)
+    int nblk_g = 0;   // ceil(ng/gblk)
+    int TX = 8, TY = 8, NTX = 0, NTY = 0;
+};
+
+// Batched reduced-QD systems (QD_FV) stored as coefficient arrays on device.
+struct QDSystems {
+    int nsys = 0, nx = 0, ny = 0;
+    size_t nrow = 0;              // unknowns per system: 3 nx ny + nx + ny
+    double *coef = nullptr;       // [nsys][16][nx*ny]
+    double *bnd = nullptr;        // [nsys][8][max(nx,ny)]
+    double *diag = nullptr;       // [nsys][nrow] Jacobi diagonal
+    double *b = nullptr;          // [nsys][nrow]
+    double *x = nullptr;          // [nsys][nrow] solution (initial guess on entry)
+    double *work = nullptr;       // 7 * nsys * nrow
+    double *red = nullptr;        // reduction partials
+    double *scal = nullptr;       // per-system scalars
+    int *flags = nullptr;         // per-system convergence flags (device)
+    int red_blocks = 0;
+};
+
+struct Ctx {
+    trt_problem p{};
+    int nx = 0, ny = 0, nm = 0, ng = 0, nmat = 0;
+    int precision = 0;
+    double time = 0.0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    long long sweeps = 0;
+    double sweep_ms = 0.0;
+
+    // registry of named FP64 device arrays (reference names)
+    std::map<std::string, DevArray> arrays;
+    std::vector<void *> owned;       // every hipMalloc of this context
+
+    // problem vectors on device
+    double *dx = nullptr, *dy = nullptr, *A = nullptr, *ox = nullptr, *oy = nullptr, *w = nullptr;
+    double *nu = nullptr, *kmult = nullptr;
+    int *mat = nullptr;
+    double *bcin = nullptr;           // [4][ng] incoming boundary intensities (L,B,R,T)
+    std::vector<double> h_bcin;       // host copy
+
+    // sweep state
+    SweepGeom sg;
+    void *Icrn[2] = {nullptr, nullptr};   // [cell][q][L][4], FP64 or FP32
+    int icur = 0;                          // buffer holding the latest I_crn
+    int iold = 0;                          // buffer holding I_crn_old
+    void *XV = nullptr, *XH = nullptr;     // tile face exchange
+    double *sc = nullptr;                  // [cell][g][2] = (KapE + 1/(c dt), RT_Src)
+    double *Pavg = nullptr, *PV = nullptr, *PH = nullptr;   // per-quadrant partial moments
+    size_t icrn_bytes = 0;
+
+    // low-order systems
+    QDSystems mg, grey;
+    double *scratch = nullptr;             // general FP64 scratch
+    size_t scratch_n = 0;
+    double *hbuf = nullptr;                // pinned host staging for scalars
+    int lin_maxit = 0;
+    double lin_rtol = 1e-13;
+    std::vector<int> last_kits;
+
+    double *arr(const char *name);
+    DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
+    void *dalloc(size_t bytes);
+};
+
+// --- launchers (each enqueues on ctx.stream) ---
+void init_state(Ctx &c);
+void material_update(Ctx &c);
+void transport_sweep(Ctx &c);
+void collapse_intensities(Ctx &c);
+void fg_calc(Ctx &c);
+void cg_calc(Ctx &c);
+void mgqd_in_calc(Ctx &c);
+void gqd_in_calc(Ctx &c);
+void old_mgqd_coefs(Ctx &c);
+void old_grey_coefs(Ctx &c);
+void mloqd_fv(Ctx &c, int *kits);
+void collapse_mg_ef(Ctx &c);
+void cbar_calc(Ctx &c);
+void grey_coefs(Ctx &c);
+int egp_fv_newt(Ctx &c, int mgqd_it);
+void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, int n, int conv_type,
+                 double eps, int its, double *norm, double *rho, int *conv);
+int step(Ctx &c, int its_out[4]);
+void pull_icrn(Ctx &c, int which, double *host);
+void push_icrn(Ctx &c, int which, const double *host);
+void sweep_setup(Ctx &c);
+size_t sweep_bytes(const trt_problem &p, SweepGeom *out);
+void qd_alloc(Ctx &c, QDSystems &s, int nsys);
+// solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess
+void qd_solve(Ctx &c, QDSystems &s, int *its_out);
+// device copy helper
+void dcopy(Ctx &c, double *dst, const double *src, size_t n);
+void dfill(Ctx &c, double *dst, double v, size_t n);
+double dmaxabs(Ctx &c, const double *x, size_t n);
+
+}  // namespace trt

@@ -1,0 +1,219 @@
+"""ctypes binding of the C ABI in include/trt_capi.h (libtrt_hip.so).
+
+The product path: every call goes to the HIP library.  There is no CPU
+fallback; if the library is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtrt_hip.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class TrtProblem(C.Structure):
+    """Mirror of trt_problem (include/trt_capi.h)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nm", C.c_int), ("ng", C.c_int), ("nmat", C.c_int),
+        ("dx", _dp), ("dy", _dp), ("ox", _dp), ("oy", _dp), ("w", _dp), ("nu", _dp),
+        ("mat", C.POINTER(C.c_int)), ("kmult", _dp),
+        ("c", C.c_double), ("dt", C.c_double), ("theta", C.c_double), ("comp_unit", C.c_double),
+        ("cv", C.c_double), ("kap0", C.c_double), ("tini", C.c_double),
+        ("bc_type", C.c_int * 4), ("bcT", C.c_double * 4),
+        ("opacity_model", C.c_int),
+        ("maxit_rte", C.c_int), ("maxit_mloqd", C.c_int), ("maxit_gloqd", C.c_int), ("conv_type", C.c_int),
+        ("conv_ho", C.c_double), ("conv_lo", C.c_double), ("conv_gr1", C.c_double), ("conv_gr2", C.c_double),
+        ("chi", C.c_double), ("line_src", C.c_double),
+        ("use_line_search", C.c_int), ("use_safety_search", C.c_int), ("kapE_dT_flag", C.c_int),
+        ("E_bound_low", C.c_double), ("T_bound_low", C.c_double),
+        ("threads", C.c_int),
+        ("precision", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
+        ("lin_rtol", C.c_double), ("lin_maxit", C.c_int),
+    ]
+
+
+EXPORTS = [
+    "trt_ctx_create", "trt_ctx_destroy", "trt_last_error", "trt_ctx_bytes", "trt_array_size", "trt_push", "trt_pull",
+    "trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
+    "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_mloqd_fv", "trt_collapse_mg_ef",
+    "trt_cbar_calc", "trt_grey_coefs", "trt_gqd_in_calc", "trt_egp_fv_newt", "trt_convergence", "trt_step",
+    "trt_time", "trt_sweep_timed", "trt_sweep_stats",
+]
+
+_lib = None
+
+
+class TrtError(RuntimeError):
+    pass
+
+
+def load(path: str = LIBPATH):
+    """Load libtrt_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TrtError(f"{path} is missing: build it with __graft_entry__.build() "
+                       f"(python 2d_trt_rom_system_amd/build.py)")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.trt_ctx_create.argtypes = [C.POINTER(vp), C.POINTER(TrtProblem)]
+    L.trt_ctx_destroy.argtypes = [vp]
+    L.trt_last_error.argtypes = [vp]
+    L.trt_last_error.restype = C.c_char_p
+    L.trt_ctx_bytes.argtypes = [C.POINTER(TrtProblem)]
+    L.trt_ctx_bytes.restype = C.c_size_t
+    L.trt_array_size.argtypes = [vp, C.c_char_p]
+    L.trt_array_size.restype = C.c_size_t
+    L.trt_push.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
+    L.trt_pull.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
+    for f in ("trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
+              "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_collapse_mg_ef", "trt_cbar_calc",
+              "trt_grey_coefs", "trt_gqd_in_calc"):
+        getattr(L, f).argtypes = [vp]
+    L.trt_mloqd_fv.argtypes = [vp, C.POINTER(C.c_int)]
+    L.trt_egp_fv_newt.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.trt_convergence.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_int, _dp, _dp,
+                                  C.POINTER(C.c_int)]
+    L.trt_step.argtypes = [vp, C.POINTER(C.c_int * 4)]
+    L.trt_time.argtypes = [vp]
+    L.trt_time.restype = C.c_double
+    L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
+    L.trt_sweep_stats.argtypes = [vp, C.POINTER(C.c_longlong), _dp]
+    _lib = L
+    return L
+
+
+def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0):
+    """Fill a TrtProblem from a problem.Problem; returns (struct, keep-alive list)."""
+    keep = []
+
+    def d(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(_dp)
+
+    mat = np.ascontiguousarray(np.asfortranarray(prob.mat).ravel(order="F"), dtype=np.int32)
+    keep.append(mat)
+    s = TrtProblem()
+    s.nx, s.ny, s.nm, s.ng, s.nmat = prob.nx, prob.ny, prob.nm, prob.ng, prob.nmat
+    s.dx, s.dy, s.ox, s.oy, s.w, s.nu = d(prob.dx), d(prob.dy), d(prob.ox), d(prob.oy), d(prob.w), d(prob.nu)
+    s.mat = mat.ctypes.data_as(C.POINTER(C.c_int))
+    s.kmult = d(prob.kmult)
+    s.c, s.dt, s.theta, s.comp_unit = prob.c, prob.dt, prob.theta, prob.comp_unit
+    s.cv, s.kap0, s.tini = prob.cv, prob.kap0, prob.tini
+    for k in range(4):
+        s.bc_type[k] = int(prob.bc_type[k])
+        s.bcT[k] = float(prob.bcT[k])
+    s.opacity_model = prob.opacity_model
+    s.maxit_rte, s.maxit_mloqd, s.maxit_gloqd, s.conv_type = (prob.maxit_rte, prob.maxit_mloqd, prob.maxit_gloqd,
+                                                              prob.conv_type)
+    s.conv_ho, s.conv_lo, s.conv_gr1, s.conv_gr2 = prob.conv_ho, prob.conv_lo, prob.conv_gr1, prob.conv_gr2
+    s.chi, s.line_src = prob.chi, prob.line_src
+    s.use_line_search, s.use_safety_search, s.kapE_dT_flag = (prob.use_line_search, prob.use_safety_search,
+                                                              prob.kapE_dT_flag)
+    s.E_bound_low, s.T_bound_low = prob.E_bound_low, prob.T_bound_low
+    s.threads = prob.threads
+    s.precision = precision
+    s.tile_x, s.tile_y = tile
+    s.lin_rtol, s.lin_maxit = lin_rtol, lin_maxit
+    return s, keep
+
+
+def device_bytes(prob, precision: int = 0, tile=(0, 0)) -> int:
+    s, keep = make_problem_struct(prob, precision, tile)
+    return int(load().trt_ctx_bytes(C.byref(s)))
+
+
+class Context:
+    """One device-resident problem (a trt_ctx)."""
+
+    def __init__(self, prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0):
+        self.L = load()
+        self.prob = prob
+        self.precision = precision
+        s, keep = make_problem_struct(prob, precision, tile, lin_rtol, lin_maxit)
+        h = C.c_void_p()
+        rc = self.L.trt_ctx_create(C.byref(h), C.byref(s))
+        if rc != 0:
+            raise TrtError(f"trt_ctx_create failed ({rc}): {self.L.trt_last_error(None).decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.trt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise TrtError(f"{what} failed ({rc}): {self.L.trt_last_error(self.h).decode()}")
+
+    def size(self, name: str) -> int:
+        return int(self.L.trt_array_size(self.h, name.encode()))
+
+    def pull(self, name: str, shape=None) -> np.ndarray:
+        n = self.size(name)
+        if n == 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.L.trt_pull(self.h, name.encode(), out.ctypes.data_as(_dp), n), f"pull {name}")
+        if shape is not None:
+            return out.reshape(shape, order="F")
+        return out
+
+    def push(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel(order="F"))
+        self._check(self.L.trt_push(self.h, name.encode(), a.ctypes.data_as(_dp), a.size), f"push {name}")
+
+    def call(self, routine: str, *args):
+        rc = getattr(self.L, "trt_" + routine)(self.h, *args)
+        self._check(rc, routine)
+
+    def mloqd_fv(self):
+        kits = (C.c_int * self.prob.ng)()
+        self._check(self.L.trt_mloqd_fv(self.h, kits), "mloqd_fv")
+        return list(kits)
+
+    def egp_fv_newt(self, mgqd_it: int) -> int:
+        its = C.c_int()
+        self._check(self.L.trt_egp_fv_newt(self.h, int(mgqd_it), C.byref(its)), "egp_fv_newt")
+        return its.value
+
+    def convergence(self, x, xo, xo2, conv_type, eps, its):
+        nrm, rho, conv = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.L.trt_convergence(self.h, x.encode(), xo.encode(), xo2.encode(), int(conv_type), float(eps),
+                                           int(its), C.byref(nrm), C.byref(rho), C.byref(conv)), "convergence")
+        return bool(conv.value), nrm.value, rho.value
+
+    def step(self):
+        it = (C.c_int * 4)()
+        self._check(self.L.trt_step(self.h, C.byref(it)), "step")
+        return tuple(it)
+
+    @property
+    def time(self) -> float:
+        return float(self.L.trt_time(self.h))
+
+    def sweep_timed(self, nsweeps: int) -> float:
+        ms = C.c_double()
+        self._check(self.L.trt_sweep_timed(self.h, int(nsweeps), C.byref(ms)), "sweep_timed")
+        return ms.value
+
+    def sweep_stats(self):
+        n, ms = C.c_longlong(), C.c_double()
+        self.L.trt_sweep_stats(self.h, C.byref(n), C.byref(ms))
+        return n.value, ms.value

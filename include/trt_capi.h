@@ -1,0 +1,119 @@
+/*
+ * trt_capi.h -- C ABI of the MI355X-native full-order TRT iteration.
+ *
+ * Plain C, no C++ or torch types: every entry point takes a context handle,
+ * plain pointers and sizes, and returns 0 on success or a negative status;
+ * trt_last_error() then gives the message.  State lives on the GPU for the
+ * whole run; host arrays cross the boundary only through trt_push/trt_pull
+ * (initialisation, restart, outputs), with the reference's own array names
+ * and Fortran (column-major) shapes, e.g. I_crn(2N_x,2N_y,N_m,N_g).
+ *
+ * Each routine replaces the reference Fortran subroutine cited beside it
+ * (paths relative to the reference repository).  The Fortran side binds them
+ * with ISO_C_BINDING (see the fortran/ directory of the package); the
+ * reference's Open_Threads arguments have no counterpart (one context = one
+ * HIP stream).
+ */
+#ifndef TRT_CAPI_H
+#define TRT_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct trt_ctx trt_ctx;
+
+/* Problem description: the values MainSrc/INPUTS.f08 parses from input.inp
+ * (INPUT :10-163, INPUT_SOLVER_OPTS :307-609, INPUT_PARAMETERS :870-1141,
+ * INPUT_ENERGY :1517-1573, INPUT_QUAD :1579-1671, INPUT_TEST_TYPE :1676-1729),
+ * in the reference's code units (eV, sh, cm, erg/comp_unit). */
+typedef struct {
+    int nx, ny, nm, ng, nmat;
+    const double *dx, *dy;          /* Delx(N_x), Dely(N_y)                         */
+    const double *ox, *oy, *w;      /* Omega_x, Omega_y, quad_weight(N_m), quadrant blocks q1..q4 */
+    const double *nu;               /* nu_g(N_g+1) group edges                      */
+    const int *mat;                 /* Mat(N_x,N_y), 0-based material ids           */
+    const double *kmult;            /* Kappa_Mult(nmat)                             */
+    double c, dt, theta, comp_unit, cv, kap0, tini;
+    int bc_type[4];                 /* left, bottom, right, top; only 0 (incoming Planckian) */
+    double bcT[4];
+    int opacity_model;              /* 0 = kapB_calc group Planck mean, 1 = group-midpoint formula */
+    int maxit_rte, maxit_mloqd, maxit_gloqd, conv_type;
+    double conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src;
+    int use_line_search, use_safety_search, kapE_dT_flag;
+    double E_bound_low, T_bound_low;
+    int threads;                    /* accepted and ignored (reference Open_Threads) */
+    /* extensions of this implementation */
+    int precision;                  /* 0 = FP64 sweep, 1 = FP32 sweep with FP64 moment accumulation */
+    int tile_x, tile_y;             /* sweep tile shape; 0 = default                 */
+    double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
+    int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default */
+} trt_problem;
+
+/* Context life cycle.  trt_ctx_create allocates the device-resident state and
+ * performs MISC_INIT, RT_INIT, MGQD_INIT, GQD_INIT and TEMP_INIT
+ * (MainSrc/ALGORITHMS.f08:203-218, MainSrc/INITIALIZERS.f08:15-427). */
+int trt_ctx_create(trt_ctx **ctx, const trt_problem *p);
+int trt_ctx_destroy(trt_ctx *ctx);
+const char *trt_last_error(const trt_ctx *ctx);
+/* device bytes held by a context for problem p (without creating it) */
+size_t trt_ctx_bytes(const trt_problem *p);
+
+/* host <-> device copies by reference array name, Fortran shapes.  n is the
+ * number of doubles the caller's buffer holds (must equal the array size).
+ * trt_array_size returns that size, or 0 for an unknown name. */
+size_t trt_array_size(const trt_ctx *ctx, const char *name);
+int trt_push(trt_ctx *ctx, const char *name, const double *host, size_t n);
+int trt_pull(trt_ctx *ctx, const char *name, double *host, size_t n);
+
+/* MATERIAL_UPDATE, MainSrc/UPDATES.f08:110-156 (kapB_calc/Bg_planck_calc/sigint,
+ * MainSrc/TEMP_FUNCTIONS.f08:11-141).  RT_Src is kept per (cell, group) since the
+ * reference writes the same value to all corners and angles (:141-150). */
+int trt_material_update(trt_ctx *ctx);
+/* TRANSPORT_SCB, MainSrc/TRANSPORT_SOLVES.f08:14-327, with the angular moments
+ * of COLLAPSE_INTENSITIES (:381-511) accumulated in the same pass. */
+int trt_transport_scb(trt_ctx *ctx);
+/* COLLAPSE_INTENSITIES, MainSrc/TRANSPORT_SOLVES.f08:381-511 (reduces the partial
+ * moments written by trt_transport_scb). */
+int trt_collapse_intensities(trt_ctx *ctx);
+/* fg_Calc MainSrc/MLOQD_SOLVES.f08:619-678; Cg_Calc :367-488; MGQD_In_Calc :493-614 */
+int trt_fg_calc(trt_ctx *ctx);
+int trt_cg_calc(trt_ctx *ctx);
+int trt_mgqd_in_calc(trt_ctx *ctx);
+/* OLD_MGQD_COEFS MainSrc/MLOQD_SOLVES.f08:12-73; OLD_GREY_COEFS MainSrc/GLOQD_SOLVES.f08:12-42 */
+int trt_old_mgqd_coefs(trt_ctx *ctx);
+int trt_old_grey_coefs(trt_ctx *ctx);
+/* MLOQD_FV, MainSrc/MLOQD_SOLVES.f08:78-362, solving the N_g reduced systems of
+ * QD_FV (MainSrc/QD_SOLVERS.f08:40-538) as one batch.  kits (may be NULL)
+ * receives N_g Krylov iteration counts. */
+int trt_mloqd_fv(trt_ctx *ctx, int *kits);
+/* COLLAPSE_MG_EF MainSrc/MLOQD_SOLVES.f08:687-730; Cbar_Calc MainSrc/GLOQD_SOLVES.f08:177-231;
+ * GREY_COEFS :47-146; GQD_In_Calc :263-290 */
+int trt_collapse_mg_ef(trt_ctx *ctx);
+int trt_cbar_calc(trt_ctx *ctx);
+int trt_grey_coefs(trt_ctx *ctx);
+int trt_gqd_in_calc(trt_ctx *ctx);
+/* EGP_FV_NEWT, MainSrc/GLOQD_SOLVES.f08:305-860; its receives the Newton count. */
+int trt_egp_fv_newt(trt_ctx *ctx, int mgqd_it, int *its);
+/* CONVERGENCE, MainSrc/CONVERGENCE_CHECKS.f08:7-44, on device arrays x, x_old, x_old2
+ * named as in trt_pull. */
+int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *x_old2,
+                    int conv_type, double eps, int its, double *norm, double *rho, int *converged);
+/* One time step of TRT_MLQD_ALGORITHM (run_type mlqd, use_grey = 1),
+ * MainSrc/ALGORITHMS.f08:389-897.  its_out (may be NULL) receives
+ * {RT iterations, MLOQD iterations, Newton iterations, sweeps}. */
+int trt_step(trt_ctx *ctx, int its_out[4]);
+double trt_time(const trt_ctx *ctx);
+
+/* Sweep timing: runs nsweeps sweeps on the context's stream and returns the
+ * total elapsed milliseconds measured with HIP events on that stream. */
+int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms);
+/* Accumulated sweep statistics since creation: sweeps run and total sweep ms. */
+int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

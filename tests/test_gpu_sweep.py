@@ -1,0 +1,100 @@
+"""GPU parity of the sweep (TRANSPORT_SCB), the moments (COLLAPSE_INTENSITIES),
+fg_Calc, Cg_Calc and MATERIAL_UPDATE against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (nx, ny, S_N, ng, tile)
+    (12, 10, 4, 3, (8, 8)),
+    (13, 11, 8, 5, (4, 4)),
+    (9, 17, 12, 2, (16, 8)),
+    (20, 20, 16, 3, (8, 4)),
+]
+
+
+def _randomized_pair(trt, oracle_mod, nx, ny, order, ng, tile, precision=0, seed=1):
+    prob = trt.problem.small(nx=nx, ny=ny, order=order, ng=ng, seed=seed)
+    o = oracle_mod.Oracle(prob, threads=4)
+    ctx = trt.native.Context(prob, precision=precision, tile=tile)
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(0.5, 800.0, size=(nx, ny))
+    o["Temp"][...] = T
+    ctx.push("Temp", T)
+    o.call("material_update")
+    ctx.call("material_update")
+    bg = o["Bg"]
+    Iold = np.asfortranarray(rng.uniform(0.2, 2.0, size=(2 * nx, 2 * ny, prob.nm, ng)))
+    Iold *= np.repeat(np.repeat(bg, 2, axis=0), 2, axis=1)[:, :, None, :]
+    o["I_crn_old"][...] = Iold
+    ctx.push("I_crn_old", Iold)
+    return prob, o, ctx
+
+
+@pytest.mark.parametrize("nx,ny,order,ng,tile", CASES)
+def test_sweep_fp64_matches_oracle(trt, oracle_mod, nx, ny, order, ng, tile):
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, nx, ny, order, ng, tile)
+    for name in ("Bg", "KapB", "KapE", "MGQD_Src", "RT_Src"):
+        assert rel_err(ctx.pull(name), o[name].ravel(order="F")) < 1e-13, name
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    Ig = ctx.pull("I_crn", o["I_crn"].shape)
+    assert rel_err(Ig, o["I_crn"]) < 1e-12
+    o.call("collapse_intensities")
+    ctx.call("collapse_intensities")
+    for name in ("Hg_avg_xx", "Hg_avg_yy", "Hg_avg_xy", "HO_Eg_avg", "Hg_edgV_xx", "Hg_edgV_xy", "HO_Eg_edgV",
+                 "HO_Fxg_edgV", "Hg_edgH_yy", "Hg_edgH_xy", "HO_Eg_edgH", "HO_Fyg_edgH", "HO_E_avg", "HO_E_edgV",
+                 "HO_E_edgH", "HO_Fx_edgV", "HO_Fy_edgH"):
+        assert rel_err(ctx.pull(name), o[name].ravel(order="F")) < 1e-12, name
+    o.call("fg_calc")
+    ctx.call("fg_calc")
+    o.call("cg_calc")
+    ctx.call("cg_calc")
+    for name in ("fg_avg_xx", "fg_avg_yy", "fg_avg_xy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy",
+                 "Cg_L", "Cg_B", "Cg_R", "Cg_T"):
+        g = ctx.pull(name)
+        r = o[name].ravel(order="F")
+        assert np.max(np.abs(g - r)) < 1e-12 * max(1.0, np.max(np.abs(r))), name
+    ctx.close()
+
+
+def test_two_sweeps_same_step_reuse_old(trt, oracle_mod):
+    """Two sweeps in one step both read I_crn_old (ALGORITHMS.f08:541 inside the RT loop)."""
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 10, 12, 8, 2, (4, 8))
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    assert rel_err(ctx.pull("I_crn", o["I_crn"].shape), o["I_crn"]) < 1e-12
+    assert rel_err(ctx.pull("I_crn_old", o["I_crn_old"].shape), o["I_crn_old"]) == 0.0
+    ctx.close()
+
+
+@pytest.mark.parametrize("nx,ny,order,ng,tile", CASES[:2])
+def test_sweep_fp32_within_tolerance(trt, oracle_mod, nx, ny, order, ng, tile):
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, nx, ny, order, ng, tile, precision=1)
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    assert rel_err(ctx.pull("I_crn", o["I_crn"].shape), o["I_crn"]) < 2e-6
+    o.call("collapse_intensities")
+    ctx.call("collapse_intensities")
+    o.call("fg_calc")
+    ctx.call("fg_calc")
+    for name in ("HO_Eg_avg", "fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgH_yy"):
+        assert rel_err(ctx.pull(name), o[name].ravel(order="F")) < 2e-6, name
+    ctx.close()
+
+
+def test_equilibrium_sweep_is_exact(trt):
+    """T = Tini everywhere with Planckian boundaries at Tini: the sweep returns B_g in every corner."""
+    prob = trt.problem.small(nx=8, ny=8, order=8, ng=3)
+    prob.bcT = (prob.tini,) * 4
+    ctx = trt.native.Context(prob)
+    ctx.call("transport_scb")
+    I = ctx.pull("I_crn", (16, 16, prob.nm, prob.ng))
+    bg = ctx.pull("Bg", (8, 8, prob.ng))[0, 0, :]
+    assert np.max(np.abs(I / bg[None, None, None, :] - 1.0)) < 1e-13
+    ctx.close()
