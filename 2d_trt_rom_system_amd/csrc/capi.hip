@@ -227,6 +227,46 @@ int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms)
     });
 }
 
+int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *kernel_ms, int *launches)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (nsweeps < 1) throw Error("nsweeps must be positive");
+        const int nl = trt::sweep_launches(c);
+        std::vector<hipEvent_t> ev(2 * (size_t)nl);
+        for (auto &e : ev) TRT_CHECK(hipEventCreate(&e));
+        double kern = 0.0, wall = 0.0;
+        for (int s = 0; s < nsweeps; ++s) {
+            trt::transport_sweep(c, &ev);
+            TRT_CHECK(hipEventSynchronize(ev.back()));
+            for (int d = 0; d < nl; ++d) {
+                float t = 0.f;
+                TRT_CHECK(hipEventElapsedTime(&t, ev[2 * d], ev[2 * d + 1]));
+                kern += t;
+            }
+            float w = 0.f;
+            TRT_CHECK(hipEventElapsedTime(&w, ev.front(), ev.back()));
+            wall += w;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        c.sweeps += nsweeps;
+        c.sweep_ms += wall;
+        if (wall_ms) *wall_ms = wall;
+        if (kernel_ms) *kernel_ms = kern;
+        if (launches) *launches = nl * nsweeps;
+    });
+}
+
+int trt_calib_copy(size_t nbytes, int reps, double *ms)
+{
+    try {
+        trt::calib_copy(nbytes, reps, ms);
+        return 0;
+    } catch (const Error &e) {
+        g_create_error = e.msg;
+    }
+    return -2;
+}
+
 int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms)
 {
     if (!ctx) return -1;

@@ -333,7 +333,7 @@ static void launch_sweep(Ctx &c, SweepArgs &a, int nblocks)
     }
 }
 
-void transport_sweep(Ctx &c)
+void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
 {
     const SweepGeom &sg = c.sg;
     // the new intensities go to the buffer that does not hold I_crn_old
@@ -356,8 +356,10 @@ void transport_sweep(Ctx &c)
         const int tx_hi = std::min(d, sg.NTX - 1);
         a.nt_d = tx_hi - a.tx_lo + 1;
         const int nblocks = 4 * a.nt_d * sg.nblk_g;
+        if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d], c.stream));
         if (c.precision) launch_sweep<float>(c, a, nblocks);
         else launch_sweep<double>(c, a, nblocks);
+        if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d + 1], c.stream));
     }
     TRT_CHECK(hipGetLastError());
     c.icur = out;
@@ -433,6 +435,40 @@ void push_icrn(Ctx &c, int which, const double *host)
     TRT_CHECK(hipStreamSynchronize(c.stream));
     TRT_CHECK(hipFree(tmp));
     c.owned.pop_back();
+}
+
+int sweep_launches(const Ctx &c) { return c.sg.NTX + c.sg.NTY - 1; }
+
+// plain device copy of n doubles, the calibration kernel for HBM byte counters
+__global__ void calib_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void calib_copy(size_t nbytes, int reps, double *ms)
+{
+    const size_t n2 = nbytes / 16;
+    void *a = nullptr, *b = nullptr;
+    TRT_CHECK(hipMalloc(&a, n2 * 16));
+    TRT_CHECK(hipMalloc(&b, n2 * 16));
+    TRT_CHECK(hipMemset(a, 0, n2 * 16));
+    hipEvent_t e0, e1;
+    TRT_CHECK(hipEventCreate(&e0));
+    TRT_CHECK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(calib_copy_kernel, dim3(4096), dim3(256), 0, nullptr, (const double2 *)a, (double2 *)b, n2);
+    TRT_CHECK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(calib_copy_kernel, dim3(4096), dim3(256), 0, nullptr, (const double2 *)a, (double2 *)b, n2);
+    TRT_CHECK(hipEventRecord(e1, nullptr));
+    TRT_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    TRT_CHECK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
 }
 
 }  // namespace trt

@@ -110,7 +110,9 @@ struct Ctx {
 // --- launchers (each enqueues on ctx.stream) ---
 void init_state(Ctx &c);
 void material_update(Ctx &c);
-void transport_sweep(Ctx &c);
+void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
+int sweep_launches(const Ctx &c);
+void calib_copy(size_t nbytes, int reps, double *ms);
 void collapse_intensities(Ctx &c);
 void fg_calc(Ctx &c);
 void cg_calc(Ctx &c);

@@ -42,7 +42,7 @@ EXPORTS = [
     "trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
     "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_mloqd_fv", "trt_collapse_mg_ef",
     "trt_cbar_calc", "trt_grey_coefs", "trt_gqd_in_calc", "trt_egp_fv_newt", "trt_convergence", "trt_step",
-    "trt_time", "trt_sweep_timed", "trt_sweep_stats",
+    "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
 ]
 
 _lib = None
@@ -85,6 +85,8 @@ def load(path: str = LIBPATH):
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
     L.trt_sweep_stats.argtypes = [vp, C.POINTER(C.c_longlong), _dp]
+    L.trt_sweep_kernel_timed.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+    L.trt_calib_copy.argtypes = [C.c_size_t, C.c_int, _dp]
     _lib = L
     return L
 
@@ -123,6 +125,15 @@ def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float =
     s.tile_x, s.tile_y = tile
     s.lin_rtol, s.lin_maxit = lin_rtol, lin_maxit
     return s, keep
+
+
+def calib_copy(nbytes: int, reps: int = 10) -> float:
+    """Mean ms of a plain device copy kernel moving nbytes (read) + nbytes (write)."""
+    L = load()
+    ms = C.c_double()
+    if L.trt_calib_copy(int(nbytes), int(reps), C.byref(ms)) != 0:
+        raise TrtError("trt_calib_copy failed: " + L.trt_last_error(None).decode())
+    return ms.value
 
 
 def device_bytes(prob, precision: int = 0, tile=(0, 0)) -> int:
@@ -212,6 +223,13 @@ class Context:
         ms = C.c_double()
         self._check(self.L.trt_sweep_timed(self.h, int(nsweeps), C.byref(ms)), "sweep_timed")
         return ms.value
+
+    def sweep_kernel_timed(self, nsweeps: int):
+        """(wall ms, summed kernel ms, launches) over nsweeps sweeps."""
+        wall, kern, nl = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.L.trt_sweep_kernel_timed(self.h, int(nsweeps), C.byref(wall), C.byref(kern), C.byref(nl)),
+                    "sweep_kernel_timed")
+        return wall.value, kern.value, nl.value
 
     def sweep_stats(self):
         n, ms = C.c_longlong(), C.c_double()

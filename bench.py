@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Sweep throughput benchmark (one GPU).
+
+A step is one TRANSPORT_SCB sweep with the fused angular moments over the
+256x256-cell, S16 (144 directions), 32-group problem of the plain-GPU config,
+inputs resident in HBM.  The headline value is the FP32-sweep form (FP64
+moment accumulation); the plain FP64 form is measured in the same run and
+reported beside it.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, AMD public specification
+
+
+def alg_bytes(prob, precision: int) -> int:
+    """Algorithmic bytes of one sweep: read 4 old corners + write 4 new corners per
+    (cell, direction, group), read KapE+1/(c dt) and the isotropic source per (cell, group),
+    write 12 FP64 moments per (cell, group)."""
+    s_i = 4 if precision else 8
+    nc = prob.nx * prob.ny
+    return nc * prob.nm * prob.ng * 8 * s_i + nc * prob.ng * (2 * 8 + 12 * 8)
+
+
+def measure(trt, prob, precision: int, steps: int, warmup: int):
+    ctx = trt.native.Context(prob, precision=precision)
+    try:
+        if warmup > 0:
+            ctx.sweep_timed(warmup)
+        t0 = time.perf_counter()
+        ms = ctx.sweep_timed(steps)                 # HIP events on the context stream
+        host_s = time.perf_counter() - t0
+        wall, kern, nl = ctx.sweep_kernel_timed(max(2, steps // 2))
+        nsw = max(2, steps // 2)
+    finally:
+        ctx.close()
+    upd = prob.updates_per_sweep()
+    b = alg_bytes(prob, precision)
+    return {
+        "ms_per_step": ms / steps,
+        "updates_per_s": upd * steps / (ms * 1e-3),
+        "host_s": host_s,
+        "kernel_ms_per_sweep": kern / nsw,
+        "launch_avg_ms": kern / nl,
+        "launches_per_sweep": nl // nsw,
+        "alg_bytes_per_sweep": b,
+        "achieved_gbs": b / (kern / nsw * 1e-3) / 1e9,
+    }
+
+
+def traffic_from_profiles(precision: int):
+    """HBM bytes per sweep-kernel launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "sweep_cfg1_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    key = "fp32" if precision else "fp64"
+    return d.get(key, {}).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(trt, budget_s: float = 12.0):
+    """The FP64 CPU restatement (oracle) of TRANSPORT_SCB + COLLAPSE_INTENSITIES on the same
+    256x256 S16 grid, OpenMP over groups as the reference, one group per thread."""
+    import oracle
+    oracle.build()
+    cores = os.cpu_count() or 1
+    threads = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 32))
+    prob = trt.problem.cfg1()
+    ng = threads
+    prob = prob.with_(ng=ng, nu=prob.nu[:ng + 1])
+    o = oracle.Oracle(prob, threads=threads)
+    n = 0
+    t_total = 0.0
+    while n < 5 and (n == 0 or t_total < budget_s):
+        t0 = time.perf_counter()
+        o.call("transport_scb")
+        o.call("collapse_intensities")
+        t_total += time.perf_counter() - t0
+        n += 1
+    del o
+    upd = prob.updates_per_sweep() * n
+    return {
+        "value": upd / t_total,
+        "unit": "cell*angle*group updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{n} x (TRANSPORT_SCB + COLLAPSE_INTENSITIES) at 256x256 cells, S16, {ng} of the 32 groups "
+                   f"(one group per OpenMP thread as in the reference), FP64, gcc -O2 -fopenmp; {t_total:.1f} s"),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
+                    help="measure one form only (profiling runs)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
+              "see the sharding note in DESIGN.md)", file=sys.stderr)
+        return 2
+    trt = importlib.import_module("2d_trt_rom_system_amd")
+    prob = trt.problem.cfg1()
+    res = {}
+    for prec, key in ((1, "fp32"), (0, "fp64")):
+        if args.only and args.only != key:
+            continue
+        print(f"[bench] {key} sweep, {args.warmup} warmup + {args.steps} timed", file=sys.stderr, flush=True)
+        res[key] = measure(trt, prob, prec, args.steps, args.warmup)
+    head_key = "fp32" if "fp32" in res else "fp64"
+    head = res[head_key]
+    prec = 1 if head_key == "fp32" else 0
+    out = {
+        "metric": "sweep cell*angle*group updates/s (FP32 sweep with FP64 moments; plain FP64 beside it)",
+        "value": head["updates_per_s"],
+        "unit": "cell*angle*group updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (FP64 moment accumulation)" if head_key == "fp32" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "TRANSPORT_SCB sweep + fused angular moments; 256x256 cells, S16 (144 directions), "
+                        "32 log-spaced groups 1e-3..1e2 keV, 8x8 lognormal(0,1.5) opacity map",
+            "cells": prob.nx * prob.ny, "directions": prob.nm, "groups": prob.ng,
+            "updates_per_step": prob.updates_per_sweep(),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(prec),
+            "kernel": "sweep_kernel",
+            "alg_bytes_per_launch": head["alg_bytes_per_sweep"] / head["launches_per_sweep"],
+            "avg_launch_ms": head["launch_avg_ms"],
+        },
+    }
+    for k, v in res.items():
+        out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+    if "fp32" in res and "fp64" in res:
+        out["fp32_over_fp64"] = res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(trt)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

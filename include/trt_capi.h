@@ -110,6 +110,13 @@ double trt_time(const trt_ctx *ctx);
 /* Sweep timing: runs nsweeps sweeps on the context's stream and returns the
  * total elapsed milliseconds measured with HIP events on that stream. */
 int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms);
+/* Sweep timing per kernel launch: an event pair around each of the sweep's
+ * tile-diagonal launches.  wall_ms spans the whole sweeps, kernel_ms sums the
+ * launches, launches counts them. */
+int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *kernel_ms, int *launches);
+/* Device-to-device copy of nbytes by a plain kernel, reps times; *ms = mean
+ * duration.  The calibration reference for HBM byte counters. */
+int trt_calib_copy(size_t nbytes, int reps, double *ms);
 /* Accumulated sweep statistics since creation: sweeps run and total sweep ms. */
 int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
 
