@@ -43,7 +43,30 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_fortran(objdir, verbose)
     return LIB
+
+
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+FSOURCES = ["trt_capi.f90", "trt_dropin.f90", "trt_mlqd_main.f90"]
+FEXE = os.path.join(HERE, "trt_mlqd")
+
+
+def build_fortran(objdir: str, verbose: bool = False) -> str:
+    """The ISO_C_BINDING drop-in modules and the Fortran driver, linked against libtrt_hip.so."""
+    fdir = os.path.join(HERE, "fortran")
+    objs = []
+    for s in FSOURCES:
+        obj = os.path.join(objdir, s.replace(".f90", ".o"))
+        cmd = [FC, "-O2", "-fPIC", "-module-dir", objdir, "-c", os.path.join(fdir, s), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    tmp = FEXE + ".tmp"
+    subprocess.check_call([FC, "-o", tmp, *objs, "-L" + HERE, "-ltrt_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FEXE)
+    return FEXE
 
 
 if __name__ == "__main__":

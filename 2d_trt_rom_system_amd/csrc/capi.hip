@@ -114,6 +114,7 @@ size_t trt_array_size(const trt_ctx *ctx, const char *name)
     if (!ctx || !name) return 0;
     const Ctx &c = ctx->c;
     if (!strcmp(name, "I_crn") || !strcmp(name, "I_crn_old")) return (size_t)4 * c.nx * c.ny * c.nm * c.ng;
+    if (const size_t d = trt::derived_size(c, name)) return d;
     auto it = c.arrays.find(name);
     return it == c.arrays.end() ? 0 : it->second.size();
 }
@@ -130,7 +131,10 @@ int trt_push(trt_ctx *ctx, const char *name, const double *host, size_t n)
             trt::push_icrn(c, c.iold, host);
             return;
         }
+        if (trt::derived_size(c, name)) throw Error(std::string(name) + " is derived from I_crn and cannot be pushed");
         TRT_CHECK(hipMemcpyAsync(c.arr(name), host, n * 8, hipMemcpyHostToDevice, c.stream));
+        // the sweep reads KapE and RT_Src through its per-(cell, group) pair array
+        if (!strcmp(name, "KapE") || !strcmp(name, "RT_Src")) trt::rebuild_sc(c);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -142,6 +146,7 @@ int trt_pull(trt_ctx *ctx, const char *name, double *host, size_t n)
         if (n != trt_array_size(ctx, name)) throw Error(std::string("size mismatch for ") + name);
         if (!strcmp(name, "I_crn")) { trt::pull_icrn(c, c.icur, host); return; }
         if (!strcmp(name, "I_crn_old")) { trt::pull_icrn(c, c.iold, host); return; }
+        if (trt::derived_size(c, name)) { trt::pull_derived(c, name, host); return; }
         TRT_CHECK(hipMemcpyAsync(host, c.arr(name), n * 8, hipMemcpyDeviceToHost, c.stream));
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });

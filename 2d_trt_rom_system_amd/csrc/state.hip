@@ -204,7 +204,7 @@ void init_state(Ctx &c)
                           "Rhat_old", "DC_xx", "DL_xx", "DR_xx", "DC_yy", "DB_yy", "DT_yy", "DL_xy", "DR_xy", "DB_xy",
                           "DT_xy", "PL", "PR", "PB", "PT", "Temp_RTold", "Temp_RTold2", "Temp_MGQDold",
                           "Temp_MGQDold2", "HO_E_avg_RTold", "HO_E_avg_RTold2", "MGQD_E_avg_MGQDold",
-                          "MGQD_E_avg_MGQDold2", "A"};
+                          "MGQD_E_avg_MGQDold2", "A", "_conv_x", "_conv_x_old", "_conv_x_old2"};
     for (auto n : grey) c.reg(n, nx, ny);
     for (auto n : {"HO_E_edgV", "HO_Fx_edgV", "MGQD_E_edgV", "MGQD_Fx_edgV", "E_edgV", "Fx_edgV"}) c.reg(n, nx + 1, ny);
     for (auto n : {"HO_E_edgH", "HO_Fy_edgH", "MGQD_E_edgH", "MGQD_Fy_edgH", "E_edgH", "Fy_edgH"}) c.reg(n, nx, ny + 1);

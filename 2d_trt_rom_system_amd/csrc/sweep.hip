@@ -26,6 +26,8 @@
 //    variant solves in FP32 without loss beyond rounding.
 #include "trt_internal.h"
 
+#include <cstring>
+
 namespace trt {
 
 struct SweepArgs {
@@ -435,6 +437,114 @@ void push_icrn(Ctx &c, int which, const double *host)
     TRT_CHECK(hipStreamSynchronize(c.stream));
     TRT_CHECK(hipFree(tmp));
     c.owned.pop_back();
+}
+
+
+// ---------------------------------------------------------------------------
+// Reference-shaped views derived from I_crn (pull only): I_avg(N_x,N_y,N_m,N_g),
+// I_edgV(N_x+1,N_y,..), I_edgH(N_x,N_y+1,..), Ic_edgV(N_x+1,2N_y,..), Ic_edgH(2N_x,N_y+1,..).
+// Interior and outgoing faces hold the upwind corner values (TRANSPORT_SOLVES.f08:86-93),
+// incoming boundary faces the boundary intensity (RT_BC_UPDATE, UPDATES.f08:26-102).
+// ---------------------------------------------------------------------------
+template <typename TI>
+__global__ void derived_kernel(const TI *I, const double *bcin, double *out, int kind, int nx, int ny, int nm, int ng)
+{
+    const int K = nm / 4;
+    const size_t NL = (size_t)ng * K;
+    int d0, d1;
+    switch (kind) {
+    case 0: d0 = nx; d1 = ny; break;
+    case 1: d0 = nx + 1; d1 = ny; break;
+    case 2: d0 = nx; d1 = ny + 1; break;
+    case 3: d0 = nx + 1; d1 = 2 * ny; break;
+    default: d0 = 2 * nx; d1 = ny + 1; break;
+    }
+    const size_t n = (size_t)d0 * d1 * nm * ng;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+        size_t r = idx;
+        const int a = r % d0; r /= d0;
+        const int b = r % d1; r /= d1;
+        const int m = r % nm;
+        const int g = (int)(r / nm);
+        const int q = m / K, k = m % K;
+        const bool px = (q == 0 || q == 3), py = (q < 2);
+        auto rec = [&](int i, int j) { return I + (((size_t)i + (size_t)nx * j) * 4 + q) * NL * 4 + ((size_t)g * K + k) * 4; };
+        double v = 0.0;
+        if (kind == 0) {
+            const TI *p = rec(a, b);
+            v = ((double)p[0] + (double)p[1] + (double)p[2] + (double)p[3]) * 0.25;
+        } else if (kind == 1 || kind == 3) {          // vertical faces
+            const int f = a;
+            const int j = kind == 1 ? b : b / 2, top = kind == 3 ? (b & 1) : -1;
+            if (px && f == 0) v = bcin[0 * ng + g];
+            else if (!px && f == nx) v = bcin[2 * ng + g];
+            else {
+                const TI *p = px ? rec(f - 1, j) : rec(f, j);
+                const double lo = px ? (double)p[1] : (double)p[0], hi = px ? (double)p[2] : (double)p[3];
+                v = top < 0 ? (lo + hi) * 0.5 : (top ? hi : lo);
+            }
+        } else {                                      // horizontal faces
+            const int f = b;
+            const int i = kind == 2 ? a : a / 2, right = kind == 4 ? (a & 1) : -1;
+            if (py && f == 0) v = bcin[1 * ng + g];
+            else if (!py && f == ny) v = bcin[3 * ng + g];
+            else {
+                const TI *p = py ? rec(i, f - 1) : rec(i, f);
+                const double lft = py ? (double)p[3] : (double)p[0], rgt = py ? (double)p[2] : (double)p[1];
+                v = right < 0 ? (lft + rgt) * 0.5 : (right ? rgt : lft);
+            }
+        }
+        out[idx] = v;
+    }
+}
+
+size_t derived_size(const Ctx &c, const char *name)
+{
+    const size_t mg = (size_t)c.nm * c.ng;
+    if (!strcmp(name, "I_avg")) return (size_t)c.nx * c.ny * mg;
+    if (!strcmp(name, "I_edgV")) return (size_t)(c.nx + 1) * c.ny * mg;
+    if (!strcmp(name, "I_edgH")) return (size_t)c.nx * (c.ny + 1) * mg;
+    if (!strcmp(name, "Ic_edgV")) return (size_t)(c.nx + 1) * 2 * c.ny * mg;
+    if (!strcmp(name, "Ic_edgH")) return (size_t)2 * c.nx * (c.ny + 1) * mg;
+    return 0;
+}
+
+void pull_derived(Ctx &c, const char *name, double *host)
+{
+    int kind = -1;
+    const char *names[5] = {"I_avg", "I_edgV", "I_edgH", "Ic_edgV", "Ic_edgH"};
+    for (int k = 0; k < 5; ++k) if (!strcmp(name, names[k])) kind = k;
+    if (kind < 0) throw Error(std::string("not a derived intensity view: ") + name);
+    const size_t n = derived_size(c, name);
+    double *tmp = static_cast<double *>(c.dalloc(n * 8));
+    if (c.precision)
+        hipLaunchKernelGGL(derived_kernel<float>, dim3(1024), dim3(256), 0, c.stream,
+                           static_cast<const float *>(c.Icrn[c.icur]), c.bcin, tmp, kind, c.nx, c.ny, c.nm, c.ng);
+    else
+        hipLaunchKernelGGL(derived_kernel<double>, dim3(1024), dim3(256), 0, c.stream,
+                           static_cast<const double *>(c.Icrn[c.icur]), c.bcin, tmp, kind, c.nx, c.ny, c.nm, c.ng);
+    TRT_CHECK(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
+    TRT_CHECK(hipFree(tmp));
+    c.owned.pop_back();
+}
+
+// (KapE + 1/(c dt), RT_Src) per (cell, group) for the sweep, from the named arrays
+__global__ void sc_kernel(const double *KE, const double *RS, double2 *sc, size_t nc, int ng, double inv_cdt)
+{
+    const size_t n = nc * ng;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t cc = idx % nc;
+        const int g = (int)(idx / nc);
+        sc[cc * ng + g] = make_double2(KE[idx] + inv_cdt, RS[idx]);
+    }
+}
+
+void rebuild_sc(Ctx &c)
+{
+    hipLaunchKernelGGL(sc_kernel, dim3(1024), dim3(256), 0, c.stream, c.arr("KapE"), c.arr("RT_Src"),
+                       reinterpret_cast<double2 *>(c.sc), (size_t)c.nx * c.ny, c.ng, 1.0 / (c.p.c * c.p.dt));
+    TRT_CHECK(hipGetLastError());
 }
 
 int sweep_launches(const Ctx &c) { return c.sg.NTX + c.sg.NTY - 1; }

@@ -112,6 +112,9 @@ void init_state(Ctx &c);
 void material_update(Ctx &c);
 void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
 int sweep_launches(const Ctx &c);
+size_t derived_size(const Ctx &c, const char *name);
+void pull_derived(Ctx &c, const char *name, double *host);
+void rebuild_sc(Ctx &c);
 void calib_copy(size_t nbytes, int reps, double *ms);
 void collapse_intensities(Ctx &c);
 void fg_calc(Ctx &c);

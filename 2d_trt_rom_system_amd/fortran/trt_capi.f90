@@ -1,0 +1,249 @@
+!==================================================================================================================================!
+! TRT_CAPI: ISO_C_BINDING interfaces to the C ABI of include/trt_capi.h, plus the context the drop-in
+! modules share.  Errors follow the reference convention: a failing call STOPs with the library's message.
+!==================================================================================================================================!
+MODULE TRT_CAPI
+
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+
+  TYPE, BIND(C) :: trt_problem
+    INTEGER(C_INT) :: nx, ny, nm, ng, nmat
+    TYPE(C_PTR) :: dx, dy, ox, oy, w, nu, mat, kmult
+    REAL(C_DOUBLE) :: c, dt, theta, comp_unit, cv, kap0, tini
+    INTEGER(C_INT) :: bc_type(4)
+    REAL(C_DOUBLE) :: bcT(4)
+    INTEGER(C_INT) :: opacity_model
+    INTEGER(C_INT) :: maxit_rte, maxit_mloqd, maxit_gloqd, conv_type
+    REAL(C_DOUBLE) :: conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src
+    INTEGER(C_INT) :: use_line_search, use_safety_search, kapE_dT_flag
+    REAL(C_DOUBLE) :: E_bound_low, T_bound_low
+    INTEGER(C_INT) :: threads, precision, tile_x, tile_y
+    REAL(C_DOUBLE) :: lin_rtol
+    INTEGER(C_INT) :: lin_maxit
+  END TYPE trt_problem
+
+  ! the context of the running problem, created by TRT_GPU_INIT
+  TYPE(C_PTR), SAVE :: TRT_CTX = C_NULL_PTR
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION trt_ctx_create(ctx, p) BIND(C, name="trt_ctx_create")
+      IMPORT :: C_INT, C_PTR, trt_problem
+      TYPE(C_PTR) :: ctx
+      TYPE(trt_problem) :: p
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_ctx_destroy(ctx) BIND(C, name="trt_ctx_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    TYPE(C_PTR) FUNCTION trt_last_error(ctx) BIND(C, name="trt_last_error")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_SIZE_T) FUNCTION trt_array_size(ctx, name) BIND(C, name="trt_array_size")
+      IMPORT :: C_SIZE_T, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: name(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_push(ctx, name, host, n) BIND(C, name="trt_push")
+      IMPORT :: C_INT, C_PTR, C_CHAR, C_DOUBLE, C_SIZE_T
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: name(*)
+      REAL(C_DOUBLE) :: host(*)
+      INTEGER(C_SIZE_T), VALUE :: n
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_pull(ctx, name, host, n) BIND(C, name="trt_pull")
+      IMPORT :: C_INT, C_PTR, C_CHAR, C_DOUBLE, C_SIZE_T
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: name(*)
+      REAL(C_DOUBLE) :: host(*)
+      INTEGER(C_SIZE_T), VALUE :: n
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_material_update(ctx) BIND(C, name="trt_material_update")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_transport_scb(ctx) BIND(C, name="trt_transport_scb")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_collapse_intensities(ctx) BIND(C, name="trt_collapse_intensities")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_fg_calc(ctx) BIND(C, name="trt_fg_calc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_cg_calc(ctx) BIND(C, name="trt_cg_calc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_mgqd_in_calc(ctx) BIND(C, name="trt_mgqd_in_calc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_old_mgqd_coefs(ctx) BIND(C, name="trt_old_mgqd_coefs")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_old_grey_coefs(ctx) BIND(C, name="trt_old_grey_coefs")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_mloqd_fv(ctx, kits) BIND(C, name="trt_mloqd_fv")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT) :: kits(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_collapse_mg_ef(ctx) BIND(C, name="trt_collapse_mg_ef")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_cbar_calc(ctx) BIND(C, name="trt_cbar_calc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_grey_coefs(ctx) BIND(C, name="trt_grey_coefs")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_gqd_in_calc(ctx) BIND(C, name="trt_gqd_in_calc")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_egp_fv_newt(ctx, mgqd_it, its) BIND(C, name="trt_egp_fv_newt")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: mgqd_it
+      INTEGER(C_INT) :: its
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_convergence(ctx, x, xo, xo2, conv_type, eps, its, norm, rho, conv) &
+        BIND(C, name="trt_convergence")
+      IMPORT :: C_INT, C_PTR, C_CHAR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: x(*), xo(*), xo2(*)
+      INTEGER(C_INT), VALUE :: conv_type, its
+      REAL(C_DOUBLE), VALUE :: eps
+      REAL(C_DOUBLE) :: norm, rho
+      INTEGER(C_INT) :: conv
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_step(ctx, its) BIND(C, name="trt_step")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT) :: its(4)
+    END FUNCTION
+    REAL(C_DOUBLE) FUNCTION trt_time(ctx) BIND(C, name="trt_time")
+      IMPORT :: C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+!----------------------------------------------------------------------------------------------------------------------------------!
+! message of the last failure of the running context (or of trt_ctx_create)
+!----------------------------------------------------------------------------------------------------------------------------------!
+FUNCTION TRT_ERRMSG() RESULT(msg)
+  CHARACTER(LEN=512) :: msg
+  TYPE(C_PTR) :: p
+  CHARACTER(KIND=C_CHAR), POINTER :: s(:)
+  INTEGER :: k
+  msg = ''
+  p = trt_last_error(TRT_CTX)
+  IF (.NOT. C_ASSOCIATED(p)) RETURN
+  CALL C_F_POINTER(p, s, [512])
+  DO k = 1, 512
+    IF (s(k) .EQ. C_NULL_CHAR) EXIT
+    msg(k:k) = s(k)
+  END DO
+END FUNCTION TRT_ERRMSG
+
+SUBROUTINE TRT_CHECK(rc, what)
+  INTEGER(C_INT), INTENT(IN) :: rc
+  CHARACTER(*), INTENT(IN) :: what
+  IF (rc .NE. 0) THEN
+    WRITE(*,*) 'TRT GPU: ', what, ' failed: ', TRIM(TRT_ERRMSG())
+    ERROR STOP 'TRT GPU call failed'
+  END IF
+END SUBROUTINE TRT_CHECK
+
+SUBROUTINE TRT_REQUIRE_CTX(what)
+  CHARACTER(*), INTENT(IN) :: what
+  IF (.NOT. C_ASSOCIATED(TRT_CTX)) THEN
+    WRITE(*,*) 'TRT GPU: ', what, ' called before TRT_GPU_INIT'
+    ERROR STOP 'TRT GPU context missing'
+  END IF
+END SUBROUTINE TRT_REQUIRE_CTX
+
+!----------------------------------------------------------------------------------------------------------------------------------!
+! host <-> device copy of a reference-named array (any rank; sequence association)
+!----------------------------------------------------------------------------------------------------------------------------------!
+SUBROUTINE TRT_PUSH_ARRAY(name, a, n)
+  CHARACTER(*), INTENT(IN) :: name
+  INTEGER, INTENT(IN) :: n
+  REAL(C_DOUBLE), INTENT(IN) :: a(n)
+  CALL TRT_REQUIRE_CTX(name)
+  CALL TRT_CHECK(trt_push(TRT_CTX, TRIM(name)//C_NULL_CHAR, a, INT(n, C_SIZE_T)), 'push '//name)
+END SUBROUTINE TRT_PUSH_ARRAY
+
+SUBROUTINE TRT_PULL_ARRAY(name, a, n)
+  CHARACTER(*), INTENT(IN) :: name
+  INTEGER, INTENT(IN) :: n
+  REAL(C_DOUBLE), INTENT(INOUT) :: a(n)
+  CALL TRT_REQUIRE_CTX(name)
+  CALL TRT_CHECK(trt_pull(TRT_CTX, TRIM(name)//C_NULL_CHAR, a, INT(n, C_SIZE_T)), 'pull '//name)
+END SUBROUTINE TRT_PULL_ARRAY
+
+!----------------------------------------------------------------------------------------------------------------------------------!
+! TRT_GPU_INIT: creates the device-resident problem from the values the reference reads in INPUT
+! (MainSrc/INPUTS.f08) and initialises it as MISC_INIT/RT_INIT/MGQD_INIT/GQD_INIT/TEMP_INIT do
+! (MainSrc/ALGORITHMS.f08:203-218).  Mat holds 1-based material ids as in the reference.
+!----------------------------------------------------------------------------------------------------------------------------------!
+SUBROUTINE TRT_GPU_INIT(Omega_x, Omega_y, quad_weight, Nu_g, Delx, Dely, Mat, Kappa_Mult, c, Delt, Theta, Comp_Unit, &
+  Cv, Kap0, Tini, BC_Type, bcT_left, bcT_bottom, bcT_right, bcT_top, opacity_model, maxit_RTE, maxit_MLOQD, maxit_GLOQD, &
+  conv_type, conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src, Use_Line_Search, Use_Safety_Search, kapE_dT_flag, &
+  E_Bound_Low, T_Bound_Low, precision)
+
+  REAL(C_DOUBLE), TARGET, INTENT(IN) :: Omega_x(:), Omega_y(:), quad_weight(:), Nu_g(:), Delx(:), Dely(:), Kappa_Mult(:)
+  INTEGER, INTENT(IN) :: Mat(:,:), BC_Type(4), opacity_model, maxit_RTE, maxit_MLOQD, maxit_GLOQD, conv_type, precision
+  REAL(C_DOUBLE), INTENT(IN) :: c, Delt, Theta, Comp_Unit, Cv, Kap0, Tini, bcT_left, bcT_bottom, bcT_right, bcT_top
+  REAL(C_DOUBLE), INTENT(IN) :: conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src, E_Bound_Low, T_Bound_Low
+  LOGICAL, INTENT(IN) :: Use_Line_Search, Use_Safety_Search, kapE_dT_flag
+
+  TYPE(trt_problem) :: p
+  INTEGER(C_INT), ALLOCATABLE, TARGET :: mat0(:,:)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: ox(:), oy(:), w(:), nu(:), dx(:), dy(:), km(:)
+
+  IF (C_ASSOCIATED(TRT_CTX)) CALL TRT_GPU_FINALIZE()
+  ALLOCATE(mat0(SIZE(Mat,1), SIZE(Mat,2)))
+  mat0 = Mat - 1
+  ox = Omega_x; oy = Omega_y; w = quad_weight; nu = Nu_g; dx = Delx; dy = Dely; km = Kappa_Mult
+
+  p%nx = SIZE(Delx); p%ny = SIZE(Dely); p%nm = SIZE(Omega_x); p%ng = SIZE(Nu_g) - 1; p%nmat = SIZE(Kappa_Mult)
+  p%dx = C_LOC(dx); p%dy = C_LOC(dy); p%ox = C_LOC(ox); p%oy = C_LOC(oy); p%w = C_LOC(w); p%nu = C_LOC(nu)
+  p%mat = C_LOC(mat0); p%kmult = C_LOC(km)
+  p%c = c; p%dt = Delt; p%theta = Theta; p%comp_unit = Comp_Unit; p%cv = Cv; p%kap0 = Kap0; p%tini = Tini
+  p%bc_type = BC_Type
+  p%bcT = [bcT_left, bcT_bottom, bcT_right, bcT_top]
+  p%opacity_model = opacity_model
+  p%maxit_rte = maxit_RTE; p%maxit_mloqd = maxit_MLOQD; p%maxit_gloqd = maxit_GLOQD; p%conv_type = conv_type
+  p%conv_ho = conv_ho; p%conv_lo = conv_lo; p%conv_gr1 = conv_gr1; p%conv_gr2 = conv_gr2
+  p%chi = chi; p%line_src = line_src
+  p%use_line_search = MERGE(1, 0, Use_Line_Search)
+  p%use_safety_search = MERGE(1, 0, Use_Safety_Search)
+  p%kapE_dT_flag = MERGE(1, 0, kapE_dT_flag)
+  p%E_bound_low = E_Bound_Low; p%T_bound_low = T_Bound_Low
+  p%threads = 1; p%precision = precision; p%tile_x = 0; p%tile_y = 0; p%lin_rtol = 0d0; p%lin_maxit = 0
+  CALL TRT_CHECK(trt_ctx_create(TRT_CTX, p), 'trt_ctx_create')
+  DEALLOCATE(mat0, ox, oy, w, nu, dx, dy, km)
+
+END SUBROUTINE TRT_GPU_INIT
+
+SUBROUTINE TRT_GPU_FINALIZE()
+  INTEGER(C_INT) :: rc
+  IF (C_ASSOCIATED(TRT_CTX)) rc = trt_ctx_destroy(TRT_CTX)
+  TRT_CTX = C_NULL_PTR
+END SUBROUTINE TRT_GPU_FINALIZE
+
+END MODULE TRT_CAPI

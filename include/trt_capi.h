@@ -63,7 +63,10 @@ size_t trt_ctx_bytes(const trt_problem *p);
 
 /* host <-> device copies by reference array name, Fortran shapes.  n is the
  * number of doubles the caller's buffer holds (must equal the array size).
- * trt_array_size returns that size, or 0 for an unknown name. */
+ * trt_array_size returns that size, or 0 for an unknown name.  I_avg, I_edgV,
+ * I_edgH, Ic_edgV and Ic_edgH are pull-only views derived from I_crn and the
+ * boundary intensities; pushing KapE or RT_Src also refreshes the sweep's
+ * per-(cell, group) coefficients. */
 size_t trt_array_size(const trt_ctx *ctx, const char *name);
 int trt_push(trt_ctx *ctx, const char *name, const double *host, size_t n);
 int trt_pull(trt_ctx *ctx, const char *name, double *host, size_t n);

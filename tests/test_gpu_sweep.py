@@ -98,3 +98,30 @@ def test_equilibrium_sweep_is_exact(trt):
     bg = ctx.pull("Bg", (8, 8, prob.ng))[0, 0, :]
     assert np.max(np.abs(I / bg[None, None, None, :] - 1.0)) < 1e-13
     ctx.close()
+
+
+def test_derived_intensity_views(trt, oracle_mod):
+    """I_avg, I_edgV, I_edgH, Ic_edgV, Ic_edgH pulled from the device equal the reference arrays."""
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 7, 6, 8, 2, (4, 4))
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    for name in ("I_avg", "I_edgV", "I_edgH", "Ic_edgV", "Ic_edgH"):
+        ref = o[name]
+        assert rel_err(ctx.pull(name, ref.shape), ref) < 1e-12, name
+    ctx.close()
+
+
+def test_push_kape_and_source_reaches_the_sweep(trt, oracle_mod):
+    """Pushing KapE and RT_Src (as the Fortran drop-in does) feeds the next sweep."""
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 6, 5, 4, 2, (8, 8))
+    rng = np.random.default_rng(9)
+    ke = o["KapE"] * rng.uniform(0.5, 2.0, o["KapE"].shape)
+    rs = o["RT_Src"] * rng.uniform(0.5, 2.0, o["RT_Src"].shape)
+    o["KapE"][...] = ke
+    o["RT_Src"][...] = rs
+    ctx.push("KapE", ke)
+    ctx.push("RT_Src", rs)
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    assert rel_err(ctx.pull("I_crn", o["I_crn"].shape), o["I_crn"]) < 1e-12
+    ctx.close()
