@@ -42,6 +42,7 @@ void validate(const trt_problem *p)
     for (int k = 0; k < 4; ++k)
         if (p->bc_type[k] != 0) throw Error("only incoming-Planckian boundaries (BC_Type 0) are supported");
     if (p->precision != 0 && p->precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 sweep)");
+    if (p->sweep_mode < 0 || p->sweep_mode > 3) throw Error("sweep_mode must be 0, 1, 2 or 3");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
     for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
         if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");
@@ -88,6 +89,7 @@ int trt_ctx_destroy(trt_ctx *ctx)
     if (!ctx) return -1;
     Ctx &c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    trt::pk_destroy(c);
     for (void *p : c.owned) (void)hipFree(p);
     if (c.hbuf) (void)hipHostFree(c.hbuf);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -181,6 +183,7 @@ int trt_transport_scb(trt_ctx *ctx)
         TRT_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         c.sweeps += 1;
         c.sweep_ms += ms;
+        trt::pk_check(c);
     });
 }
 
@@ -211,7 +214,10 @@ int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *
 
 int trt_step(trt_ctx *ctx, int its_out[4])
 {
-    return guarded(ctx, [&](Ctx &c) { trt::step(c, its_out); });
+    return guarded(ctx, [&](Ctx &c) {
+        trt::step(c, its_out);
+        trt::pk_check(c);
+    });
 }
 
 double trt_time(const trt_ctx *ctx) { return ctx ? ctx->c.time : 0.0; }
@@ -229,6 +235,7 @@ int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms)
         c.sweeps += nsweeps;
         c.sweep_ms += t;
         if (ms) *ms = t;
+        trt::pk_check(c);
     });
 }
 
@@ -258,6 +265,7 @@ int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *k
         if (wall_ms) *wall_ms = wall;
         if (kernel_ms) *kernel_ms = kern;
         if (launches) *launches = nl * nsweeps;
+        trt::pk_check(c);
     });
 }
 

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace trt {
@@ -35,19 +36,24 @@ struct QDView {
     size_t nc, nv, nh, nrow;
     const double *coef;   // [sys][NCOEF][nc]
     const double *bnd;    // [sys][NBND][nb]
+    const double *cs;     // [sys][nrow] column scales: the row computes (A S y)_row for x = S y
 };
 
-__device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, const double *x, double &ax,
+// Row `row` of system `sys` applied to the scaled unknowns y (x = S y, S = diag(cs)):
+// ax = (A S y)_row, diag = A_row,row * cs_row, rhs = b_row.
+__device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, const double *y, double &ax,
                                        double &diag, double &rhs)
 {
     const int nx = V.nx, ny = V.ny;
     const double *cf = V.coef + (size_t)sys * NCOEF * V.nc;
     const double *bd = V.bnd + (size_t)sys * NBND * V.nb;
-    const double *xc = x, *xv = x + V.nc, *xh = x + V.nc + V.nv;
+    const double *s = V.cs + (size_t)sys * V.nrow;
+    const double *xc = y, *xv = y + V.nc, *xh = y + V.nc + V.nv;
+    const double *sc = s, *sv = s + V.nc, *sh = s + V.nc + V.nv;
 #define CF(k, i, j) cf[(size_t)(k) * V.nc + (size_t)(i) + (size_t)nx * (j)]
-#define XC(i, j) xc[(size_t)(i) + (size_t)nx * (j)]
-#define XV(i, j) xv[(size_t)(i) + (size_t)(nx + 1) * (j)]
-#define XH(i, j) xh[(size_t)(i) + (size_t)nx * (j)]
+#define XC(i, j) (xc[(size_t)(i) + (size_t)nx * (j)] * sc[(size_t)(i) + (size_t)nx * (j)])
+#define XV(i, j) (xv[(size_t)(i) + (size_t)(nx + 1) * (j)] * sv[(size_t)(i) + (size_t)(nx + 1) * (j)])
+#define XH(i, j) (xh[(size_t)(i) + (size_t)nx * (j)] * sh[(size_t)(i) + (size_t)nx * (j)])
     if (row < V.nc) {                       // energy balance of cell (i,j)
         const int i = (int)(row % nx), j = (int)(row / nx);
         ax = CF(EBL, i, j) * XV(i, j) + CF(EBB, i, j) * XH(i, j) + CF(EBC, i, j) * XC(i, j)
@@ -99,6 +105,7 @@ __device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, con
             rhs = CF(MYRHS, i, j);
         }
     }
+    diag *= s[row];
 #undef CF
 #undef XC
 #undef XV
@@ -106,10 +113,15 @@ __device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, con
 }
 
 // ---------------------------------------------------------------------------
-// batched BiCGSTAB (right Jacobi preconditioning)
+// batched BiCGSTAB on the two-sided scaled system D^-1 A S y = D^-1 b, x = S y,
+// S = diag(|x_ref| + floor), D = diag(A S).  The unknowns of one system span
+// many orders of magnitude (E in a cold region is ~1e-12 of E near a hot
+// boundary) and so do the rows, so the solve is posed in relative terms: the
+// stopping test then bounds every unknown's error relative to its own size,
+// which a norm of the unscaled residual cannot do.
 // ---------------------------------------------------------------------------
 // per-system scalars
-enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_BNORM, S_RNORM, S_DONE, S_ITS, S_NSCAL };
+enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_BNORM, S_RNORM, S_DONE, S_ITS, S_XMAX, S_ACC, S_NSCAL };
 
 constexpr int RT = 256;   // threads of the vector kernels
 
@@ -129,7 +141,7 @@ struct KrylovArgs {
     QDView V;
     int nsys, RB;
     const double *b;
-    double *x, *r, *rh, *p, *v, *s, *t, *ph, *sh, *dg;
+    double *x, *r, *rh, *p, *v, *s, *t, *dg;
     double *red;      // [sys][4][RB]
     double *scal;     // [sys][S_NSCAL]
     double tol;
@@ -151,10 +163,10 @@ __global__ __launch_bounds__(RT) void k_init(KrylovArgs a)
     for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
         double ax, dg, rhs;
         qd_row(a.V, sys, row, a.x + off, ax, dg, rhs);
-        const double r = rhs - ax;
+        const double r = (rhs - ax) / dg, bs = rhs / dg;
         a.r[off + row] = r; a.rh[off + row] = r; a.p[off + row] = 0.0; a.v[off + row] = 0.0;
         a.dg[off + row] = dg;
-        bb += rhs * rhs; rr += r * r;
+        bb += bs * bs; rr += r * r;
     }
     const double s1 = block_sum(bb, shm);
     const double s2 = block_sum(rr, shm);
@@ -173,13 +185,18 @@ __global__ void k_init_fin(KrylovArgs a)
     double *S = a.scal + sys * S_NSCAL;
     S[S_BNORM] = sqrt(bb);
     S[S_RNORM] = sqrt(rr);
+    if (S[S_ACC] != 0.0) {   // accepted in an earlier stage: leave it alone
+        S[S_DONE] = 1.0;
+        S[S_ITS] = 0.0;
+        return;
+    }
     S[S_RHO] = rr;           // (rh, r) with rh = r
     S[S_ALPHA] = 1.0; S[S_OMEGA] = 1.0; S[S_BETA] = 0.0;
     S[S_ITS] = 0.0;
     S[S_DONE] = (sqrt(rr) <= a.tol * sqrt(bb) || rr == 0.0) ? 1.0 : 0.0;
 }
 
-// p = r + beta (p - omega v); ph = p / diag
+// p = r + beta (p - omega v)
 __global__ __launch_bounds__(RT) void k_p(KrylovArgs a)
 {
     const double *S = a.scal + blockIdx.y * S_NSCAL;
@@ -188,11 +205,10 @@ __global__ __launch_bounds__(RT) void k_p(KrylovArgs a)
         const size_t k = off + row;
         const double p = a.r[k] + beta * (a.p[k] - om * a.v[k]);
         a.p[k] = p;
-        a.ph[k] = p / a.dg[k];
     }
 }
 
-// v = A ph; partial (rh, v)
+// v = D^-1 A p; partial (rh, v)
 __global__ __launch_bounds__(RT) void k_av(KrylovArgs a)
 {
     __shared__ double shm[8];
@@ -202,9 +218,10 @@ __global__ __launch_bounds__(RT) void k_av(KrylovArgs a)
     double acc = 0.0;
     for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
         double ax, dg, rhs;
-        qd_row(a.V, sys, row, a.ph + off, ax, dg, rhs);
-        a.v[off + row] = ax;
-        acc += a.rh[off + row] * ax;
+        qd_row(a.V, sys, row, a.p + off, ax, dg, rhs);
+        const double v = ax / dg;
+        a.v[off + row] = v;
+        acc += a.rh[off + row] * v;
     }
     const double s = block_sum(acc, shm);
     if (threadIdx.x == 0) a.red[((size_t)sys * 4 + 0) * a.RB + blockIdx.x] = s;
@@ -222,7 +239,7 @@ __global__ void k_alpha(KrylovArgs a)
     S[S_ALPHA] = S[S_RHO] / d;
 }
 
-// s = r - alpha v; sh = s / diag
+// s = r - alpha v
 __global__ __launch_bounds__(RT) void k_s(KrylovArgs a)
 {
     const double al = a.scal[blockIdx.y * S_NSCAL + S_ALPHA];
@@ -230,7 +247,6 @@ __global__ __launch_bounds__(RT) void k_s(KrylovArgs a)
         const size_t k = off + row;
         const double s = a.r[k] - al * a.v[k];
         a.s[k] = s;
-        a.sh[k] = s / a.dg[k];
     }
 }
 
@@ -244,10 +260,11 @@ __global__ __launch_bounds__(RT) void k_at(KrylovArgs a)
     double ts = 0.0, tt = 0.0;
     for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
         double ax, dg, rhs;
-        qd_row(a.V, sys, row, a.sh + off, ax, dg, rhs);
-        a.t[off + row] = ax;
-        ts += ax * a.s[off + row];
-        tt += ax * ax;
+        qd_row(a.V, sys, row, a.s + off, ax, dg, rhs);
+        const double t = ax / dg;
+        a.t[off + row] = t;
+        ts += t * a.s[off + row];
+        tt += t * t;
     }
     const double s1 = block_sum(ts, shm);
     const double s2 = block_sum(tt, shm);
@@ -268,7 +285,7 @@ __global__ void k_omega(KrylovArgs a)
     S[S_OMEGA] = tt > 0.0 ? ts / tt : 0.0;
 }
 
-// x += alpha ph + omega sh; r = s - omega t; partials (r,r), (rh,r)
+// x += alpha p + omega s; r = s - omega t; partials (r,r), (rh,r)
 __global__ __launch_bounds__(RT) void k_xr(KrylovArgs a)
 {
     __shared__ double shm[8];
@@ -280,7 +297,7 @@ __global__ __launch_bounds__(RT) void k_xr(KrylovArgs a)
     double rr = 0.0, rhr = 0.0;
     for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
         const size_t k = off + row;
-        a.x[k] += al * a.ph[k] + om * a.sh[k];
+        a.x[k] += al * a.p[k] + om * a.s[k];
         const double r = a.s[k] - om * a.t[k];
         a.r[k] = r;
         rr += r * r;
@@ -320,20 +337,97 @@ __global__ __launch_bounds__(RT) void k_true_res(KrylovArgs a)
     for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
         double ax, dg, rhs;
         qd_row(a.V, sys, row, a.x + off, ax, dg, rhs);
-        rr += (rhs - ax) * (rhs - ax);
+        const double r = (rhs - ax) / dg;
+        rr += r * r;
     }
     const double s = block_sum(rr, shm);
     if (threadIdx.x == 0) a.red[((size_t)sys * 4 + 2) * a.RB + blockIdx.x] = s;
 }
 
-__global__ void k_true_fin(KrylovArgs a, double *out)
+// true residual norms; a system whose residual meets thr * |b| is accepted
+__global__ void k_true_fin(KrylovArgs a, double *out, double thr)
 {
     const int sys = blockIdx.x * blockDim.x + threadIdx.x;
     if (sys >= a.nsys) return;
     double rr = 0.0;
     for (int k = 0; k < a.RB; ++k) rr += a.red[((size_t)sys * 4 + 2) * a.RB + k];
-    out[2 * sys] = sqrt(rr);
-    out[2 * sys + 1] = a.scal[sys * S_NSCAL + S_BNORM];
+    double *S = a.scal + sys * S_NSCAL;
+    const double r = sqrt(rr), b = S[S_BNORM];
+    if (r <= thr * b || rr == 0.0) S[S_ACC] = 1.0;
+    out[3 * sys] = r;
+    out[3 * sys + 1] = b;
+    out[3 * sys + 2] = S[S_ACC];
+}
+
+__global__ void k_clear_acc(KrylovArgs a)
+{
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys < a.nsys) a.scal[sys * S_NSCAL + S_ACC] = 0.0;
+}
+
+// systems not accepted by a stage start the next one from the original guess
+__global__ __launch_bounds__(RT) void k_restore(KrylovArgs a, const double *x0)
+{
+    const int sys = blockIdx.y;
+    if (a.scal[sys * S_NSCAL + S_ACC] != 0.0) return;
+    const size_t off = (size_t)sys * a.V.nrow;
+    for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT)
+        a.x[off + row] = x0[off + row];
+}
+
+// column scales from a reference vector (the initial guess, or the state the
+// unknowns correct): per-system max |ref| first, then cs = |ref| + 1e-20 max
+__global__ __launch_bounds__(RT) void k_refmax(KrylovArgs a, const double *ref)
+{
+    __shared__ double shm[8];
+    const int sys = blockIdx.y;
+    const size_t off = (size_t)sys * a.V.nrow;
+    double m = 0.0;
+    for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT)
+        m = fmax(m, fabs(ref[off + row]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) shm[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t = fmax(t, shm[k]);
+        a.red[((size_t)sys * 4 + 3) * a.RB + blockIdx.x] = t;
+    }
+}
+
+__global__ void k_refmax_fin(KrylovArgs a)
+{
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= a.nsys) return;
+    double m = 0.0;
+    for (int k = 0; k < a.RB; ++k) m = fmax(m, a.red[((size_t)sys * 4 + 3) * a.RB + k]);
+    a.scal[sys * S_NSCAL + S_XMAX] = m;
+}
+
+// cs = |ref| + rel_floor * max|ref| (cs = 1 when rel_floor < 0); x (unscaled guess) -> y = x / cs
+__global__ __launch_bounds__(RT) void k_colscale(KrylovArgs a, const double *ref, double *cs, double rel_floor)
+{
+    const int sys = blockIdx.y;
+    const double mx = a.scal[sys * S_NSCAL + S_XMAX];
+    const bool unit = rel_floor < 0.0 || !(mx > 0.0);
+    const double floor = unit ? 1.0 : rel_floor * mx;
+    const size_t off = (size_t)sys * a.V.nrow;
+    for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT) {
+        const size_t k = off + row;
+        const double s = unit ? 1.0 : fabs(ref[k]) + floor;
+        cs[k] = s;
+        a.x[k] = a.x[k] / s;
+    }
+}
+
+// x = S y
+__global__ __launch_bounds__(RT) void k_unscale(KrylovArgs a)
+{
+    const int sys = blockIdx.y;
+    const size_t off = (size_t)sys * a.V.nrow;
+    for (size_t row = (size_t)blockIdx.x * RT + threadIdx.x; row < a.V.nrow; row += (size_t)a.RB * RT)
+        a.x[off + row] *= a.V.cs[off + row];
 }
 
 void qd_alloc(Ctx &c, QDSystems &s, int nsys)
@@ -348,7 +442,10 @@ void qd_alloc(Ctx &c, QDSystems &s, int nsys)
     s.b = nullptr;
     s.x = static_cast<double *>(c.dalloc((size_t)nsys * s.nrow * 8));
     s.diag = static_cast<double *>(c.dalloc((size_t)nsys * s.nrow * 8));
-    s.work = static_cast<double *>(c.dalloc((size_t)8 * nsys * s.nrow * 8));
+    s.cs = static_cast<double *>(c.dalloc((size_t)nsys * s.nrow * 8));
+    s.ref = static_cast<double *>(c.dalloc((size_t)nsys * s.nrow * 8));
+    s.x0 = static_cast<double *>(c.dalloc((size_t)nsys * s.nrow * 8));
+    s.work = static_cast<double *>(c.dalloc((size_t)6 * nsys * s.nrow * 8));
     s.red_blocks = (int)std::min<size_t>(256, std::max<size_t>(1, (s.nrow + RT * 4 - 1) / (RT * 4)));
     s.red = static_cast<double *>(c.dalloc((size_t)nsys * 4 * s.red_blocks * 8));
     s.scal = static_cast<double *>(c.dalloc((size_t)nsys * S_NSCAL * 8));
@@ -363,68 +460,94 @@ static KrylovArgs kargs(Ctx &c, QDSystems &s)
     a.V.nx = s.nx; a.V.ny = s.ny; a.V.nb = std::max(s.nx, s.ny);
     a.V.nc = (size_t)s.nx * s.ny; a.V.nv = (size_t)(s.nx + 1) * s.ny; a.V.nh = (size_t)s.nx * (s.ny + 1);
     a.V.nrow = s.nrow;
-    a.V.coef = s.coef; a.V.bnd = s.bnd;
+    a.V.coef = s.coef; a.V.bnd = s.bnd; a.V.cs = s.cs;
     a.nsys = s.nsys; a.RB = s.red_blocks;
     a.x = s.x;
     const size_t n = (size_t)s.nsys * s.nrow;
     double *w = s.work;
     a.r = w; a.rh = w + n; a.p = w + 2 * n; a.v = w + 3 * n; a.s = w + 4 * n; a.t = w + 5 * n;
-    a.ph = w + 6 * n; a.sh = w + 7 * n;
     a.dg = s.diag;
     a.red = s.red; a.scal = s.scal;
     a.tol = c.lin_rtol;
     return a;
 }
 
-void qd_solve(Ctx &c, QDSystems &s, int *its_out)
+// Solves every system in up to three stages of decreasing column scaling:
+// scales |ref| + 1e-20 max|ref| (relative accuracy of every unknown), then
+// |ref| + 1e-6 max|ref|, then none.  Strong scaling can leave the scaled matrix
+// too far from diagonal dominance for Jacobi-BiCGSTAB (a hot cell next to a
+// cold one), so a system that a stage does not bring to 4 rtol on its true
+// residual restarts the next stage from its original guess.
+void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
 {
     KrylovArgs a = kargs(c, s);
     const dim3 grid(a.RB, s.nsys), blk(RT);
     const int fin_blocks = (s.nsys + 63) / 64;
+    const size_t n = (size_t)s.nsys * s.nrow;
+    if (!ref_given) dcopy(c, s.ref, s.x, n);
+    dcopy(c, s.x0, s.x, n);
+    hipLaunchKernelGGL(k_refmax, grid, blk, 0, c.stream, a, s.ref);
+    hipLaunchKernelGGL(k_refmax_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+    hipLaunchKernelGGL(k_clear_acc, dim3(fin_blocks), dim3(64), 0, c.stream, a);
     std::vector<double> hs((size_t)s.nsys * S_NSCAL);
     std::vector<int> total_its(s.nsys, 0);
-    for (int restart = 0; restart < 4; ++restart) {
-        hipLaunchKernelGGL(k_init, grid, blk, 0, c.stream, a);
-        hipLaunchKernelGGL(k_init_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
-        int it = 0;
-        int check = 4;
-        while (it < c.lin_maxit) {
-            for (int k = 0; k < check && it < c.lin_maxit; ++k, ++it) {
-                hipLaunchKernelGGL(k_p, grid, blk, 0, c.stream, a);
-                hipLaunchKernelGGL(k_av, grid, blk, 0, c.stream, a);
-                hipLaunchKernelGGL(k_alpha, dim3(fin_blocks), dim3(64), 0, c.stream, a);
-                hipLaunchKernelGGL(k_s, grid, blk, 0, c.stream, a);
-                hipLaunchKernelGGL(k_at, grid, blk, 0, c.stream, a);
-                hipLaunchKernelGGL(k_omega, dim3(fin_blocks), dim3(64), 0, c.stream, a);
-                hipLaunchKernelGGL(k_xr, grid, blk, 0, c.stream, a);
-                hipLaunchKernelGGL(k_rho, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+    const double floors[3] = {1e-20, 1e-6, -1.0};
+    bool all_ok = false;
+    std::vector<double> h(3 * (size_t)s.nsys);
+    for (int stage = 0; stage < 3 && !all_ok; ++stage) {
+        if (stage > 0) hipLaunchKernelGGL(k_restore, grid, blk, 0, c.stream, a, s.x0);
+        hipLaunchKernelGGL(k_colscale, grid, blk, 0, c.stream, a, s.ref, s.cs, floors[stage]);
+        const double thr = (stage == 2 ? 100.0 : 4.0) * c.lin_rtol;
+        for (int restart = 0; restart < 4; ++restart) {
+            hipLaunchKernelGGL(k_init, grid, blk, 0, c.stream, a);
+            hipLaunchKernelGGL(k_init_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+            int it = 0;
+            int check = 4;
+            while (it < c.lin_maxit) {
+                for (int k = 0; k < check && it < c.lin_maxit; ++k, ++it) {
+                    hipLaunchKernelGGL(k_p, grid, blk, 0, c.stream, a);
+                    hipLaunchKernelGGL(k_av, grid, blk, 0, c.stream, a);
+                    hipLaunchKernelGGL(k_alpha, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+                    hipLaunchKernelGGL(k_s, grid, blk, 0, c.stream, a);
+                    hipLaunchKernelGGL(k_at, grid, blk, 0, c.stream, a);
+                    hipLaunchKernelGGL(k_omega, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+                    hipLaunchKernelGGL(k_xr, grid, blk, 0, c.stream, a);
+                    hipLaunchKernelGGL(k_rho, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+                }
+                TRT_CHECK(hipGetLastError());
+                TRT_CHECK(hipMemcpyAsync(hs.data(), a.scal, hs.size() * 8, hipMemcpyDeviceToHost, c.stream));
+                TRT_CHECK(hipStreamSynchronize(c.stream));
+                bool all = true;
+                for (int k = 0; k < s.nsys; ++k) all = all && hs[(size_t)k * S_NSCAL + S_DONE] != 0.0;
+                if (all) break;
+                check = std::min(check * 2, 64);
             }
-            TRT_CHECK(hipGetLastError());
-            TRT_CHECK(hipMemcpyAsync(hs.data(), a.scal, hs.size() * 8, hipMemcpyDeviceToHost, c.stream));
+            for (int k = 0; k < s.nsys; ++k) total_its[k] += (int)hs[(size_t)k * S_NSCAL + S_ITS];
+            // acceptance on the true residual
+            double *tr = c.scratch;
+            hipLaunchKernelGGL(k_true_res, grid, blk, 0, c.stream, a);
+            hipLaunchKernelGGL(k_true_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a, tr, thr);
+            TRT_CHECK(hipMemcpyAsync(h.data(), tr, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
             TRT_CHECK(hipStreamSynchronize(c.stream));
-            bool all = true;
-            for (int k = 0; k < s.nsys; ++k) all = all && hs[(size_t)k * S_NSCAL + S_DONE] != 0.0;
-            if (all) break;
-            check = std::min(check * 2, 64);
+            all_ok = true;
+            bool diverged = false;
+            for (int k = 0; k < s.nsys; ++k) {
+                if (h[3 * k + 2] == 0.0) all_ok = false;
+                if (!(h[3 * k] <= h[3 * k + 1])) diverged = true;    // worse than x = 0, or not finite
+            }
+            if (all_ok || (diverged && stage < 2)) break;
         }
-        for (int k = 0; k < s.nsys; ++k) total_its[k] += (int)hs[(size_t)k * S_NSCAL + S_ITS];
-        // acceptance on the true residual
-        double *tr = c.scratch;
-        hipLaunchKernelGGL(k_true_res, grid, blk, 0, c.stream, a);
-        hipLaunchKernelGGL(k_true_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a, tr);
-        std::vector<double> h(2 * (size_t)s.nsys);
-        TRT_CHECK(hipMemcpyAsync(h.data(), tr, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
-        TRT_CHECK(hipStreamSynchronize(c.stream));
-        bool ok = true;
+        hipLaunchKernelGGL(k_unscale, grid, blk, 0, c.stream, a);
+    }
+    TRT_CHECK(hipGetLastError());
+    if (!all_ok) {
         for (int k = 0; k < s.nsys; ++k)
-            if (!(h[2 * k] <= 100.0 * c.lin_rtol * h[2 * k + 1] || h[2 * k] == 0.0)) ok = false;
-        if (ok) break;
-        if (restart == 3) {
-            for (int k = 0; k < s.nsys; ++k)
-                if (!(h[2 * k] <= 1e4 * c.lin_rtol * h[2 * k + 1] || h[2 * k] == 0.0))
-                    throw Error("QD Krylov solve did not converge (system " + std::to_string(k) + ", |r|/|b| = " +
-                                std::to_string(h[2 * k] / h[2 * k + 1]) + ")");
-        }
+            if (!(h[3 * k] <= 1e4 * c.lin_rtol * h[3 * k + 1] || h[3 * k] == 0.0)) {
+                char msg[160];
+                snprintf(msg, sizeof msg, "QD Krylov solve did not converge (system %d, |r|/|b| = %.3e)", k,
+                         h[3 * k] / h[3 * k + 1]);
+                throw Error(msg);
+            }
     }
     if (its_out) for (int k = 0; k < s.nsys; ++k) its_out[k] = total_its[k];
 }
@@ -697,7 +820,7 @@ void mloqd_fv(Ctx &c, int *kits)
     hipLaunchKernelGGL(qd_gather, dim3(nb(c.mg.nrow * c.ng)), dim3(256), 0, c.stream, c.mg.x, Ea, Ev, Eh, nc, nv, nh, c.ng);
     TRT_CHECK(hipGetLastError());
     std::vector<int> its(c.ng);
-    qd_solve(c, c.mg, its.data());
+    qd_solve(c, c.mg, its.data(), false);
     hipLaunchKernelGGL(qd_scatter, dim3(nb(c.mg.nrow * c.ng)), dim3(256), 0, c.stream, c.mg.x, Ea, Ev, Eh, nc, nv, nh, c.ng);
     hipLaunchKernelGGL(ml_flux, dim3(nb(tot)), dim3(256), 0, c.stream, a, Ea, Ev, Eh, c.arr("Fxg_edgV"), c.arr("Fyg_edgH"));
     TRT_CHECK(hipGetLastError());
@@ -1073,9 +1196,11 @@ int egp_fv_newt(Ctx &c, int mgqd_it)
         hipLaunchKernelGGL(newt_faces, dim3(nb(nc)), dim3(256), 0, c.stream, a);
         // previous Delta E is the initial guess (GL:565)
         hipLaunchKernelGGL(qd_gather, dim3(nb(c.grey.nrow)), dim3(256), 0, c.stream, c.grey.x, a.dEa, a.dEv, a.dEh, nc, nv, nh, 1);
+        // the deltas are measured against the grey E they correct
+        hipLaunchKernelGGL(qd_gather, dim3(nb(c.grey.nrow)), dim3(256), 0, c.stream, c.grey.ref, a.Ea, a.Ev, a.Eh, nc, nv, nh, 1);
         TRT_CHECK(hipGetLastError());
         int kits = 0;
-        qd_solve(c, c.grey, &kits);
+        qd_solve(c, c.grey, &kits, true);
         hipLaunchKernelGGL(qd_scatter, dim3(nb(c.grey.nrow)), dim3(256), 0, c.stream, c.grey.x, a.dEa, a.dEv, a.dEh, nc, nv, nh, 1);
         hipLaunchKernelGGL(newt_deltas, dim3(nb(nc)), dim3(256), 0, c.stream, a);
         // LINE_SEARCH on the six delta norms in the reference order (GL:604-636)

@@ -284,8 +284,8 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     sg.gblk = SWEEP_THREADS / sg.K;
     if (sg.gblk > p.ng) sg.gblk = p.ng;
     sg.nblk_g = (p.ng + sg.gblk - 1) / sg.gblk;
-    sg.TX = p.tile_x > 0 ? p.tile_x : 8;
-    sg.TY = p.tile_y > 0 ? p.tile_y : 8;
+    sg.TX = p.tile_x > 0 ? p.tile_x : 4;
+    sg.TY = p.tile_y > 0 ? p.tile_y : 4;
     sg.NTX = (p.nx + sg.TX - 1) / sg.TX;
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }
@@ -311,7 +311,10 @@ void sweep_setup(Ctx &c)
     const SweepGeom &sg = c.sg;
     if (c.nm % 4) throw Error("N_m must be a multiple of 4 (quadrant blocks)");
     if (sg.K > SWEEP_THREADS) throw Error("more than 256 directions per quadrant is not supported");
-    if (sg.TX != 4 && sg.TX != 8 && sg.TX != 16) throw Error("tile_x must be 4, 8 or 16");
+    c.sweep_mode = c.p.sweep_mode == 0 ? 3 : c.p.sweep_mode;
+    if (c.sweep_mode == 1 && sg.TX != 4 && sg.TX != 8 && sg.TX != 16)
+        throw Error("tile_x must be 4, 8 or 16 for sweep_mode 1");
+    if (c.sweep_mode >= 2 && !pk_supported(c)) throw Error("tile_x must be 2, 4 or 8 for sweep_mode 2 and 3");
     const size_t nc = (size_t)c.nx * c.ny, NL = (size_t)c.ng * sg.K;
     const size_t ti = c.precision ? 4 : 8;
     c.icrn_bytes = nc * 4 * NL * 4 * ti;
@@ -338,6 +341,14 @@ static void launch_sweep(Ctx &c, SweepArgs &a, int nblocks)
 void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
 {
     const SweepGeom &sg = c.sg;
+    if (c.sweep_mode >= 2) {
+        const int out = 1 - c.iold;
+        if (c.sweep_mode == 2) pk_sweep(c, ev);
+        else diag_sweep(c, ev);
+        TRT_CHECK(hipGetLastError());
+        c.icur = out;
+        return;
+    }
     // the new intensities go to the buffer that does not hold I_crn_old
     const int out = 1 - c.iold;
     SweepArgs a{};
@@ -547,7 +558,7 @@ void rebuild_sc(Ctx &c)
     TRT_CHECK(hipGetLastError());
 }
 
-int sweep_launches(const Ctx &c) { return c.sg.NTX + c.sg.NTY - 1; }
+int sweep_launches(const Ctx &c) { return c.sweep_mode == 2 ? 1 : c.sg.NTX + c.sg.NTY - 1; }
 
 // plain device copy of n doubles, the calibration kernel for HBM byte counters
 __global__ void calib_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)

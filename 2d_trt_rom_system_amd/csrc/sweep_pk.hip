@@ -1,0 +1,649 @@
+// K1 (persistent form): the SCB sweep of TRANSPORT_SCB (MainSrc/TRANSPORT_SOLVES.f08:14-327)
+// with the angular sums of COLLAPSE_INTENSITIES (:381-511) as ONE launch per sweep.
+//
+// Work items are (problem, quadrant, group block, tile); an item depends on the
+// tiles left of and below it in its quadrant's reflected frame.  Items are
+// listed by tile diagonal, so the list is a topological order, and resident
+// workgroups take them from an atomic counter.  A workgroup waits (bounded
+// spin) until the two upwind tiles have published their faces, sweeps its tile
+// and publishes its own completion with an agent-scope release.  Because every
+// item a workgroup waits for was taken earlier by a running workgroup, the grid
+// size never matters for progress.  Completion flags carry the sweep's epoch,
+// so they are never cleared between sweeps.
+//
+// Per tile, a thread owns one (quadrant, group, direction) and walks the rows
+// of a TX x TY tile with the upwind corner-edge intensities in registers, one
+// row of old intensities prefetched ahead.  After each row the block stages the
+// corner averages of every (cell, direction) in LDS and reduces them over the
+// directions of each group: S adjacent lanes per (cell, group) pair each sum
+// ceil(K/S) directions against a per-quadrant table of weighted direction
+// cosines and combine with cross-lane shuffles, so a row costs one block
+// barrier (double-buffered staging) instead of two per cell.
+//
+// The corner solve, the reflection to the first quadrant and the intensity
+// layout are those of the per-diagonal kernel in sweep.hip.
+#include "trt_internal.h"
+
+#include <algorithm>
+
+namespace trt {
+
+namespace {
+
+constexpr int PK_THREADS = 256;
+constexpr int NMOM = 13;
+
+struct PkProb {
+    int nx, ny, ng, K, gblk, TY, NTX, NTY;
+    int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per row
+    double inv_cdt, eps;
+    const double *dx, *dy, *ox, *oy, *w;
+    const double2 *sc;     // [cell][g] (KapE + 1/(c dt), RT_Src)
+    const double *bcin;    // [4][ng]
+    const void *Iold;
+    void *Inew;
+    void *XV, *XH;
+    double *Pavg, *PV, *PH;
+};
+
+struct PkArgs {
+    const PkProb *probs;
+    const int4 *items;     // {problem, own flag, txl | tyl << 16, q | gb << 8}
+    int nitems;
+    int *queue;            // [0] next item, [1] epoch
+    int *flags;
+    int *err;
+    int spin_limit;
+    int stage_bufs;        // 1 or 2 LDS staging buffers
+};
+
+template <typename T> struct V4;
+template <> struct V4<double> {
+    double v[4];
+    __device__ static V4 load(const double *p) {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+        V4 r; r.v[0] = a.x; r.v[1] = a.y; r.v[2] = b.x; r.v[3] = b.y; return r;
+    }
+    __device__ void store(double *p) const {
+        *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+    }
+};
+template <> struct V4<float> {
+    float v[4];
+    __device__ static V4 load(const float *p) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        V4 r; r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w; return r;
+    }
+    __device__ void store(float *p) const {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
+// Bounded wait for an upwind tile.  A wait that runs out sets the error word, and
+// every later wait of the launch sees it and returns at once, so a broken
+// dependency ends the launch within one spin limit instead of hanging it.
+__device__ __forceinline__ void wait_flag(const int *f, int epoch, int limit, int *err)
+{
+    int n = 0;
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        __builtin_amdgcn_s_sleep(2);
+        ++n;
+        if ((n & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (n > limit) {
+            atomicOr(err, 1);
+            break;
+        }
+    }
+}
+
+// Sum the staged corner averages of one row over the directions of each group and
+// write the quadrant partial moments of the row's cells and downwind faces.
+template <typename TS, int TX, bool px, bool py>
+__device__ __forceinline__ void reduce_row(const PkProb &P, const TS *st, const double *ctab, int q, int g0, int i0,
+                                           int iw, int j)
+{
+    const int tid = threadIdx.x;
+    const int S = P.S, K = P.K, ng = P.ng, nx = P.nx, gblk = P.gblk;
+    const int s = tid & (S - 1), pofs = tid / S, pstride = PK_THREADS / S;
+    const double eps = P.eps;
+    for (int pb = 0; pb < P.npair; pb += pstride) {
+        const int p = pb + pofs;
+        const int ii = p / gblk, gl = p - ii * gblk;
+        const int g = g0 + gl;
+        const bool ok = p < P.npair && ii < iw && g < ng;
+        double m[NMOM];
+#pragma unroll
+        for (int v = 0; v < NMOM; ++v) m[v] = 0.0;
+        if (ok) {
+            const int k0 = s * P.ks, k1 = min(K, k0 + P.ks);
+            const TS *sa = st + (0 * TX + ii) * PK_THREADS + gl * K;
+            const TS *sv = st + (1 * TX + ii) * PK_THREADS + gl * K;
+            const TS *sh = st + (2 * TX + ii) * PK_THREADS + gl * K;
+            for (int k = k0; k < k1; ++k) {
+                const double ia = (double)sa[k] + eps, iv = (double)sv[k] + eps, ih = (double)sh[k] + eps;
+                const double *cc = ctab + k * 6;   // w, w ox^2, w oy^2, w ox oy, w ox, w oy
+                m[0] += cc[1] * ia; m[1] += cc[2] * ia; m[2] += cc[3] * ia; m[3] += cc[0] * ia;
+                m[4] += cc[1] * iv; m[5] += cc[3] * iv; m[6] += cc[4] * iv; m[7] += cc[0] * iv;
+                m[8] += cc[2] * ih; m[9] += cc[3] * ih; m[10] += cc[4] * ih; m[11] += cc[5] * ih;
+                m[12] += cc[0] * ih;
+            }
+        }
+        for (int off = 1; off < S; off <<= 1) {
+#pragma unroll
+            for (int v = 0; v < NMOM; ++v) m[v] += __shfl_xor(m[v], off);
+        }
+        if (ok) {
+            const int il = i0 + ii;
+            const int i = px ? il : nx - 1 - il;
+            const size_t c = (size_t)i + (size_t)nx * j;
+            const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
+            const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
+            if (s == 0) {
+                double *o = P.Pavg + ((c * 4 + q) * ng + g) * 4;
+                *reinterpret_cast<double2 *>(o) = make_double2(m[0], m[1]);
+                *reinterpret_cast<double2 *>(o + 2) = make_double2(m[2], m[3]);
+            }
+            if (s == (1 & (S - 1))) {
+                double *o = P.PV + ((fV * 4 + q) * ng + g) * 4;
+                *reinterpret_cast<double2 *>(o) = make_double2(m[4], m[5]);
+                *reinterpret_cast<double2 *>(o + 2) = make_double2(m[6], m[7]);
+            }
+            if (s == (2 & (S - 1))) {
+                double *o = P.PH + ((fH * 4 + q) * ng + g) * 5;
+                o[0] = m[8]; o[1] = m[9]; o[2] = m[10]; o[3] = m[11]; o[4] = m[12];
+            }
+        }
+    }
+}
+
+template <typename TI, int TX, bool PF, bool px, bool py>
+__device__ __forceinline__ void pk_tile(const PkProb &P, int stage_bufs, const double *ctab, TI *stage, int q, int gb,
+                                        int txl, int tyl)
+{
+    const int tid = threadIdx.x;
+    const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
+    const int NL = ng * K;
+    const int gl = tid / K, k = tid - gl * K;
+    const int g0 = gb * P.gblk;
+    const int g = g0 + gl;
+    const bool act = (gl < P.gblk) && (g < ng);
+    const int L = act ? g * K + k : 0;
+    const int m = q * K + (act ? k : 0);
+    const double sox = P.ox[m], soy = P.oy[m];
+    const double axn = fabs(sox), ayn = fabs(soy);
+    constexpr int pmask = (px ? 0 : 1) | (py ? 0 : 2);
+    constexpr int perm[4] = {pmask == 0 ? 0 : pmask == 1 ? 1 : pmask == 2 ? 3 : 2,
+                             pmask == 0 ? 1 : pmask == 1 ? 0 : pmask == 2 ? 2 : 3,
+                             pmask == 0 ? 2 : pmask == 1 ? 3 : pmask == 2 ? 1 : 0,
+                             pmask == 0 ? 3 : pmask == 1 ? 2 : pmask == 2 ? 0 : 1};
+    const TI *Iold = static_cast<const TI *>(P.Iold);
+    TI *Inew = static_cast<TI *>(P.Inew);
+    TI *XV = static_cast<TI *>(P.XV);
+    TI *XH = static_cast<TI *>(P.XH);
+    const TI inv_cdt = (TI)P.inv_cdt;
+    const int i0 = txl * TX, iw = min(TX, nx - i0);
+    const int j0 = tyl * P.TY, jh = min(P.TY, ny - j0);
+
+    TI hl[TX], hr[TX];
+    {
+        const TI bcy = act ? (TI)P.bcin[(py ? 1 : 3) * ng + g] : (TI)0;
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            hl[ii] = bcy; hr[ii] = bcy;
+            if (tyl > 0 && ii < iw && act) {
+                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
+                hl[ii] = XH[base];
+                hr[ii] = XH[base + NL];
+            }
+        }
+    }
+    const TI bcx = act ? (TI)P.bcin[(px ? 0 : 2) * ng + g] : (TI)0;
+
+    V4<TI> io[TX];
+    double2 scv[TX];
+    auto load_row = [&](int jl, V4<TI> (&dst)[TX], double2 (&sdst)[TX]) {
+        const int j = py ? jl : ny - 1 - jl;
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (ii < iw && act) {
+                const int il = i0 + ii;
+                const int i = px ? il : nx - 1 - il;
+                const size_t c = (size_t)i + (size_t)nx * j;
+                dst[ii] = V4<TI>::load(Iold + ((c * 4 + q) * NL + L) * 4);
+                sdst[ii] = P.sc[c * ng + g];
+            } else {
+                dst[ii].v[0] = dst[ii].v[1] = dst[ii].v[2] = dst[ii].v[3] = (TI)0;
+                sdst[ii] = make_double2(1.0, 0.0);
+            }
+        }
+    };
+    if (PF) load_row(j0, io, scv);
+
+    for (int r = 0; r < jh; ++r) {
+        const int jl = j0 + r;
+        const int j = py ? jl : ny - 1 - jl;
+        TI vb = bcx, vt = bcx;
+        if (txl > 0 && act) {
+            const size_t base = ((size_t)(q * (P.NTX - 1) + txl - 1) * (2 * ny) + 2 * jl) * NL + L;
+            vb = XV[base];
+            vt = XV[base + NL];
+        }
+        V4<TI> ion[TX];
+        double2 scn[TX];
+        if (PF) {
+            if (r + 1 < jh) load_row(jl + 1, ion, scn);
+        } else {
+            load_row(jl, io, scv);
+        }
+        const double dyc = P.dy[j];
+        TI *st = stage + (size_t)(stage_bufs == 2 ? (r & 1) : 0) * 3 * TX * PK_THREADS;
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (ii >= iw) continue;
+            const int il = i0 + ii;
+            const int i = px ? il : nx - 1 - il;
+            const size_t c = (size_t)i + (size_t)nx * j;
+            const double dxc = P.dx[i];
+            const TI A = (TI)(dxc * dyc);
+            const TI A4 = A * (TI)0.25;
+            const TI aa = (TI)(axn * dyc * 0.25), bb = (TI)(ayn * dxc * 0.25);
+            const TI kt = (TI)scv[ii].x, src = (TI)scv[ii].y;
+            const TI gam = ((TI)(axn * dyc) + (TI)(ayn * dxc) + kt * A) * (TI)0.25;
+            const TI s0 = src + io[ii].v[perm[0]] * inv_cdt;
+            const TI s1 = src + io[ii].v[perm[1]] * inv_cdt;
+            const TI s2 = src + io[ii].v[perm[2]] * inv_cdt;
+            const TI s3 = src + io[ii].v[perm[3]] * inv_cdt;
+            const TI Q0 = A4 * s0 + (TI)2 * aa * vb + (TI)2 * bb * hl[ii];
+            const TI Q1 = A4 * s1 + (TI)2 * bb * hr[ii];
+            const TI Q2 = A4 * s2;
+            const TI Q3 = A4 * s3 + (TI)2 * aa * vt;
+            const TI g2 = gam * gam;
+            const TI al = g2 + aa * aa + bb * bb, be = (TI)2 * aa * bb;
+            const TI y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
+            const TI y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
+            const TI dm = aa - bb, dp = aa + bb;
+            const TI rD = (TI)1 / ((g2 + dm * dm) * (g2 + dp * dp));
+            const TI x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
+            const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
+            const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
+            const TI x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
+            if (act) {
+                V4<TI> o;
+                o.v[perm[0]] = x0; o.v[perm[1]] = x1; o.v[perm[2]] = x2; o.v[perm[3]] = x3;
+                o.store(Inew + ((c * 4 + q) * NL + L) * 4);
+            }
+            vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
+            // corner averages for COLLAPSE_INTENSITIES (:447-470): cell, downwind V and H faces
+            st[(0 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25);
+            st[(1 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x1 + (double)x2) * 0.5);
+            st[(2 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x3 + (double)x2) * 0.5);
+        }
+        if (txl < P.NTX - 1 && act) {
+            const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * jl) * NL + L;
+            XV[base] = vb;
+            XV[base + NL] = vt;
+        }
+        __syncthreads();
+        reduce_row<TI, TX, px, py>(P, st, ctab, q, g0, i0, iw, j);
+        if (stage_bufs == 1) __syncthreads();
+        if (PF) {
+#pragma unroll
+            for (int ii = 0; ii < TX; ++ii) { io[ii] = ion[ii]; scv[ii] = scn[ii]; }
+        }
+    }
+    if (tyl < P.NTY - 1 && act) {
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (ii < iw) {
+                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
+                XH[base] = hl[ii];
+                XH[base + NL] = hr[ii];
+            }
+        }
+    }
+}
+
+template <typename TI, int TX, bool PF>
+__global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
+{
+    extern __shared__ double dyn[];
+    __shared__ int s_next;
+    const int tid = threadIdx.x;
+    const int epoch = a.queue[1];
+    if (tid == 0) s_next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    int cur = __builtin_amdgcn_readfirstlane(s_next);
+    while (cur < a.nitems) {
+        int nxt = 0;
+        if (tid == 0) nxt = atomicAdd(a.queue, 1);
+        const int4 it = a.items[cur];
+        const int pi = __builtin_amdgcn_readfirstlane(it.x);
+        const int fl = __builtin_amdgcn_readfirstlane(it.y);
+        const int tpos = __builtin_amdgcn_readfirstlane(it.z);
+        const int qg = __builtin_amdgcn_readfirstlane(it.w);
+        const int txl = tpos & 0xffff, tyl = tpos >> 16, q = qg & 0xff, gb = qg >> 8;
+        const PkProb &P = a.probs[pi];
+        double *ctab = dyn;
+        TI *stage = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        if (tid == 0) {
+            if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, a.err);
+            if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, a.err);
+        }
+        for (int t = tid; t < P.K; t += PK_THREADS) {
+            const int mm = q * P.K + t;
+            const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
+            double *cc = ctab + t * 6;
+            cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        switch (q) {
+        case 0: pk_tile<TI, TX, PF, true, true>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+        case 1: pk_tile<TI, TX, PF, false, true>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+        case 2: pk_tile<TI, TX, PF, false, false>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+        default: pk_tile<TI, TX, PF, true, false>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (tid == 0) s_next = nxt;
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        cur = __builtin_amdgcn_readfirstlane(s_next);
+    }
+}
+
+// One launch per tile diagonal: the tiles of diagonal d of every quadrant and
+// group block, with the same tile body as the persistent kernel.
+template <typename TI, int TX, bool PF>
+__global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, int d, int tx_lo, int nt_d, int nblk_g,
+                                                         int stage_bufs)
+{
+    extern __shared__ double dyn[];
+    int bid = blockIdx.x;
+    const int gb = bid % nblk_g;
+    bid /= nblk_g;
+    const int t = bid % nt_d;
+    const int q = bid / nt_d;
+    const int txl = tx_lo + t, tyl = d - txl;
+    const PkProb &P = probs[0];
+    double *ctab = dyn;
+    TI *stage = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+    for (int k = threadIdx.x; k < P.K; k += PK_THREADS) {
+        const int mm = q * P.K + k;
+        const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
+        double *cc = ctab + k * 6;
+        cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
+    }
+    __syncthreads();
+    switch (q) {
+    case 0: pk_tile<TI, TX, PF, true, true>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+    case 1: pk_tile<TI, TX, PF, false, true>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+    case 2: pk_tile<TI, TX, PF, false, false>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+    default: pk_tile<TI, TX, PF, true, false>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
+    }
+}
+
+__global__ void pk_reset(int *queue)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        queue[0] = 0;
+        queue[1] = queue[1] + 1;
+    }
+}
+
+// Incoming boundary faces: their moments are those of the boundary intensities
+// (RT_BC_UPDATE, MainSrc/UPDATES.f08:12-105), the same for every face of a side.
+template <typename TI>
+__global__ void pk_boundary_moments(const PkProb *probs)
+{
+    const PkProb &P = probs[0];
+    const int nx = P.nx, ny = P.ny, ng = P.ng, K = P.K;
+    const int per = ny + nx;
+    const int tot = 4 * ng * per;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+        const int q = t / (ng * per);
+        const int r = t - q * ng * per;
+        const int g = r / per, f = r - g * per;
+        const bool px = (q == 0 || q == 3), py = (q == 0 || q == 1);
+        if (f < ny) {
+            const int j = f;
+            const double b = (double)(TI)P.bcin[(px ? 0 : 2) * ng + g] + P.eps;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < K; ++k) {
+                const int m = q * K + k;
+                const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
+                s[0] += wm * ox * ox * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * b;
+            }
+            const size_t fv = (size_t)(px ? 0 : nx) + (size_t)(nx + 1) * j;
+            double *o = P.PV + ((fv * 4 + q) * ng + g) * 4;
+            o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3];
+        } else {
+            const int i = f - ny;
+            const double b = (double)(TI)P.bcin[(py ? 1 : 3) * ng + g] + P.eps;
+            double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < K; ++k) {
+                const int m = q * K + k;
+                const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
+                s[0] += wm * oy * oy * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * oy * b;
+                s[4] += wm * b;
+            }
+            const size_t fh = (size_t)i + (size_t)nx * (py ? 0 : ny);
+            double *o = P.PH + ((fh * 4 + q) * ng + g) * 5;
+            o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3]; o[4] = s[4];
+        }
+    }
+}
+
+template <typename TI>
+constexpr bool pk_prefetch(int TX) { return sizeof(TI) == 4 ? TX <= 8 : TX <= 4; }
+
+template <typename TI, int TX>
+const void *pk_kernel()
+{
+    if (pk_prefetch<TI>(TX)) return reinterpret_cast<const void *>(&sweep_pk<TI, TX, true>);
+    return reinterpret_cast<const void *>(&sweep_pk<TI, TX, false>);
+}
+
+template <typename TI, int TX>
+const void *diag_kernel()
+{
+    if (pk_prefetch<TI>(TX)) return reinterpret_cast<const void *>(&sweep_diag<TI, TX, true>);
+    return reinterpret_cast<const void *>(&sweep_diag<TI, TX, false>);
+}
+
+const void *diag_kernel_for(int precision, int TX)
+{
+    if (precision) {
+        switch (TX) {
+        case 2: return diag_kernel<float, 2>();
+        case 4: return diag_kernel<float, 4>();
+        default: return diag_kernel<float, 8>();
+        }
+    }
+    switch (TX) {
+    case 2: return diag_kernel<double, 2>();
+    case 4: return diag_kernel<double, 4>();
+    default: return diag_kernel<double, 8>();
+    }
+}
+
+const void *pk_kernel_for(int precision, int TX)
+{
+    if (precision) {
+        switch (TX) {
+        case 2: return pk_kernel<float, 2>();
+        case 4: return pk_kernel<float, 4>();
+        default: return pk_kernel<float, 8>();
+        }
+    }
+    switch (TX) {
+    case 2: return pk_kernel<double, 2>();
+    case 4: return pk_kernel<double, 4>();
+    default: return pk_kernel<double, 8>();
+    }
+}
+
+}  // namespace
+
+// Host-side state of the persistent sweep of one context.
+struct PkState {
+    PkProb *probs = nullptr;     // [2]: table for I_crn_old in buffer 0 / buffer 1
+    int4 *items = nullptr;
+    int nitems = 0;
+    int *queue = nullptr;        // [0] counter, [1] epoch, [2] error word
+    int *flags = nullptr;
+    size_t lds = 0;
+    int stage_bufs = 1;
+    int grid = 0;
+    const void *kern = nullptr;
+    const void *dkern = nullptr;   // per-diagonal form
+};
+
+void pk_destroy(Ctx &c)
+{
+    delete c.pk;
+    c.pk = nullptr;
+}
+
+bool pk_supported(const Ctx &c)
+{
+    const int TX = c.sg.TX;
+    return TX == 2 || TX == 4 || TX == 8;
+}
+
+static void pk_setup(Ctx &c)
+{
+    const SweepGeom &sg = c.sg;
+    PkState *pk = new PkState();
+    c.pk = pk;
+    const size_t ti = c.precision ? 4 : 8;
+    const int TX = sg.TX;
+    // moment reduction shape
+    PkProb P{};
+    P.nx = c.nx; P.ny = c.ny; P.ng = c.ng; P.K = sg.K; P.gblk = sg.gblk; P.TY = sg.TY; P.NTX = sg.NTX; P.NTY = sg.NTY;
+    P.npair = TX * sg.gblk;
+    int S = 8;
+    while (S > 1 && (S * P.npair > PK_THREADS || S > sg.K)) S >>= 1;
+    P.S = S;
+    P.ks = (sg.K + S - 1) / S;
+    P.inv_cdt = 1.0 / (c.p.c * c.p.dt);
+    P.eps = 1e-25 / c.p.comp_unit;
+    P.dx = c.dx; P.dy = c.dy; P.ox = c.ox; P.oy = c.oy; P.w = c.w;
+    P.sc = reinterpret_cast<const double2 *>(c.sc);
+    P.bcin = c.bcin;
+    P.XV = c.XV; P.XH = c.XH;
+    P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
+    PkProb tab[2] = {P, P};
+    tab[0].Iold = c.Icrn[0]; tab[0].Inew = c.Icrn[1];
+    tab[1].Iold = c.Icrn[1]; tab[1].Inew = c.Icrn[0];
+    pk->probs = static_cast<PkProb *>(c.dalloc(sizeof(tab)));
+    TRT_CHECK(hipMemcpy(pk->probs, tab, sizeof(tab), hipMemcpyHostToDevice));
+
+    // items in tile-diagonal order
+    std::vector<int4> items;
+    const int nchain = 4 * sg.nblk_g;
+    items.reserve((size_t)nchain * sg.NTX * sg.NTY);
+    for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
+        const int lo = std::max(0, d - (sg.NTY - 1)), hi = std::min(d, sg.NTX - 1);
+        for (int txl = lo; txl <= hi; ++txl) {
+            const int tyl = d - txl;
+            for (int q = 0; q < 4; ++q)
+                for (int gb = 0; gb < sg.nblk_g; ++gb) {
+                    const int flag = ((q * sg.nblk_g + gb) * sg.NTY + tyl) * sg.NTX + txl;
+                    items.push_back(make_int4(0, flag, txl | (tyl << 16), q | (gb << 8)));
+                }
+        }
+    }
+    pk->nitems = (int)items.size();
+    pk->items = static_cast<int4 *>(c.dalloc(items.size() * sizeof(int4)));
+    TRT_CHECK(hipMemcpy(pk->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    pk->flags = static_cast<int *>(c.dalloc((size_t)nchain * sg.NTX * sg.NTY * sizeof(int)));
+    TRT_CHECK(hipMemset(pk->flags, 0, (size_t)nchain * sg.NTX * sg.NTY * sizeof(int)));
+    pk->queue = static_cast<int *>(c.dalloc(4 * sizeof(int)));
+    TRT_CHECK(hipMemset(pk->queue, 0, 4 * sizeof(int)));
+
+    const size_t one = (size_t)3 * TX * PK_THREADS * ti;
+    pk->stage_bufs = (2 * one <= 48 * 1024) ? 2 : 1;
+    pk->lds = (size_t)sg.K * 6 * 8 + pk->stage_bufs * one;
+    pk->kern = pk_kernel_for(c.precision, TX);
+    pk->dkern = diag_kernel_for(c.precision, TX);
+    int dev = 0, ncu = 0, per_cu = 0;
+    TRT_CHECK(hipGetDevice(&dev));
+    TRT_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TRT_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk->kern, PK_THREADS, pk->lds));
+    if (per_cu < 1) throw Error("persistent sweep kernel does not fit on a compute unit");
+    pk->grid = std::max(1, std::min(pk->nitems, per_cu * ncu));
+}
+
+// Enqueues one sweep: work-list reset, boundary-face moments, persistent kernel.
+// With ev, (*ev)[0] and (*ev)[1] bracket the persistent kernel alone.
+void pk_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
+{
+    if (!c.pk) pk_setup(c);
+    PkState &pk = *c.pk;
+    const PkProb *tab = pk.probs + c.iold;
+    hipLaunchKernelGGL(pk_reset, dim3(1), dim3(64), 0, c.stream, pk.queue);
+    const int nbt = 4 * c.ng * (c.nx + c.ny);
+    if (c.precision)
+        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    else
+        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    PkArgs a{};
+    a.probs = tab;
+    a.items = pk.items;
+    a.nitems = pk.nitems;
+    a.queue = pk.queue;
+    a.flags = pk.flags;
+    a.err = pk.queue + 2;
+    a.spin_limit = 1 << 20;
+    a.stage_bufs = pk.stage_bufs;
+    void *args[] = {&a};
+    if (ev) TRT_CHECK(hipEventRecord((*ev)[0], c.stream));
+    TRT_CHECK(hipLaunchKernel(pk.kern, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
+    if (ev) TRT_CHECK(hipEventRecord((*ev)[1], c.stream));
+}
+
+// The same sweep as one launch per tile diagonal (no cross-workgroup waits:
+// kernel boundaries order the tiles).  With ev, (*ev)[2d], (*ev)[2d+1] bracket
+// the launch of diagonal d.
+void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
+{
+    if (!c.pk) pk_setup(c);
+    PkState &pk = *c.pk;
+    const SweepGeom &sg = c.sg;
+    const PkProb *tab = pk.probs + c.iold;
+    const int nbt = 4 * c.ng * (c.nx + c.ny);
+    if (c.precision)
+        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    else
+        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
+        int tx_lo = std::max(0, d - (sg.NTY - 1));
+        const int tx_hi = std::min(d, sg.NTX - 1);
+        int nt_d = tx_hi - tx_lo + 1;
+        int nblk_g = sg.nblk_g, sb = pk.stage_bufs;
+        void *args[] = {&tab, &d, &tx_lo, &nt_d, &nblk_g, &sb};
+        if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d], c.stream));
+        TRT_CHECK(hipLaunchKernel(pk.dkern, dim3(4 * nt_d * nblk_g), dim3(PK_THREADS), args, pk.lds, c.stream));
+        if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d + 1], c.stream));
+    }
+}
+
+// Reads and clears the persistent sweep's error word (synchronises the stream).
+void pk_check(Ctx &c)
+{
+    if (!c.pk) return;
+    int e = 0;
+    TRT_CHECK(hipMemcpyAsync(&e, c.pk->queue + 2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
+    if (e) {
+        const int zero = 0;
+        TRT_CHECK(hipMemcpy(c.pk->queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+        throw Error("persistent sweep: a dependency wait exceeded its spin limit");
+    }
+}
+
+int pk_grid(const Ctx &c) { return c.pk ? c.pk->grid : 0; }
+
+}  // namespace trt

@@ -51,7 +51,10 @@ struct QDSystems {
     size_t nrow = 0;              // unknowns per system: 3 nx ny + nx + ny
     double *coef = nullptr;       // [nsys][16][nx*ny]
     double *bnd = nullptr;        // [nsys][8][max(nx,ny)]
-    double *diag = nullptr;       // [nsys][nrow] Jacobi diagonal
+    double *diag = nullptr;       // [nsys][nrow] Jacobi diagonal of A S
+    double *cs = nullptr;         // [nsys][nrow] column scales S
+    double *ref = nullptr;        // [nsys][nrow] scale reference (when not the initial guess)
+    double *x0 = nullptr;         // [nsys][nrow] initial guess, kept for restarts
     double *b = nullptr;          // [nsys][nrow]
     double *x = nullptr;          // [nsys][nrow] solution (initial guess on entry)
     double *work = nullptr;       // 7 * nsys * nrow
@@ -60,6 +63,8 @@ struct QDSystems {
     int *flags = nullptr;         // per-system convergence flags (device)
     int red_blocks = 0;
 };
+
+struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
 
 struct Ctx {
     trt_problem p{};
@@ -92,6 +97,8 @@ struct Ctx {
     double *sc = nullptr;                  // [cell][g][2] = (KapE + 1/(c dt), RT_Src)
     double *Pavg = nullptr, *PV = nullptr, *PH = nullptr;   // per-quadrant partial moments
     size_t icrn_bytes = 0;
+    int sweep_mode = 3;                    // 1, 2, 3 as trt_problem.sweep_mode
+    PkState *pk = nullptr;
 
     // low-order systems
     QDSystems mg, grey;
@@ -134,10 +141,18 @@ int step(Ctx &c, int its_out[4]);
 void pull_icrn(Ctx &c, int which, double *host);
 void push_icrn(Ctx &c, int which, const double *host);
 void sweep_setup(Ctx &c);
+// persistent sweep (sweep_pk.hip)
+bool pk_supported(const Ctx &c);
+void pk_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
+void pk_check(Ctx &c);
+void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
+void pk_destroy(Ctx &c);
+int pk_grid(const Ctx &c);
 size_t sweep_bytes(const trt_problem &p, SweepGeom *out);
 void qd_alloc(Ctx &c, QDSystems &s, int nsys);
-// solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess
-void qd_solve(Ctx &c, QDSystems &s, int *its_out);
+// solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess.
+// The unknowns are scaled by |s.ref| (ref_given) or by |initial guess|.
+void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given);
 // device copy helper
 void dcopy(Ctx &c, double *dst, const double *src, size_t n);
 void dfill(Ctx &c, double *dst, double v, size_t n);

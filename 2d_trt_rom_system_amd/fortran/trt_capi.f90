@@ -21,6 +21,7 @@ MODULE TRT_CAPI
     INTEGER(C_INT) :: threads, precision, tile_x, tile_y
     REAL(C_DOUBLE) :: lin_rtol
     INTEGER(C_INT) :: lin_maxit
+    INTEGER(C_INT) :: sweep_mode
   END TYPE trt_problem
 
   ! the context of the running problem, created by TRT_GPU_INIT
@@ -234,7 +235,7 @@ SUBROUTINE TRT_GPU_INIT(Omega_x, Omega_y, quad_weight, Nu_g, Delx, Dely, Mat, Ka
   p%use_safety_search = MERGE(1, 0, Use_Safety_Search)
   p%kapE_dT_flag = MERGE(1, 0, kapE_dT_flag)
   p%E_bound_low = E_Bound_Low; p%T_bound_low = T_Bound_Low
-  p%threads = 1; p%precision = precision; p%tile_x = 0; p%tile_y = 0; p%lin_rtol = 0d0; p%lin_maxit = 0
+  p%threads = 1; p%precision = precision; p%tile_x = 0; p%tile_y = 0; p%lin_rtol = 0d0; p%lin_maxit = 0; p%sweep_mode = 0
   CALL TRT_CHECK(trt_ctx_create(TRT_CTX, p), 'trt_ctx_create')
   DEALLOCATE(mat0, ox, oy, w, nu, dx, dy, km)
 

@@ -33,7 +33,7 @@ class TrtProblem(C.Structure):
         ("E_bound_low", C.c_double), ("T_bound_low", C.c_double),
         ("threads", C.c_int),
         ("precision", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
-        ("lin_rtol", C.c_double), ("lin_maxit", C.c_int),
+        ("lin_rtol", C.c_double), ("lin_maxit", C.c_int), ("sweep_mode", C.c_int),
     ]
 
 
@@ -91,7 +91,8 @@ def load(path: str = LIBPATH):
     return L
 
 
-def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0):
+def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0,
+                        sweep_mode: int = 0):
     """Fill a TrtProblem from a problem.Problem; returns (struct, keep-alive list)."""
     keep = []
 
@@ -124,6 +125,7 @@ def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float =
     s.precision = precision
     s.tile_x, s.tile_y = tile
     s.lin_rtol, s.lin_maxit = lin_rtol, lin_maxit
+    s.sweep_mode = sweep_mode
     return s, keep
 
 
@@ -136,19 +138,20 @@ def calib_copy(nbytes: int, reps: int = 10) -> float:
     return ms.value
 
 
-def device_bytes(prob, precision: int = 0, tile=(0, 0)) -> int:
-    s, keep = make_problem_struct(prob, precision, tile)
+def device_bytes(prob, precision: int = 0, tile=(0, 0), sweep_mode: int = 0) -> int:
+    s, keep = make_problem_struct(prob, precision, tile, sweep_mode=sweep_mode)
     return int(load().trt_ctx_bytes(C.byref(s)))
 
 
 class Context:
     """One device-resident problem (a trt_ctx)."""
 
-    def __init__(self, prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0):
+    def __init__(self, prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0,
+                 sweep_mode: int = 0):
         self.L = load()
         self.prob = prob
         self.precision = precision
-        s, keep = make_problem_struct(prob, precision, tile, lin_rtol, lin_maxit)
+        s, keep = make_problem_struct(prob, precision, tile, lin_rtol, lin_maxit, sweep_mode)
         h = C.c_void_p()
         rc = self.L.trt_ctx_create(C.byref(h), C.byref(s))
         if rc != 0:

@@ -73,6 +73,8 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = None
         L.orc_egp_fv_newt.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_lin_noise.argtypes = [C.c_double]
+        L.orc_set_lin_noise.restype = None
         L.orc_egp_fv_newt.restype = C.c_int
         L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_int * 4)]
         L.orc_step.restype = C.c_int
@@ -250,3 +252,8 @@ def qd_fv(nx, ny, coef, bnd):
     if err:
         raise RuntimeError(f"orc_qd_fv failed: {err}")
     return Ea.reshape((nx, ny), order="F"), Ev.reshape((nx + 1, ny), order="F"), Eh.reshape((nx, ny + 1), order="F")
+
+
+def set_lin_noise(eps: float) -> None:
+    """Sensitivity studies: perturb every QD linear solution by relative noise eps (0 = off)."""
+    lib().orc_set_lin_noise(float(eps))

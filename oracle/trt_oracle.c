@@ -26,6 +26,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <stdint.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -786,6 +787,22 @@ static int band_solve(int n, const int *ia, const int *ja, const double *a, doub
     return 0;
 }
 
+/* Sensitivity studies only (0 = off, the default): after each direct solve, every
+ * unknown is multiplied by 1 + eps*u, u in [-1,1) a hash of its index and value.
+ * This emulates a linear solve stopped at relative accuracy eps, as the
+ * reference's BiCGSTAB (QS:437-445, rtol 1e-13) is. */
+static double g_lin_noise = 0.0;
+void orc_set_lin_noise(double eps) { g_lin_noise = eps; }
+
+static double hash_unit(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
 int orc_qd_fv(int nx, int ny, const double *const coef[16], const double *const bnd[8],
               double *E_avg, double *E_edgV, double *E_edgH)
 {
@@ -797,6 +814,13 @@ int orc_qd_fv(int nx, int ny, const double *const coef[16], const double *const 
     double *sol = (double *)malloc(sizeof(double) * (size_t)nrows);
     orc_qd_build_csr(nx, ny, coef, bnd, ia, ja, a, sol);
     const int err = band_solve(nrows, ia, ja, a, sol);
+    if (err == 0 && g_lin_noise > 0.0) {
+        for (int r = 0; r < nrows; ++r) {
+            uint64_t bits;
+            memcpy(&bits, &sol[r], sizeof bits);
+            sol[r] *= 1.0 + g_lin_noise * hash_unit(bits ^ ((uint64_t)r << 40));
+        }
+    }
     if (err == 0) {
         /* collecting the solution, QS:512-534 */
         E_edgV[IX2(0, 0, nx + 1)] = sol[0];
