@@ -284,8 +284,11 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     sg.gblk = SWEEP_THREADS / sg.K;
     if (sg.gblk > p.ng) sg.gblk = p.ng;
     sg.nblk_g = (p.ng + sg.gblk - 1) / sg.gblk;
+    // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x4 (FP32) / 4x2 (FP64),
+    // the largest tiles whose loads fit in registers and staged averages in 48 KiB of LDS
+    const bool engine = p.sweep_mode != 1;
     sg.TX = p.tile_x > 0 ? p.tile_x : 4;
-    sg.TY = p.tile_y > 0 ? p.tile_y : 4;
+    sg.TY = p.tile_y > 0 ? p.tile_y : (engine && !p.precision ? 2 : 4);
     sg.NTX = (p.nx + sg.TX - 1) / sg.TX;
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }
@@ -314,7 +317,8 @@ void sweep_setup(Ctx &c)
     c.sweep_mode = c.p.sweep_mode == 0 ? 3 : c.p.sweep_mode;
     if (c.sweep_mode == 1 && sg.TX != 4 && sg.TX != 8 && sg.TX != 16)
         throw Error("tile_x must be 4, 8 or 16 for sweep_mode 1");
-    if (c.sweep_mode >= 2 && !pk_supported(c)) throw Error("tile_x must be 2, 4 or 8 for sweep_mode 2 and 3");
+    if (c.sweep_mode >= 2 && !pk_supported(c))
+        throw Error("sweep_mode 2 and 3 take tiles 2x2, 2x4, 4x2 (FP64) and also 4x4, 8x2, 2x8 (FP32)");
     const size_t nc = (size_t)c.nx * c.ny, NL = (size_t)c.ng * sg.K;
     const size_t ti = c.precision ? 4 : 8;
     c.icrn_bytes = nc * 4 * NL * 4 * ti;

@@ -1,27 +1,38 @@
-// K1 (persistent form): the SCB sweep of TRANSPORT_SCB (MainSrc/TRANSPORT_SOLVES.f08:14-327)
-// with the angular sums of COLLAPSE_INTENSITIES (:381-511) as ONE launch per sweep.
+// K1 (tile engine): the SCB sweep of TRANSPORT_SCB (MainSrc/TRANSPORT_SOLVES.f08:14-327)
+// with the angular sums of COLLAPSE_INTENSITIES (:381-511), in two launch forms
+// that share one tile body:
 //
-// Work items are (problem, quadrant, group block, tile); an item depends on the
-// tiles left of and below it in its quadrant's reflected frame.  Items are
-// listed by tile diagonal, so the list is a topological order, and resident
-// workgroups take them from an atomic counter.  A workgroup waits (bounded
-// spin) until the two upwind tiles have published their faces, sweeps its tile
-// and publishes its own completion with an agent-scope release.  Because every
-// item a workgroup waits for was taken earlier by a running workgroup, the grid
-// size never matters for progress.  Completion flags carry the sweep's epoch,
-// so they are never cleared between sweeps.
+//  * persistent (sweep_mode 2): ONE launch per sweep.  Work items are
+//    (quadrant, group block, tile); an item depends on the tiles left of and
+//    below it in its quadrant's reflected frame.  Items are listed by tile
+//    diagonal, so the list is a topological order, and resident workgroups
+//    take them from an atomic counter.  Every item a workgroup waits for was
+//    taken earlier by a running workgroup, so the grid size never matters for
+//    progress.  A workgroup waits (bounded spin) for the two upwind tiles'
+//    completion flags, which carry the sweep's epoch and are never cleared.
+//  * per-diagonal (sweep_mode 3): one launch per tile diagonal; kernel
+//    boundaries order the tiles.
 //
-// Per tile, a thread owns one (quadrant, group, direction) and walks the rows
-// of a TX x TY tile with the upwind corner-edge intensities in registers, one
-// row of old intensities prefetched ahead.  After each row the block stages the
-// corner averages of every (cell, direction) in LDS and reduces them over the
-// directions of each group: S adjacent lanes per (cell, group) pair each sum
-// ceil(K/S) directions against a per-quadrant table of weighted direction
-// cosines and combine with cross-lane shuffles, so a row costs one block
-// barrier (double-buffered staging) instead of two per cell.
+// Tile body.  A thread owns one (quadrant, group, direction).  At the start of
+// a tile it issues every load the tile needs -- the old corner intensities and
+// sweep coefficients of all TX x TY cells and the incoming faces -- so a tile
+// costs one memory round trip, then solves the cells in upwind order with the
+// corner-edge intensities in registers.  The corner averages of every (cell,
+// direction) are staged in LDS; after one block barrier the block sums them
+// over the directions of each group (S adjacent lanes per (cell, group) pair,
+// ceil(K/S) directions each, against a per-quadrant table of weighted
+// direction cosines, combined with cross-lane shuffles) and writes the FP64
+// quadrant partial moments.
+//
+// Faces between tiles go through exchange buffers with agent-coherent (sc1)
+// stores and loads.  The producer waits for its stores to complete
+// (s_waitcnt vmcnt(0)) before the barrier after which one lane publishes the
+// completion flag, and the consumer reads the faces only after it has seen the
+// flag; no L2 writeback or invalidate is needed for the rest of the data,
+// which no other workgroup of the launch reads.
 //
 // The corner solve, the reflection to the first quadrant and the intensity
-// layout are those of the per-diagonal kernel in sweep.hip.
+// layout [cell][quadrant][group][direction][corner] are those of sweep.hip.
 #include "trt_internal.h"
 
 #include <algorithm>
@@ -34,8 +45,8 @@ constexpr int PK_THREADS = 256;
 constexpr int NMOM = 13;
 
 struct PkProb {
-    int nx, ny, ng, K, gblk, TY, NTX, NTY;
-    int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per row
+    int nx, ny, ng, K, gblk, NTX, NTY;
+    int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per tile
     double inv_cdt, eps;
     const double *dx, *dy, *ox, *oy, *w;
     const double2 *sc;     // [cell][g] (KapE + 1/(c dt), RT_Src)
@@ -50,11 +61,9 @@ struct PkArgs {
     const PkProb *probs;
     const int4 *items;     // {problem, own flag, txl | tyl << 16, q | gb << 8}
     int nitems;
-    int *queue;            // [0] next item, [1] epoch
+    int *queue;            // [0] next item, [1] epoch, [2] error word
     int *flags;
-    int *err;
     int spin_limit;
-    int stage_bufs;        // 1 or 2 LDS staging buffers
 };
 
 template <typename T> struct V4;
@@ -81,6 +90,15 @@ template <> struct V4<float> {
     }
 };
 
+// face exchange: agent-coherent accesses
+template <typename T>
+__device__ __forceinline__ T xload(const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T>
+__device__ __forceinline__ void xstore(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// s_waitcnt vmcnt(0): every global load and store of this wave has completed
+__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Bounded wait for an upwind tile.  A wait that runs out sets the error word, and
 // every later wait of the launch sees it and returns at once, so a broken
 // dependency ends the launch within one spin limit instead of hanging it.
@@ -98,32 +116,186 @@ __device__ __forceinline__ void wait_flag(const int *f, int epoch, int limit, in
     }
 }
 
-// Sum the staged corner averages of one row over the directions of each group and
-// write the quadrant partial moments of the row's cells and downwind faces.
-template <typename TS, int TX, bool px, bool py>
-__device__ __forceinline__ void reduce_row(const PkProb &P, const TS *st, const double *ctab, int q, int g0, int i0,
-                                           int iw, int j)
+__device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q)
+{
+    for (int t = threadIdx.x; t < P.K; t += PK_THREADS) {
+        const int mm = q * P.K + t;
+        const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
+        double *cc = ctab + t * 6;
+        cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
+    }
+}
+
+// Sweep one TX x TY tile of quadrant q (signs px, py) for group block gb; stage
+// the corner averages (cell, downwind V face, downwind H face) in st.
+template <typename TI, int TX, int TY, bool px, bool py>
+__device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl)
 {
     const int tid = threadIdx.x;
-    const int S = P.S, K = P.K, ng = P.ng, nx = P.nx, gblk = P.gblk;
+    const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
+    const int NL = ng * K;
+    const int gl = tid / K, k = tid - gl * K;
+    const int g = gb * P.gblk + gl;
+    const bool act = (gl < P.gblk) && (g < ng);
+    const int L = act ? g * K + k : 0;
+    const int m = q * K + k;
+    const double axn = fabs(P.ox[m]), ayn = fabs(P.oy[m]);
+    constexpr int pmask = (px ? 0 : 1) | (py ? 0 : 2);
+    constexpr int perm[4] = {pmask == 0 ? 0 : pmask == 1 ? 1 : pmask == 2 ? 3 : 2,
+                             pmask == 0 ? 1 : pmask == 1 ? 0 : pmask == 2 ? 2 : 3,
+                             pmask == 0 ? 2 : pmask == 1 ? 3 : pmask == 2 ? 1 : 0,
+                             pmask == 0 ? 3 : pmask == 1 ? 2 : pmask == 2 ? 0 : 1};
+    const TI *Iold = static_cast<const TI *>(P.Iold);
+    TI *Inew = static_cast<TI *>(P.Inew);
+    TI *XV = static_cast<TI *>(P.XV);
+    TI *XH = static_cast<TI *>(P.XH);
+    const TI inv_cdt = (TI)P.inv_cdt;
+    const int i0 = txl * TX, iw = min(TX, nx - i0);
+    const int j0 = tyl * TY, jh = min(TY, ny - j0);
+    const int gg = act ? g : 0;
+
+    // ---- every load of the tile, issued up front ----
+    V4<TI> io[TY][TX];
+    TI kt[TY][TX], sr[TY][TX];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (r < jh && ii < iw && act) {
+                const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+                const size_t c = (size_t)i + (size_t)nx * j;
+                io[r][ii] = V4<TI>::load(Iold + ((c * 4 + q) * NL + L) * 4);
+                const double2 s = P.sc[c * ng + gg];
+                kt[r][ii] = (TI)s.x;
+                sr[r][ii] = (TI)s.y;
+            } else {
+                io[r][ii].v[0] = io[r][ii].v[1] = io[r][ii].v[2] = io[r][ii].v[3] = (TI)0;
+                kt[r][ii] = (TI)1;
+                sr[r][ii] = (TI)0;
+            }
+        }
+    }
+    const TI bcx = (TI)P.bcin[(px ? 0 : 2) * ng + gg];
+    const TI bcy = (TI)P.bcin[(py ? 1 : 3) * ng + gg];
+    TI vbr[TY], vtr[TY], hl[TX], hr[TX];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        vbr[r] = bcx; vtr[r] = bcx;
+        if (txl > 0 && r < jh && act) {
+            const size_t base = ((size_t)(q * (P.NTX - 1) + txl - 1) * (2 * ny) + 2 * (j0 + r)) * NL + L;
+            vbr[r] = xload(XV + base);
+            vtr[r] = xload(XV + base + NL);
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < TX; ++ii) {
+        hl[ii] = bcy; hr[ii] = bcy;
+        if (tyl > 0 && ii < iw && act) {
+            const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
+            hl[ii] = xload(XH + base);
+            hr[ii] = xload(XH + base + NL);
+        }
+    }
+
+    // ---- cells in upwind order ----
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        if (r >= jh) continue;
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+        const double dyc = P.dy[j];
+        TI vb = vbr[r], vt = vtr[r];
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (ii >= iw) continue;
+            const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+            const size_t c = (size_t)i + (size_t)nx * j;
+            const double dxc = P.dx[i];
+            const TI A = (TI)(dxc * dyc);
+            const TI A4 = A * (TI)0.25;
+            const TI aa = (TI)(axn * dyc * 0.25), bb = (TI)(ayn * dxc * 0.25);
+            const TI gam = ((TI)(axn * dyc) + (TI)(ayn * dxc) + kt[r][ii] * A) * (TI)0.25;
+            const TI src = sr[r][ii];
+            // S = RT_Src + I_old / (c dt)  (TRANSPORT_SOLVES.f08:48); Q = B x (:270-294)
+            const TI s0 = src + io[r][ii].v[perm[0]] * inv_cdt;
+            const TI s1 = src + io[r][ii].v[perm[1]] * inv_cdt;
+            const TI s2 = src + io[r][ii].v[perm[2]] * inv_cdt;
+            const TI s3 = src + io[r][ii].v[perm[3]] * inv_cdt;
+            const TI Q0 = A4 * s0 + (TI)2 * aa * vb + (TI)2 * bb * hl[ii];
+            const TI Q1 = A4 * s1 + (TI)2 * bb * hr[ii];
+            const TI Q2 = A4 * s2;
+            const TI Q3 = A4 * s3 + (TI)2 * aa * vt;
+            const TI g2 = gam * gam;
+            const TI al = g2 + aa * aa + bb * bb, be = (TI)2 * aa * bb;
+            const TI y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
+            const TI y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
+            const TI dm = aa - bb, dp = aa + bb;
+            const TI rD = (TI)1 / ((g2 + dm * dm) * (g2 + dp * dp));
+            const TI x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
+            const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
+            const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
+            const TI x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
+            if (act) {
+                V4<TI> o;
+                o.v[perm[0]] = x0; o.v[perm[1]] = x1; o.v[perm[2]] = x2; o.v[perm[3]] = x3;
+                o.store(Inew + ((c * 4 + q) * NL + L) * 4);
+            }
+            // outgoing corner edges (:86-89)
+            vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
+            // corner averages for COLLAPSE_INTENSITIES (:447-470)
+            const int cidx = r * TX + ii;
+            st[(0 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25);
+            st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x1 + (double)x2) * 0.5);
+            st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x3 + (double)x2) * 0.5);
+        }
+        if (txl < P.NTX - 1 && act) {
+            const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * (j0 + r)) * NL + L;
+            xstore(XV + base, vb);
+            xstore(XV + base + NL, vt);
+        }
+    }
+    if (tyl < P.NTY - 1 && act) {
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (ii < iw) {
+                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
+                xstore(XH + base, hl[ii]);
+                xstore(XH + base + NL, hr[ii]);
+            }
+        }
+    }
+}
+
+// Sum the staged corner averages of a tile over the directions of each group and
+// write the quadrant partial moments of its cells and downwind faces.
+template <typename TS, int TX, int TY, bool px, bool py>
+__device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const double *ctab, int q, int gb, int txl,
+                                            int tyl)
+{
+    const int tid = threadIdx.x;
+    const int S = P.S, K = P.K, ng = P.ng, nx = P.nx, ny = P.ny, gblk = P.gblk;
     const int s = tid & (S - 1), pofs = tid / S, pstride = PK_THREADS / S;
+    const int i0 = txl * TX, iw = min(TX, nx - i0);
+    const int j0 = tyl * TY, jh = min(TY, ny - j0);
+    const int g0 = gb * gblk;
     const double eps = P.eps;
     for (int pb = 0; pb < P.npair; pb += pstride) {
         const int p = pb + pofs;
-        const int ii = p / gblk, gl = p - ii * gblk;
+        const int cidx = p / gblk, gl = p - cidx * gblk;
+        const int r = cidx / TX, ii = cidx - r * TX;
         const int g = g0 + gl;
-        const bool ok = p < P.npair && ii < iw && g < ng;
+        const bool ok = p < P.npair && ii < iw && r < jh && g < ng;
         double m[NMOM];
 #pragma unroll
         for (int v = 0; v < NMOM; ++v) m[v] = 0.0;
         if (ok) {
             const int k0 = s * P.ks, k1 = min(K, k0 + P.ks);
-            const TS *sa = st + (0 * TX + ii) * PK_THREADS + gl * K;
-            const TS *sv = st + (1 * TX + ii) * PK_THREADS + gl * K;
-            const TS *sh = st + (2 * TX + ii) * PK_THREADS + gl * K;
-            for (int k = k0; k < k1; ++k) {
-                const double ia = (double)sa[k] + eps, iv = (double)sv[k] + eps, ih = (double)sh[k] + eps;
-                const double *cc = ctab + k * 6;   // w, w ox^2, w oy^2, w ox oy, w ox, w oy
+            const TS *sa = st + (0 * TX * TY + cidx) * PK_THREADS + gl * K;
+            const TS *sv = st + (1 * TX * TY + cidx) * PK_THREADS + gl * K;
+            const TS *sh = st + (2 * TX * TY + cidx) * PK_THREADS + gl * K;
+            for (int kk = k0; kk < k1; ++kk) {
+                const double ia = (double)sa[kk] + eps, iv = (double)sv[kk] + eps, ih = (double)sh[kk] + eps;
+                const double *cc = ctab + kk * 6;   // w, w ox^2, w oy^2, w ox oy, w ox, w oy
                 m[0] += cc[1] * ia; m[1] += cc[2] * ia; m[2] += cc[3] * ia; m[3] += cc[0] * ia;
                 m[4] += cc[1] * iv; m[5] += cc[3] * iv; m[6] += cc[4] * iv; m[7] += cc[0] * iv;
                 m[8] += cc[2] * ih; m[9] += cc[3] * ih; m[10] += cc[4] * ih; m[11] += cc[5] * ih;
@@ -135,8 +307,8 @@ __device__ __forceinline__ void reduce_row(const PkProb &P, const TS *st, const 
             for (int v = 0; v < NMOM; ++v) m[v] += __shfl_xor(m[v], off);
         }
         if (ok) {
-            const int il = i0 + ii;
-            const int i = px ? il : nx - 1 - il;
+            const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+            const int j = py ? j0 + r : ny - 1 - (j0 + r);
             const size_t c = (size_t)i + (size_t)nx * j;
             const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
             const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
@@ -158,160 +330,37 @@ __device__ __forceinline__ void reduce_row(const PkProb &P, const TS *st, const 
     }
 }
 
-template <typename TI, int TX, bool PF, bool px, bool py>
-__device__ __forceinline__ void pk_tile(const PkProb &P, int stage_bufs, const double *ctab, TI *stage, int q, int gb,
-                                        int txl, int tyl)
+template <typename TI, int TX, int TY>
+__device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl)
 {
-    const int tid = threadIdx.x;
-    const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
-    const int NL = ng * K;
-    const int gl = tid / K, k = tid - gl * K;
-    const int g0 = gb * P.gblk;
-    const int g = g0 + gl;
-    const bool act = (gl < P.gblk) && (g < ng);
-    const int L = act ? g * K + k : 0;
-    const int m = q * K + (act ? k : 0);
-    const double sox = P.ox[m], soy = P.oy[m];
-    const double axn = fabs(sox), ayn = fabs(soy);
-    constexpr int pmask = (px ? 0 : 1) | (py ? 0 : 2);
-    constexpr int perm[4] = {pmask == 0 ? 0 : pmask == 1 ? 1 : pmask == 2 ? 3 : 2,
-                             pmask == 0 ? 1 : pmask == 1 ? 0 : pmask == 2 ? 2 : 3,
-                             pmask == 0 ? 2 : pmask == 1 ? 3 : pmask == 2 ? 1 : 0,
-                             pmask == 0 ? 3 : pmask == 1 ? 2 : pmask == 2 ? 0 : 1};
-    const TI *Iold = static_cast<const TI *>(P.Iold);
-    TI *Inew = static_cast<TI *>(P.Inew);
-    TI *XV = static_cast<TI *>(P.XV);
-    TI *XH = static_cast<TI *>(P.XH);
-    const TI inv_cdt = (TI)P.inv_cdt;
-    const int i0 = txl * TX, iw = min(TX, nx - i0);
-    const int j0 = tyl * P.TY, jh = min(P.TY, ny - j0);
-
-    TI hl[TX], hr[TX];
-    {
-        const TI bcy = act ? (TI)P.bcin[(py ? 1 : 3) * ng + g] : (TI)0;
-#pragma unroll
-        for (int ii = 0; ii < TX; ++ii) {
-            hl[ii] = bcy; hr[ii] = bcy;
-            if (tyl > 0 && ii < iw && act) {
-                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-                hl[ii] = XH[base];
-                hr[ii] = XH[base + NL];
-            }
-        }
-    }
-    const TI bcx = act ? (TI)P.bcin[(px ? 0 : 2) * ng + g] : (TI)0;
-
-    V4<TI> io[TX];
-    double2 scv[TX];
-    auto load_row = [&](int jl, V4<TI> (&dst)[TX], double2 (&sdst)[TX]) {
-        const int j = py ? jl : ny - 1 - jl;
-#pragma unroll
-        for (int ii = 0; ii < TX; ++ii) {
-            if (ii < iw && act) {
-                const int il = i0 + ii;
-                const int i = px ? il : nx - 1 - il;
-                const size_t c = (size_t)i + (size_t)nx * j;
-                dst[ii] = V4<TI>::load(Iold + ((c * 4 + q) * NL + L) * 4);
-                sdst[ii] = P.sc[c * ng + g];
-            } else {
-                dst[ii].v[0] = dst[ii].v[1] = dst[ii].v[2] = dst[ii].v[3] = (TI)0;
-                sdst[ii] = make_double2(1.0, 0.0);
-            }
-        }
-    };
-    if (PF) load_row(j0, io, scv);
-
-    for (int r = 0; r < jh; ++r) {
-        const int jl = j0 + r;
-        const int j = py ? jl : ny - 1 - jl;
-        TI vb = bcx, vt = bcx;
-        if (txl > 0 && act) {
-            const size_t base = ((size_t)(q * (P.NTX - 1) + txl - 1) * (2 * ny) + 2 * jl) * NL + L;
-            vb = XV[base];
-            vt = XV[base + NL];
-        }
-        V4<TI> ion[TX];
-        double2 scn[TX];
-        if (PF) {
-            if (r + 1 < jh) load_row(jl + 1, ion, scn);
-        } else {
-            load_row(jl, io, scv);
-        }
-        const double dyc = P.dy[j];
-        TI *st = stage + (size_t)(stage_bufs == 2 ? (r & 1) : 0) * 3 * TX * PK_THREADS;
-#pragma unroll
-        for (int ii = 0; ii < TX; ++ii) {
-            if (ii >= iw) continue;
-            const int il = i0 + ii;
-            const int i = px ? il : nx - 1 - il;
-            const size_t c = (size_t)i + (size_t)nx * j;
-            const double dxc = P.dx[i];
-            const TI A = (TI)(dxc * dyc);
-            const TI A4 = A * (TI)0.25;
-            const TI aa = (TI)(axn * dyc * 0.25), bb = (TI)(ayn * dxc * 0.25);
-            const TI kt = (TI)scv[ii].x, src = (TI)scv[ii].y;
-            const TI gam = ((TI)(axn * dyc) + (TI)(ayn * dxc) + kt * A) * (TI)0.25;
-            const TI s0 = src + io[ii].v[perm[0]] * inv_cdt;
-            const TI s1 = src + io[ii].v[perm[1]] * inv_cdt;
-            const TI s2 = src + io[ii].v[perm[2]] * inv_cdt;
-            const TI s3 = src + io[ii].v[perm[3]] * inv_cdt;
-            const TI Q0 = A4 * s0 + (TI)2 * aa * vb + (TI)2 * bb * hl[ii];
-            const TI Q1 = A4 * s1 + (TI)2 * bb * hr[ii];
-            const TI Q2 = A4 * s2;
-            const TI Q3 = A4 * s3 + (TI)2 * aa * vt;
-            const TI g2 = gam * gam;
-            const TI al = g2 + aa * aa + bb * bb, be = (TI)2 * aa * bb;
-            const TI y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
-            const TI y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
-            const TI dm = aa - bb, dp = aa + bb;
-            const TI rD = (TI)1 / ((g2 + dm * dm) * (g2 + dp * dp));
-            const TI x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
-            const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
-            const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
-            const TI x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
-            if (act) {
-                V4<TI> o;
-                o.v[perm[0]] = x0; o.v[perm[1]] = x1; o.v[perm[2]] = x2; o.v[perm[3]] = x3;
-                o.store(Inew + ((c * 4 + q) * NL + L) * 4);
-            }
-            vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
-            // corner averages for COLLAPSE_INTENSITIES (:447-470): cell, downwind V and H faces
-            st[(0 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25);
-            st[(1 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x1 + (double)x2) * 0.5);
-            st[(2 * TX + ii) * PK_THREADS + tid] = (TI)(((double)x3 + (double)x2) * 0.5);
-        }
-        if (txl < P.NTX - 1 && act) {
-            const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * jl) * NL + L;
-            XV[base] = vb;
-            XV[base + NL] = vt;
-        }
-        __syncthreads();
-        reduce_row<TI, TX, px, py>(P, st, ctab, q, g0, i0, iw, j);
-        if (stage_bufs == 1) __syncthreads();
-        if (PF) {
-#pragma unroll
-            for (int ii = 0; ii < TX; ++ii) { io[ii] = ion[ii]; scv[ii] = scn[ii]; }
-        }
-    }
-    if (tyl < P.NTY - 1 && act) {
-#pragma unroll
-        for (int ii = 0; ii < TX; ++ii) {
-            if (ii < iw) {
-                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-                XH[base] = hl[ii];
-                XH[base + NL] = hr[ii];
-            }
-        }
+    switch (q) {
+    case 0: tile_body<TI, TX, TY, true, true>(P, st, q, gb, txl, tyl); break;
+    case 1: tile_body<TI, TX, TY, false, true>(P, st, q, gb, txl, tyl); break;
+    case 2: tile_body<TI, TX, TY, false, false>(P, st, q, gb, txl, tyl); break;
+    default: tile_body<TI, TX, TY, true, false>(P, st, q, gb, txl, tyl); break;
     }
 }
 
-template <typename TI, int TX, bool PF>
+template <typename TI, int TX, int TY>
+__device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, const double *ctab, int q, int gb, int txl,
+                                            int tyl)
+{
+    switch (q) {
+    case 0: reduce_tile<TI, TX, TY, true, true>(P, st, ctab, q, gb, txl, tyl); break;
+    case 1: reduce_tile<TI, TX, TY, false, true>(P, st, ctab, q, gb, txl, tyl); break;
+    case 2: reduce_tile<TI, TX, TY, false, false>(P, st, ctab, q, gb, txl, tyl); break;
+    default: reduce_tile<TI, TX, TY, true, false>(P, st, ctab, q, gb, txl, tyl); break;
+    }
+}
+
+template <typename TI, int TX, int TY>
 __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 {
     extern __shared__ double dyn[];
     __shared__ int s_next;
     const int tid = threadIdx.x;
     const int epoch = a.queue[1];
+    int *err = a.queue + 2;
     if (tid == 0) s_next = atomicAdd(a.queue, 1);
     __syncthreads();
     int cur = __builtin_amdgcn_readfirstlane(s_next);
@@ -326,38 +375,29 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
         const int txl = tpos & 0xffff, tyl = tpos >> 16, q = qg & 0xff, gb = qg >> 8;
         const PkProb &P = a.probs[pi];
         double *ctab = dyn;
-        TI *stage = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
         if (tid == 0) {
-            if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, a.err);
-            if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, a.err);
+            if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
+            if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
         }
-        for (int t = tid; t < P.K; t += PK_THREADS) {
-            const int mm = q * P.K + t;
-            const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
-            double *cc = ctab + t * 6;
-            cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
-        }
+        load_ctab(P, ctab, q);
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        switch (q) {
-        case 0: pk_tile<TI, TX, PF, true, true>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-        case 1: pk_tile<TI, TX, PF, false, true>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-        case 2: pk_tile<TI, TX, PF, false, false>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-        default: pk_tile<TI, TX, PF, true, false>(P, a.stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (tid == 0) s_next = nxt;
+        tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl);
+        wait_vm_stores();                 // this wave's face stores have completed
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_next = nxt;
+        }
+        tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+        __syncthreads();
         cur = __builtin_amdgcn_readfirstlane(s_next);
     }
 }
 
-// One launch per tile diagonal: the tiles of diagonal d of every quadrant and
-// group block, with the same tile body as the persistent kernel.
-template <typename TI, int TX, bool PF>
-__global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, int d, int tx_lo, int nt_d, int nblk_g,
-                                                         int stage_bufs)
+// One launch per tile diagonal d: the tiles of d of every quadrant and group block.
+template <typename TI, int TX, int TY>
+__global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, int d, int tx_lo, int nt_d, int nblk_g)
 {
     extern __shared__ double dyn[];
     int bid = blockIdx.x;
@@ -368,20 +408,11 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, in
     const int txl = tx_lo + t, tyl = d - txl;
     const PkProb &P = probs[0];
     double *ctab = dyn;
-    TI *stage = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
-    for (int k = threadIdx.x; k < P.K; k += PK_THREADS) {
-        const int mm = q * P.K + k;
-        const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
-        double *cc = ctab + k * 6;
-        cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
-    }
+    TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+    load_ctab(P, ctab, q);
+    tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl);
     __syncthreads();
-    switch (q) {
-    case 0: pk_tile<TI, TX, PF, true, true>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-    case 1: pk_tile<TI, TX, PF, false, true>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-    case 2: pk_tile<TI, TX, PF, false, false>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-    default: pk_tile<TI, TX, PF, true, false>(P, stage_bufs, ctab, stage, q, gb, txl, tyl); break;
-    }
+    tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
 }
 
 __global__ void pk_reset(int *queue)
@@ -435,58 +466,34 @@ __global__ void pk_boundary_moments(const PkProb *probs)
     }
 }
 
-template <typename TI>
-constexpr bool pk_prefetch(int TX) { return sizeof(TI) == 4 ? TX <= 8 : TX <= 4; }
+// Tile shapes of the tile engine: the staged corner averages of a tile
+// (3 TX TY 256 values) fit 48 KiB of LDS and the tile's loads fit in registers.
+struct TileKernels {
+    int tx, ty, precision;
+    const void *pk, *diag;
+};
 
-template <typename TI, int TX>
-const void *pk_kernel()
+template <typename TI, int TX, int TY>
+TileKernels tk(int precision)
 {
-    if (pk_prefetch<TI>(TX)) return reinterpret_cast<const void *>(&sweep_pk<TI, TX, true>);
-    return reinterpret_cast<const void *>(&sweep_pk<TI, TX, false>);
+    return TileKernels{TX, TY, precision, reinterpret_cast<const void *>(&sweep_pk<TI, TX, TY>),
+                       reinterpret_cast<const void *>(&sweep_diag<TI, TX, TY>)};
 }
 
-template <typename TI, int TX>
-const void *diag_kernel()
+const TileKernels *find_tile_kernels(int precision, int tx, int ty)
 {
-    if (pk_prefetch<TI>(TX)) return reinterpret_cast<const void *>(&sweep_diag<TI, TX, true>);
-    return reinterpret_cast<const void *>(&sweep_diag<TI, TX, false>);
-}
-
-const void *diag_kernel_for(int precision, int TX)
-{
-    if (precision) {
-        switch (TX) {
-        case 2: return diag_kernel<float, 2>();
-        case 4: return diag_kernel<float, 4>();
-        default: return diag_kernel<float, 8>();
-        }
-    }
-    switch (TX) {
-    case 2: return diag_kernel<double, 2>();
-    case 4: return diag_kernel<double, 4>();
-    default: return diag_kernel<double, 8>();
-    }
-}
-
-const void *pk_kernel_for(int precision, int TX)
-{
-    if (precision) {
-        switch (TX) {
-        case 2: return pk_kernel<float, 2>();
-        case 4: return pk_kernel<float, 4>();
-        default: return pk_kernel<float, 8>();
-        }
-    }
-    switch (TX) {
-    case 2: return pk_kernel<double, 2>();
-    case 4: return pk_kernel<double, 4>();
-    default: return pk_kernel<double, 8>();
-    }
+    static const TileKernels table[] = {
+        tk<float, 2, 2>(1), tk<float, 2, 4>(1), tk<float, 4, 2>(1), tk<float, 4, 4>(1), tk<float, 8, 2>(1),
+        tk<float, 2, 8>(1), tk<double, 2, 2>(0), tk<double, 2, 4>(0), tk<double, 4, 2>(0),
+    };
+    for (const TileKernels &t : table)
+        if (t.tx == tx && t.ty == ty && t.precision == precision) return &t;
+    return nullptr;
 }
 
 }  // namespace
 
-// Host-side state of the persistent sweep of one context.
+// Host-side state of the tile engine of one context.
 struct PkState {
     PkProb *probs = nullptr;     // [2]: table for I_crn_old in buffer 0 / buffer 1
     int4 *items = nullptr;
@@ -494,10 +501,8 @@ struct PkState {
     int *queue = nullptr;        // [0] counter, [1] epoch, [2] error word
     int *flags = nullptr;
     size_t lds = 0;
-    int stage_bufs = 1;
     int grid = 0;
-    const void *kern = nullptr;
-    const void *dkern = nullptr;   // per-diagonal form
+    const TileKernels *k = nullptr;
 };
 
 void pk_destroy(Ctx &c)
@@ -506,23 +511,19 @@ void pk_destroy(Ctx &c)
     c.pk = nullptr;
 }
 
-bool pk_supported(const Ctx &c)
-{
-    const int TX = c.sg.TX;
-    return TX == 2 || TX == 4 || TX == 8;
-}
+bool pk_supported(const Ctx &c) { return find_tile_kernels(c.precision, c.sg.TX, c.sg.TY) != nullptr; }
 
 static void pk_setup(Ctx &c)
 {
     const SweepGeom &sg = c.sg;
     PkState *pk = new PkState();
     c.pk = pk;
+    pk->k = find_tile_kernels(c.precision, sg.TX, sg.TY);
+    if (!pk->k) throw Error("no tile kernel for this tile shape and precision");
     const size_t ti = c.precision ? 4 : 8;
-    const int TX = sg.TX;
-    // moment reduction shape
     PkProb P{};
-    P.nx = c.nx; P.ny = c.ny; P.ng = c.ng; P.K = sg.K; P.gblk = sg.gblk; P.TY = sg.TY; P.NTX = sg.NTX; P.NTY = sg.NTY;
-    P.npair = TX * sg.gblk;
+    P.nx = c.nx; P.ny = c.ny; P.ng = c.ng; P.K = sg.K; P.gblk = sg.gblk; P.NTX = sg.NTX; P.NTY = sg.NTY;
+    P.npair = sg.TX * sg.TY * sg.gblk;
     int S = 8;
     while (S > 1 && (S * P.npair > PK_THREADS || S > sg.K)) S >>= 1;
     P.S = S;
@@ -563,17 +564,22 @@ static void pk_setup(Ctx &c)
     pk->queue = static_cast<int *>(c.dalloc(4 * sizeof(int)));
     TRT_CHECK(hipMemset(pk->queue, 0, 4 * sizeof(int)));
 
-    const size_t one = (size_t)3 * TX * PK_THREADS * ti;
-    pk->stage_bufs = (2 * one <= 48 * 1024) ? 2 : 1;
-    pk->lds = (size_t)sg.K * 6 * 8 + pk->stage_bufs * one;
-    pk->kern = pk_kernel_for(c.precision, TX);
-    pk->dkern = diag_kernel_for(c.precision, TX);
+    pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
     int dev = 0, ncu = 0, per_cu = 0;
     TRT_CHECK(hipGetDevice(&dev));
     TRT_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    TRT_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk->kern, PK_THREADS, pk->lds));
-    if (per_cu < 1) throw Error("persistent sweep kernel does not fit on a compute unit");
+    TRT_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk->k->pk, PK_THREADS, pk->lds));
+    if (per_cu < 1) throw Error("sweep tile kernel does not fit on a compute unit");
     pk->grid = std::max(1, std::min(pk->nitems, per_cu * ncu));
+}
+
+static void boundary_moments(Ctx &c, const PkProb *tab)
+{
+    const int nbt = 4 * c.ng * (c.nx + c.ny);
+    if (c.precision)
+        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    else
+        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
 }
 
 // Enqueues one sweep: work-list reset, boundary-face moments, persistent kernel.
@@ -584,48 +590,37 @@ void pk_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     PkState &pk = *c.pk;
     const PkProb *tab = pk.probs + c.iold;
     hipLaunchKernelGGL(pk_reset, dim3(1), dim3(64), 0, c.stream, pk.queue);
-    const int nbt = 4 * c.ng * (c.nx + c.ny);
-    if (c.precision)
-        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
-    else
-        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    boundary_moments(c, tab);
     PkArgs a{};
     a.probs = tab;
     a.items = pk.items;
     a.nitems = pk.nitems;
     a.queue = pk.queue;
     a.flags = pk.flags;
-    a.err = pk.queue + 2;
     a.spin_limit = 1 << 20;
-    a.stage_bufs = pk.stage_bufs;
     void *args[] = {&a};
     if (ev) TRT_CHECK(hipEventRecord((*ev)[0], c.stream));
-    TRT_CHECK(hipLaunchKernel(pk.kern, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
+    TRT_CHECK(hipLaunchKernel(pk.k->pk, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
     if (ev) TRT_CHECK(hipEventRecord((*ev)[1], c.stream));
 }
 
-// The same sweep as one launch per tile diagonal (no cross-workgroup waits:
-// kernel boundaries order the tiles).  With ev, (*ev)[2d], (*ev)[2d+1] bracket
-// the launch of diagonal d.
+// The same sweep as one launch per tile diagonal.  With ev, (*ev)[2d], (*ev)[2d+1]
+// bracket the launch of diagonal d.
 void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
 {
     if (!c.pk) pk_setup(c);
     PkState &pk = *c.pk;
     const SweepGeom &sg = c.sg;
     const PkProb *tab = pk.probs + c.iold;
-    const int nbt = 4 * c.ng * (c.nx + c.ny);
-    if (c.precision)
-        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
-    else
-        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    boundary_moments(c, tab);
     for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
         int tx_lo = std::max(0, d - (sg.NTY - 1));
         const int tx_hi = std::min(d, sg.NTX - 1);
         int nt_d = tx_hi - tx_lo + 1;
-        int nblk_g = sg.nblk_g, sb = pk.stage_bufs;
-        void *args[] = {&tab, &d, &tx_lo, &nt_d, &nblk_g, &sb};
+        int nblk_g = sg.nblk_g;
+        void *args[] = {&tab, &d, &tx_lo, &nt_d, &nblk_g};
         if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d], c.stream));
-        TRT_CHECK(hipLaunchKernel(pk.dkern, dim3(4 * nt_d * nblk_g), dim3(PK_THREADS), args, pk.lds, c.stream));
+        TRT_CHECK(hipLaunchKernel(pk.k->diag, dim3(4 * nt_d * nblk_g), dim3(PK_THREADS), args, pk.lds, c.stream));
         if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d + 1], c.stream));
     }
 }

@@ -9,20 +9,21 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # (nx, ny, S_N, ng, tile, sweep_mode): 1 = per-diagonal launches, per-cell moment reduction;
-    # 2 = one persistent launch; 3 = per-diagonal launches, per-row moment reduction
+    # 2 = one persistent launch; 3 = per-diagonal launches of the tile engine
     (12, 10, 4, 3, (8, 8), 1),
     (13, 11, 8, 5, (4, 4), 1),
     (9, 17, 12, 2, (16, 8), 1),
     (20, 20, 16, 3, (8, 4), 1),
-    (12, 10, 4, 3, (4, 4), 2),
+    (12, 10, 4, 3, (4, 2), 2),
     (13, 11, 8, 5, (2, 4), 2),
-    (9, 17, 12, 2, (8, 8), 2),
-    (20, 20, 16, 3, (4, 8), 2),
-    (11, 9, 16, 9, (4, 4), 2),     # two group blocks of 7 groups
-    (10, 7, 4, 40, (8, 2), 2),     # 320 (cell, group) pairs per row: two reduction passes
+    (9, 17, 12, 2, (2, 2), 2),
+    (20, 20, 16, 3, (4, 2), 2),
+    (11, 9, 16, 9, (2, 4), 2),     # two group blocks of 7 groups
+    (10, 7, 4, 40, (4, 2), 2),     # 320 (cell, group) pairs per tile: two reduction passes
 ]
-# mode 3: per-diagonal launches of the persistent kernel's tile body
 CASES += [c[:5] + (3,) for c in CASES if c[5] == 2]
+# tile shapes only the FP32 tile engine has
+FP32_CASES = [(13, 11, 8, 5, (4, 4), 3), (20, 20, 16, 3, (8, 2), 2), (10, 7, 4, 40, (2, 8), 3)]
 
 
 def _randomized_pair(trt, oracle_mod, nx, ny, order, ng, tile, precision=0, seed=1, mode=0):
@@ -70,10 +71,10 @@ def test_sweep_fp64_matches_oracle(trt, oracle_mod, nx, ny, order, ng, tile, mod
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
-def test_two_sweeps_same_step_reuse_old(trt, oracle_mod, mode):
+@pytest.mark.parametrize("mode,tile", [(1, (4, 8)), (2, (2, 4)), (3, (2, 4))])
+def test_two_sweeps_same_step_reuse_old(trt, oracle_mod, mode, tile):
     """Two sweeps in one step both read I_crn_old (ALGORITHMS.f08:541 inside the RT loop)."""
-    prob, o, ctx = _randomized_pair(trt, oracle_mod, 10, 12, 8, 2, (4, 8), mode=mode)
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 10, 12, 8, 2, tile, mode=mode)
     o.call("transport_scb")
     ctx.call("transport_scb")
     o.call("transport_scb")
@@ -83,7 +84,7 @@ def test_two_sweeps_same_step_reuse_old(trt, oracle_mod, mode):
     ctx.close()
 
 
-@pytest.mark.parametrize("nx,ny,order,ng,tile,mode", CASES[:2] + CASES[4:6] + CASES[8:10] + CASES[-2:])
+@pytest.mark.parametrize("nx,ny,order,ng,tile,mode", CASES[:2] + CASES[4:6] + CASES[-2:] + FP32_CASES)
 def test_sweep_fp32_within_tolerance(trt, oracle_mod, nx, ny, order, ng, tile, mode):
     prob, o, ctx = _randomized_pair(trt, oracle_mod, nx, ny, order, ng, tile, precision=1, mode=mode)
     o.call("transport_scb")
@@ -114,7 +115,7 @@ def test_equilibrium_sweep_is_exact(trt):
 
 def test_derived_intensity_views(trt, oracle_mod):
     """I_avg, I_edgV, I_edgH, Ic_edgV, Ic_edgH pulled from the device equal the reference arrays."""
-    prob, o, ctx = _randomized_pair(trt, oracle_mod, 7, 6, 8, 2, (4, 4))
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 7, 6, 8, 2, (0, 0))
     o.call("transport_scb")
     ctx.call("transport_scb")
     for name in ("I_avg", "I_edgV", "I_edgH", "Ic_edgV", "Ic_edgH"):
@@ -125,7 +126,7 @@ def test_derived_intensity_views(trt, oracle_mod):
 
 def test_push_kape_and_source_reaches_the_sweep(trt, oracle_mod):
     """Pushing KapE and RT_Src (as the Fortran drop-in does) feeds the next sweep."""
-    prob, o, ctx = _randomized_pair(trt, oracle_mod, 6, 5, 4, 2, (8, 8))
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 6, 5, 4, 2, (0, 0))
     rng = np.random.default_rng(9)
     ke = o["KapE"] * rng.uniform(0.5, 2.0, o["KapE"].shape)
     rs = o["RT_Src"] * rng.uniform(0.5, 2.0, o["RT_Src"].shape)
@@ -137,3 +138,27 @@ def test_push_kape_and_source_reaches_the_sweep(trt, oracle_mod):
     ctx.call("transport_scb")
     assert rel_err(ctx.pull("I_crn", o["I_crn"].shape), o["I_crn"]) < 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("precision,tile", [(0, (4, 2)), (0, (2, 2)), (1, (4, 4)), (1, (2, 8))])
+def test_persistent_equals_per_diagonal_bitwise(trt, precision, tile):
+    """The persistent launch and the per-diagonal launches run the same tile body on
+    the same data, so any stale or early read of an exchanged face would change bits."""
+    prob = trt.problem.small(nx=37, ny=29, order=8, ng=9, seed=5)
+    rng = np.random.default_rng(5)
+    T = rng.uniform(0.5, 800.0, size=(prob.nx, prob.ny))
+    outs = []
+    for mode in (2, 3):
+        ctx = trt.native.Context(prob, precision=precision, tile=tile, sweep_mode=mode)
+        ctx.push("Temp", T)
+        ctx.call("material_update")
+        res = []
+        for _ in range(3):
+            ctx.call("transport_scb")
+            ctx.call("collapse_intensities")
+            res.append([ctx.pull(n) for n in ("I_crn", "HO_Eg_avg", "HO_Fxg_edgV", "HO_Fyg_edgH", "Hg_avg_xy")])
+        outs.append(res)
+        ctx.close()
+    for a, b in zip(*outs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
