@@ -96,6 +96,18 @@ __device__ __forceinline__ T xload(const T *p) { return __hip_atomic_load(p, __A
 template <typename T>
 __device__ __forceinline__ void xstore(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// 1/x: hardware reciprocal, refined for FP64 by two Newton-Raphson steps
+// (r <- r + r (1 - x r)), each of which squares the relative error.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // s_waitcnt vmcnt(0): every global load and store of this wave has completed
 __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -198,23 +210,39 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         }
     }
 
+    // ---- geometry of the tile's rows and columns for this direction ----
+    // aa = |Omega_x| dy / 4, bb = |Omega_y| dx / 4, A/4 = dx dy / 4 (SCB_BGEN, :332-371)
+    TI aar[TY], dyr[TY], bbc[TX], dx4c[TX];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+        const double dyc = r < jh ? P.dy[j] : 0.0;
+        dyr[r] = (TI)dyc;
+        aar[r] = (TI)(axn * dyc * 0.25);
+    }
+#pragma unroll
+    for (int ii = 0; ii < TX; ++ii) {
+        const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+        const double dxc = ii < iw ? P.dx[i] : 0.0;
+        dx4c[ii] = (TI)(dxc * 0.25);
+        bbc[ii] = (TI)(ayn * dxc * 0.25);
+    }
+
     // ---- cells in upwind order ----
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
         if (r >= jh) continue;
         const int j = py ? j0 + r : ny - 1 - (j0 + r);
-        const double dyc = P.dy[j];
         TI vb = vbr[r], vt = vtr[r];
+        const TI aa = aar[r];
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             if (ii >= iw) continue;
             const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
             const size_t c = (size_t)i + (size_t)nx * j;
-            const double dxc = P.dx[i];
-            const TI A = (TI)(dxc * dyc);
-            const TI A4 = A * (TI)0.25;
-            const TI aa = (TI)(axn * dyc * 0.25), bb = (TI)(ayn * dxc * 0.25);
-            const TI gam = ((TI)(axn * dyc) + (TI)(ayn * dxc) + kt[r][ii] * A) * (TI)0.25;
+            const TI bb = bbc[ii];
+            const TI A4 = dx4c[ii] * dyr[r];
+            const TI gam = aa + bb + kt[r][ii] * A4;
             const TI src = sr[r][ii];
             // S = RT_Src + I_old / (c dt)  (TRANSPORT_SOLVES.f08:48); Q = B x (:270-294)
             const TI s0 = src + io[r][ii].v[perm[0]] * inv_cdt;
@@ -230,7 +258,7 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             const TI y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
             const TI y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
             const TI dm = aa - bb, dp = aa + bb;
-            const TI rD = (TI)1 / ((g2 + dm * dm) * (g2 + dp * dp));
+            const TI rD = recip((g2 + dm * dm) * (g2 + dp * dp));
             const TI x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
             const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
             const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
@@ -244,9 +272,9 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
             // corner averages for COLLAPSE_INTENSITIES (:447-470)
             const int cidx = r * TX + ii;
-            st[(0 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25);
-            st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x1 + (double)x2) * 0.5);
-            st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (TI)(((double)x3 + (double)x2) * 0.5);
+            st[(0 * TX * TY + cidx) * PK_THREADS + tid] = (x0 + x1 + x2 + x3) * (TI)0.25;
+            st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (x1 + x2) * (TI)0.5;
+            st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (x3 + x2) * (TI)0.5;
         }
         if (txl < P.NTX - 1 && act) {
             const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * (j0 + r)) * NL + L;
