@@ -16,6 +16,7 @@ struct MatArgs {
     const int *mat;
     double *Bg, *KapB, *KapE, *KapR, *MGQD_Src, *RT_Src;
     double2 *sc;
+    float2 *scf;   // FP32 copy for the FP32 sweep, or null
 };
 
 __global__ void material_kernel(MatArgs a)
@@ -36,6 +37,7 @@ __global__ void material_kernel(MatArgs a)
         a.MGQD_Src[idx] = 4.0 * PI * kap * bg;
         a.RT_Src[idx] = kap * bg;
         a.sc[c * a.ng + g] = make_double2(kap + a.inv_cdt, kap * bg);
+        if (a.scf) a.scf[c * a.ng + g] = make_float2((float)(kap + a.inv_cdt), (float)(kap * bg));
     }
 }
 
@@ -57,6 +59,7 @@ void material_update(Ctx &c)
     a.Bg = c.arr("Bg"); a.KapB = c.arr("KapB"); a.KapE = c.arr("KapE"); a.KapR = c.arr("KapR");
     a.MGQD_Src = c.arr("MGQD_Src"); a.RT_Src = c.arr("RT_Src");
     a.sc = reinterpret_cast<double2 *>(c.sc);
+    a.scf = reinterpret_cast<float2 *>(c.scf);
     const size_t n = (size_t)c.nx * c.ny * c.ng;
     hipLaunchKernelGGL(material_kernel, dim3(nblocks_for(n)), dim3(256), 0, c.stream, a);
     TRT_CHECK(hipGetLastError());

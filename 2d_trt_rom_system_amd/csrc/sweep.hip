@@ -284,11 +284,11 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     sg.gblk = SWEEP_THREADS / sg.K;
     if (sg.gblk > p.ng) sg.gblk = p.ng;
     sg.nblk_g = (p.ng + sg.gblk - 1) / sg.gblk;
-    // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x4 (FP32) / 4x2 (FP64),
-    // the largest tiles whose loads fit in registers and staged averages in 48 KiB of LDS
+    // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x2 (FP32) / 2x4 (FP64),
+    // the fastest measured on the 256x256, S16, 32-group problem
     const bool engine = p.sweep_mode != 1;
-    sg.TX = p.tile_x > 0 ? p.tile_x : 4;
-    sg.TY = p.tile_y > 0 ? p.tile_y : (engine && !p.precision ? 2 : 4);
+    sg.TX = p.tile_x > 0 ? p.tile_x : (engine && !p.precision ? 2 : 4);
+    sg.TY = p.tile_y > 0 ? p.tile_y : (engine ? (p.precision ? 2 : 4) : 4);
     sg.NTX = (p.nx + sg.TX - 1) / sg.TX;
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }
@@ -304,6 +304,7 @@ size_t sweep_bytes(const trt_problem &p, SweepGeom *out)
     b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * p.ny * NL * ti;         // XV
     b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * p.nx * NL * ti;         // XH
     b += nc * p.ng * 2 * 8;                                                // sc
+    if (p.precision) b += nc * p.ng * 2 * 4;                               // FP32 copy of sc
     b += (nc * 4 + (size_t)(p.nx + 1) * p.ny * 4 + (size_t)p.nx * (p.ny + 1) * 5) * 4 * p.ng * 8;
     return b;
 }
@@ -314,7 +315,7 @@ void sweep_setup(Ctx &c)
     const SweepGeom &sg = c.sg;
     if (c.nm % 4) throw Error("N_m must be a multiple of 4 (quadrant blocks)");
     if (sg.K > SWEEP_THREADS) throw Error("more than 256 directions per quadrant is not supported");
-    c.sweep_mode = c.p.sweep_mode == 0 ? 3 : c.p.sweep_mode;
+    c.sweep_mode = c.p.sweep_mode == 0 ? 2 : c.p.sweep_mode;
     if (c.sweep_mode == 1 && sg.TX != 4 && sg.TX != 8 && sg.TX != 16)
         throw Error("tile_x must be 4, 8 or 16 for sweep_mode 1");
     if (c.sweep_mode >= 2 && !pk_supported(c))
@@ -327,6 +328,7 @@ void sweep_setup(Ctx &c)
     c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * c.ny * NL * ti));
     c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * c.nx * NL * ti));
     c.sc = static_cast<double *>(c.dalloc(nc * c.ng * 2 * 8));
+    if (c.precision) c.scf = static_cast<float *>(c.dalloc(nc * c.ng * 2 * 4));
     c.Pavg = static_cast<double *>(c.dalloc(nc * 4 * c.ng * 4 * 8));
     c.PV = static_cast<double *>(c.dalloc((size_t)(c.nx + 1) * c.ny * 4 * c.ng * 4 * 8));
     c.PH = static_cast<double *>(c.dalloc((size_t)c.nx * (c.ny + 1) * 4 * c.ng * 5 * 8));
@@ -545,20 +547,23 @@ void pull_derived(Ctx &c, const char *name, double *host)
 }
 
 // (KapE + 1/(c dt), RT_Src) per (cell, group) for the sweep, from the named arrays
-__global__ void sc_kernel(const double *KE, const double *RS, double2 *sc, size_t nc, int ng, double inv_cdt)
+__global__ void sc_kernel(const double *KE, const double *RS, double2 *sc, float2 *scf, size_t nc, int ng,
+                          double inv_cdt)
 {
     const size_t n = nc * ng;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
         const size_t cc = idx % nc;
         const int g = (int)(idx / nc);
         sc[cc * ng + g] = make_double2(KE[idx] + inv_cdt, RS[idx]);
+        if (scf) scf[cc * ng + g] = make_float2((float)(KE[idx] + inv_cdt), (float)RS[idx]);
     }
 }
 
 void rebuild_sc(Ctx &c)
 {
     hipLaunchKernelGGL(sc_kernel, dim3(1024), dim3(256), 0, c.stream, c.arr("KapE"), c.arr("RT_Src"),
-                       reinterpret_cast<double2 *>(c.sc), (size_t)c.nx * c.ny, c.ng, 1.0 / (c.p.c * c.p.dt));
+                       reinterpret_cast<double2 *>(c.sc), reinterpret_cast<float2 *>(c.scf), (size_t)c.nx * c.ny, c.ng,
+                       1.0 / (c.p.c * c.p.dt));
     TRT_CHECK(hipGetLastError());
 }
 

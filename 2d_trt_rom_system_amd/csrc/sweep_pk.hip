@@ -49,7 +49,7 @@ struct PkProb {
     int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per tile
     double inv_cdt, eps;
     const double *dx, *dy, *ox, *oy, *w;
-    const double2 *sc;     // [cell][g] (KapE + 1/(c dt), RT_Src)
+    const void *sc;        // [cell][g] (KapE + 1/(c dt), RT_Src) in the sweep precision
     const double *bcin;    // [4][ng]
     const void *Iold;
     void *Inew;
@@ -66,35 +66,62 @@ struct PkArgs {
     int spin_limit;
 };
 
+// Pointers of the problem descriptor are generic once the descriptor has been
+// read from memory; accesses through them would be flat instructions, which the
+// compiler orders against LDS traffic with full waits.  The tile engine casts
+// them to the global address space at the point of use.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GLOBAL_AS T *gptr(T *p) { return (GLOBAL_AS T *)p; }
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef d2v type; };
+template <> struct Vec2<float> { typedef f2v type; };
+
 template <typename T> struct V4;
 template <> struct V4<double> {
     double v[4];
-    __device__ static V4 load(const double *p) {
-        const double2 a = *reinterpret_cast<const double2 *>(p);
-        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    __device__ static V4 load(GLOBAL_AS const double *p) {
+        const d2v a = *reinterpret_cast<GLOBAL_AS const d2v *>(p);
+        const d2v b = *reinterpret_cast<GLOBAL_AS const d2v *>(p + 2);
         V4 r; r.v[0] = a.x; r.v[1] = a.y; r.v[2] = b.x; r.v[3] = b.y; return r;
     }
-    __device__ void store(double *p) const {
-        *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+    __device__ void store(GLOBAL_AS double *p) const {
+        d2v a, b;
+        a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3];
+        *reinterpret_cast<GLOBAL_AS d2v *>(p) = a;
+        *reinterpret_cast<GLOBAL_AS d2v *>(p + 2) = b;
     }
 };
 template <> struct V4<float> {
     float v[4];
-    __device__ static V4 load(const float *p) {
-        const float4 a = *reinterpret_cast<const float4 *>(p);
+    __device__ static V4 load(GLOBAL_AS const float *p) {
+        const f4v a = *reinterpret_cast<GLOBAL_AS const f4v *>(p);
         V4 r; r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w; return r;
     }
-    __device__ void store(float *p) const {
-        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    __device__ void store(GLOBAL_AS float *p) const {
+        f4v a;
+        a.x = v[0]; a.y = v[1]; a.z = v[2]; a.w = v[3];
+        *reinterpret_cast<GLOBAL_AS f4v *>(p) = a;
     }
 };
 
+__device__ __forceinline__ void store4(GLOBAL_AS double *o, double a, double b, double c, double d)
+{
+    d2v x, y;
+    x.x = a; x.y = b; y.x = c; y.y = d;
+    *reinterpret_cast<GLOBAL_AS d2v *>(o) = x;
+    *reinterpret_cast<GLOBAL_AS d2v *>(o + 2) = y;
+}
+
 // face exchange: agent-coherent accesses
 template <typename T>
-__device__ __forceinline__ T xload(const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ T xload(GLOBAL_AS const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 template <typename T>
-__device__ __forceinline__ void xstore(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void xstore(GLOBAL_AS T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // 1/x: hardware reciprocal, refined for FP64 by two Newton-Raphson steps
 // (r <- r + r (1 - x r)), each of which squares the relative error.
@@ -132,16 +159,19 @@ __device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q)
 {
     for (int t = threadIdx.x; t < P.K; t += PK_THREADS) {
         const int mm = q * P.K + t;
-        const double wm = P.w[mm], ox = P.ox[mm], oy = P.oy[mm];
+        const double wm = gptr(P.w)[mm], ox = gptr(P.ox)[mm], oy = gptr(P.oy)[mm];
         double *cc = ctab + t * 6;
         cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
     }
 }
 
 // Sweep one TX x TY tile of quadrant q (signs px, py) for group block gb; stage
-// the corner averages (cell, downwind V face, downwind H face) in st.
-template <typename TI, int TX, int TY, bool px, bool py>
-__device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl)
+// the corner averages (cell, downwind V face, downwind H face) in st.  The loads
+// that do not depend on other tiles (old intensities, coefficients, geometry)
+// are issued before sync(), which waits for the upwind tiles; the faces those
+// tiles wrote are read after it.
+template <typename TI, int TX, int TY, bool px, bool py, typename Sync>
+__device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
 {
     const int tid = threadIdx.x;
     const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
@@ -151,22 +181,36 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     const bool act = (gl < P.gblk) && (g < ng);
     const int L = act ? g * K + k : 0;
     const int m = q * K + k;
-    const double axn = fabs(P.ox[m]), ayn = fabs(P.oy[m]);
+    const double axn = fabs(gptr(P.ox)[m]), ayn = fabs(gptr(P.oy)[m]);
     constexpr int pmask = (px ? 0 : 1) | (py ? 0 : 2);
     constexpr int perm[4] = {pmask == 0 ? 0 : pmask == 1 ? 1 : pmask == 2 ? 3 : 2,
                              pmask == 0 ? 1 : pmask == 1 ? 0 : pmask == 2 ? 2 : 3,
                              pmask == 0 ? 2 : pmask == 1 ? 3 : pmask == 2 ? 1 : 0,
                              pmask == 0 ? 3 : pmask == 1 ? 2 : pmask == 2 ? 0 : 1};
-    const TI *Iold = static_cast<const TI *>(P.Iold);
-    TI *Inew = static_cast<TI *>(P.Inew);
-    TI *XV = static_cast<TI *>(P.XV);
-    TI *XH = static_cast<TI *>(P.XH);
+    GLOBAL_AS const TI *Iold = gptr(static_cast<const TI *>(P.Iold));
+    GLOBAL_AS TI *Inew = gptr(static_cast<TI *>(P.Inew));
+    GLOBAL_AS TI *XV = gptr(static_cast<TI *>(P.XV));
+    GLOBAL_AS TI *XH = gptr(static_cast<TI *>(P.XH));
+    typedef typename Vec2<TI>::type TI2;
+    GLOBAL_AS const TI2 *scv = reinterpret_cast<GLOBAL_AS const TI2 *>(gptr(P.sc));
     const TI inv_cdt = (TI)P.inv_cdt;
     const int i0 = txl * TX, iw = min(TX, nx - i0);
     const int j0 = tyl * TY, jh = min(TY, ny - j0);
     const int gg = act ? g : 0;
 
-    // ---- every load of the tile, issued up front ----
+    // ---- every load of the tile, issued up front (the values are used after all
+    // loads have been issued, so the tile costs one memory round trip) ----
+    double dyv[TY], dxv[TX];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+        dyv[r] = r < jh ? gptr(P.dy)[j] : 0.0;
+    }
+#pragma unroll
+    for (int ii = 0; ii < TX; ++ii) {
+        const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+        dxv[ii] = ii < iw ? gptr(P.dx)[i] : 0.0;
+    }
     V4<TI> io[TY][TX];
     TI kt[TY][TX], sr[TY][TX];
 #pragma unroll
@@ -178,9 +222,9 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
                 const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
                 const size_t c = (size_t)i + (size_t)nx * j;
                 io[r][ii] = V4<TI>::load(Iold + ((c * 4 + q) * NL + L) * 4);
-                const double2 s = P.sc[c * ng + gg];
-                kt[r][ii] = (TI)s.x;
-                sr[r][ii] = (TI)s.y;
+                const TI2 s = scv[c * ng + gg];
+                kt[r][ii] = s.x;
+                sr[r][ii] = s.y;
             } else {
                 io[r][ii].v[0] = io[r][ii].v[1] = io[r][ii].v[2] = io[r][ii].v[3] = (TI)0;
                 kt[r][ii] = (TI)1;
@@ -188,8 +232,11 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             }
         }
     }
-    const TI bcx = (TI)P.bcin[(px ? 0 : 2) * ng + gg];
-    const TI bcy = (TI)P.bcin[(py ? 1 : 3) * ng + gg];
+    const TI bcx = (TI)gptr(P.bcin)[(px ? 0 : 2) * ng + gg];
+    const TI bcy = (TI)gptr(P.bcin)[(py ? 1 : 3) * ng + gg];
+
+    sync();
+
     TI vbr[TY], vtr[TY], hl[TX], hr[TX];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
@@ -215,17 +262,13 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     TI aar[TY], dyr[TY], bbc[TX], dx4c[TX];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
-        const int j = py ? j0 + r : ny - 1 - (j0 + r);
-        const double dyc = r < jh ? P.dy[j] : 0.0;
-        dyr[r] = (TI)dyc;
-        aar[r] = (TI)(axn * dyc * 0.25);
+        dyr[r] = (TI)dyv[r];
+        aar[r] = (TI)(axn * dyv[r] * 0.25);
     }
 #pragma unroll
     for (int ii = 0; ii < TX; ++ii) {
-        const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
-        const double dxc = ii < iw ? P.dx[i] : 0.0;
-        dx4c[ii] = (TI)(dxc * 0.25);
-        bbc[ii] = (TI)(ayn * dxc * 0.25);
+        dx4c[ii] = (TI)(dxv[ii] * 0.25);
+        bbc[ii] = (TI)(ayn * dxv[ii] * 0.25);
     }
 
     // ---- cells in upwind order ----
@@ -340,32 +383,24 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const
             const size_t c = (size_t)i + (size_t)nx * j;
             const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
             const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
-            if (s == 0) {
-                double *o = P.Pavg + ((c * 4 + q) * ng + g) * 4;
-                *reinterpret_cast<double2 *>(o) = make_double2(m[0], m[1]);
-                *reinterpret_cast<double2 *>(o + 2) = make_double2(m[2], m[3]);
-            }
-            if (s == (1 & (S - 1))) {
-                double *o = P.PV + ((fV * 4 + q) * ng + g) * 4;
-                *reinterpret_cast<double2 *>(o) = make_double2(m[4], m[5]);
-                *reinterpret_cast<double2 *>(o + 2) = make_double2(m[6], m[7]);
-            }
+            if (s == 0) store4(gptr(P.Pavg) + ((c * 4 + q) * ng + g) * 4, m[0], m[1], m[2], m[3]);
+            if (s == (1 & (S - 1))) store4(gptr(P.PV) + ((fV * 4 + q) * ng + g) * 4, m[4], m[5], m[6], m[7]);
             if (s == (2 & (S - 1))) {
-                double *o = P.PH + ((fH * 4 + q) * ng + g) * 5;
+                GLOBAL_AS double *o = gptr(P.PH) + ((fH * 4 + q) * ng + g) * 5;
                 o[0] = m[8]; o[1] = m[9]; o[2] = m[10]; o[3] = m[11]; o[4] = m[12];
             }
         }
     }
 }
 
-template <typename TI, int TX, int TY>
-__device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl)
+template <typename TI, int TX, int TY, typename Sync>
+__device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
 {
     switch (q) {
-    case 0: tile_body<TI, TX, TY, true, true>(P, st, q, gb, txl, tyl); break;
-    case 1: tile_body<TI, TX, TY, false, true>(P, st, q, gb, txl, tyl); break;
-    case 2: tile_body<TI, TX, TY, false, false>(P, st, q, gb, txl, tyl); break;
-    default: tile_body<TI, TX, TY, true, false>(P, st, q, gb, txl, tyl); break;
+    case 0: tile_body<TI, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync); break;
+    case 1: tile_body<TI, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync); break;
+    case 2: tile_body<TI, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync); break;
+    default: tile_body<TI, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync); break;
     }
 }
 
@@ -381,11 +416,18 @@ __device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, const
     }
 }
 
+// The problem descriptor of the current item is copied into LDS once per item:
+// read from global memory at every use, each pointer field would cost a vector
+// load and a wait in front of the loads it feeds.
+constexpr int PKPROB_WORDS = (int)(sizeof(PkProb) / 8);
+static_assert(sizeof(PkProb) % 8 == 0, "PkProb is copied as 8-byte words");
+
 template <typename TI, int TX, int TY>
 __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 {
     extern __shared__ double dyn[];
     __shared__ int s_next;
+    __shared__ unsigned long long s_prob[PKPROB_WORDS];
     const int tid = threadIdx.x;
     const int epoch = a.queue[1];
     int *err = a.queue + 2;
@@ -401,16 +443,20 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
         const int tpos = __builtin_amdgcn_readfirstlane(it.z);
         const int qg = __builtin_amdgcn_readfirstlane(it.w);
         const int txl = tpos & 0xffff, tyl = tpos >> 16, q = qg & 0xff, gb = qg >> 8;
-        const PkProb &P = a.probs[pi];
+        const PkProb &Pg = a.probs[pi];
+        if (tid < PKPROB_WORDS) s_prob[tid] = reinterpret_cast<const unsigned long long *>(&Pg)[tid];
+        __syncthreads();
+        const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
         double *ctab = dyn;
         TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
-        if (tid == 0) {
-            if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
-            if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
-        }
-        load_ctab(P, ctab, q);
-        __syncthreads();
-        tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl);
+        tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
+            if (tid == 0) {
+                if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
+                if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
+            }
+            load_ctab(P, ctab, q);
+            __syncthreads();
+        });
         wait_vm_stores();                 // this wave's face stores have completed
         __syncthreads();
         if (tid == 0) {
@@ -425,7 +471,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 
 // One launch per tile diagonal d: the tiles of d of every quadrant and group block.
 template <typename TI, int TX, int TY>
-__global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, int d, int tx_lo, int nt_d, int nblk_g)
+__global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, int tx_lo, int nt_d, int nblk_g)
 {
     extern __shared__ double dyn[];
     int bid = blockIdx.x;
@@ -434,11 +480,12 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb *probs, in
     const int t = bid % nt_d;
     const int q = bid / nt_d;
     const int txl = tx_lo + t, tyl = d - txl;
-    const PkProb &P = probs[0];
     double *ctab = dyn;
     TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
-    load_ctab(P, ctab, q);
-    tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl);
+    tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
+        load_ctab(P, ctab, q);
+        __syncthreads();
+    });
     __syncthreads();
     tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
 }
@@ -531,6 +578,7 @@ struct PkState {
     size_t lds = 0;
     int grid = 0;
     const TileKernels *k = nullptr;
+    PkProb host[2];              // host copies of the two tables
 };
 
 void pk_destroy(Ctx &c)
@@ -559,13 +607,15 @@ static void pk_setup(Ctx &c)
     P.inv_cdt = 1.0 / (c.p.c * c.p.dt);
     P.eps = 1e-25 / c.p.comp_unit;
     P.dx = c.dx; P.dy = c.dy; P.ox = c.ox; P.oy = c.oy; P.w = c.w;
-    P.sc = reinterpret_cast<const double2 *>(c.sc);
+    P.sc = c.precision ? static_cast<const void *>(c.scf) : static_cast<const void *>(c.sc);
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
     P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
     PkProb tab[2] = {P, P};
     tab[0].Iold = c.Icrn[0]; tab[0].Inew = c.Icrn[1];
     tab[1].Iold = c.Icrn[1]; tab[1].Inew = c.Icrn[0];
+    pk->host[0] = tab[0];
+    pk->host[1] = tab[1];
     pk->probs = static_cast<PkProb *>(c.dalloc(sizeof(tab)));
     TRT_CHECK(hipMemcpy(pk->probs, tab, sizeof(tab), hipMemcpyHostToDevice));
 
@@ -640,13 +690,14 @@ void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     PkState &pk = *c.pk;
     const SweepGeom &sg = c.sg;
     const PkProb *tab = pk.probs + c.iold;
+    PkProb hp = pk.host[c.iold];
     boundary_moments(c, tab);
     for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
         int tx_lo = std::max(0, d - (sg.NTY - 1));
         const int tx_hi = std::min(d, sg.NTX - 1);
         int nt_d = tx_hi - tx_lo + 1;
         int nblk_g = sg.nblk_g;
-        void *args[] = {&tab, &d, &tx_lo, &nt_d, &nblk_g};
+        void *args[] = {&hp, &d, &tx_lo, &nt_d, &nblk_g};
         if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d], c.stream));
         TRT_CHECK(hipLaunchKernel(pk.k->diag, dim3(4 * nt_d * nblk_g), dim3(PK_THREADS), args, pk.lds, c.stream));
         if (ev) TRT_CHECK(hipEventRecord((*ev)[2 * d + 1], c.stream));

@@ -95,9 +95,10 @@ struct Ctx {
     int iold = 0;                          // buffer holding I_crn_old
     void *XV = nullptr, *XH = nullptr;     // tile face exchange
     double *sc = nullptr;                  // [cell][g][2] = (KapE + 1/(c dt), RT_Src)
+    float *scf = nullptr;                  // FP32 copy of sc for the FP32 sweep
     double *Pavg = nullptr, *PV = nullptr, *PH = nullptr;   // per-quadrant partial moments
     size_t icrn_bytes = 0;
-    int sweep_mode = 3;                    // 1, 2, 3 as trt_problem.sweep_mode
+    int sweep_mode = 2;                    // 1, 2, 3 as trt_problem.sweep_mode
     PkState *pk = nullptr;
 
     // low-order systems

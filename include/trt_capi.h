@@ -50,7 +50,7 @@ typedef struct {
     int tile_x, tile_y;             /* sweep tile shape; 0 = default                 */
     double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
     int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default */
-    int sweep_mode;                 /* 0 = default (3); 1 = one launch per tile diagonal with a
+    int sweep_mode;                 /* 0 = default (2); 1 = one launch per tile diagonal with a
                                        per-cell moment reduction; 2 = one persistent launch per
                                        sweep; 3 = one launch per tile diagonal with per-row
                                        moment reduction                                   */
