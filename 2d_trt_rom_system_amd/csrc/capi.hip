@@ -288,4 +288,14 @@ int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms)
     return 0;
 }
 
+int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, int *grid)
+{
+    if (!ctx) return -1;
+    if (mode) *mode = ctx->c.sweep_mode;
+    if (tile_x) *tile_x = ctx->c.sg.TX;
+    if (tile_y) *tile_y = ctx->c.sg.TY;
+    if (grid) *grid = trt::pk_grid(ctx->c);
+    return 0;
+}
+
 }  // extern "C"

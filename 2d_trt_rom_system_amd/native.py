@@ -43,6 +43,7 @@ EXPORTS = [
     "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_mloqd_fv", "trt_collapse_mg_ef",
     "trt_cbar_calc", "trt_grey_coefs", "trt_gqd_in_calc", "trt_egp_fv_newt", "trt_convergence", "trt_step",
     "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
+    "trt_sweep_config",
 ]
 
 _lib = None
@@ -85,6 +86,8 @@ def load(path: str = LIBPATH):
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
     L.trt_sweep_stats.argtypes = [vp, C.POINTER(C.c_longlong), _dp]
+    ip = C.POINTER(C.c_int)
+    L.trt_sweep_config.argtypes = [vp, ip, ip, ip, ip]
     L.trt_sweep_kernel_timed.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
     L.trt_calib_copy.argtypes = [C.c_size_t, C.c_int, _dp]
     _lib = L
@@ -233,6 +236,12 @@ class Context:
         self._check(self.L.trt_sweep_kernel_timed(self.h, int(nsweeps), C.byref(wall), C.byref(kern), C.byref(nl)),
                     "sweep_kernel_timed")
         return wall.value, kern.value, nl.value
+
+    def sweep_config(self):
+        """(sweep_mode, tile_x, tile_y, persistent grid)."""
+        v = [C.c_int() for _ in range(4)]
+        self._check(self.L.trt_sweep_config(self.h, *[C.byref(x) for x in v]), "sweep_config")
+        return tuple(x.value for x in v)
 
     def sweep_stats(self):
         n, ms = C.c_longlong(), C.c_double()

@@ -126,6 +126,9 @@ int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *k
 int trt_calib_copy(size_t nbytes, int reps, double *ms);
 /* Accumulated sweep statistics since creation: sweeps run and total sweep ms. */
 int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
+/* The sweep configuration in use: sweep_mode (1, 2 or 3), tile shape, and the
+ * persistent launch's grid (workgroups; 0 before the first persistent sweep). */
+int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, int *grid);
 
 #ifdef __cplusplus
 }
