@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrt_hip.so")
-SOURCES = ["capi.hip", "state.hip", "sweep.hip", "sweep_pk.hip", "physics.hip", "loqd.hip"]
+SOURCES = ["capi.hip", "state.hip", "sweep.hip", "sweep_pk.hip", "physics.hip", "loqd.hip", "qd_direct.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -41,7 +41,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
             sys.stderr.write(f"--- {s} ---\n{t}\n")
         raise RuntimeError("hipcc failed for " + ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+                           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     build_fortran(objdir, verbose)
     return LIB

@@ -2,6 +2,7 @@
 // capture and the host side of every entry point.
 #include "trt_internal.h"
 
+#include <cstdlib>
 #include <cstring>
 
 using trt::Ctx;
@@ -65,7 +66,8 @@ int trt_ctx_create(trt_ctx **out, const trt_problem *p)
         c.nx = p->nx; c.ny = p->ny; c.nm = p->nm; c.ng = p->ng; c.nmat = p->nmat;
         c.precision = p->precision;
         c.lin_rtol = p->lin_rtol > 0 ? p->lin_rtol : 1e-13;
-        c.lin_maxit = p->lin_maxit > 0 ? p->lin_maxit : 20000;
+        c.lin_maxit = p->lin_maxit > 0 ? p->lin_maxit : 2000;
+        if (const char *v = getenv("TRT_VERBOSE")) c.verbose = atoi(v);
         TRT_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         TRT_CHECK(hipEventCreate(&c.ev0));
         TRT_CHECK(hipEventCreate(&c.ev1));
@@ -90,6 +92,7 @@ int trt_ctx_destroy(trt_ctx *ctx)
     Ctx &c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     trt::pk_destroy(c);
+    trt::direct_destroy(c);
     for (void *p : c.owned) (void)hipFree(p);
     if (c.hbuf) (void)hipHostFree(c.hbuf);
     if (c.ev0) (void)hipEventDestroy(c.ev0);

@@ -14,7 +14,9 @@
 // vertical face, y-momentum continuity (or bottom/top condition) at a horizontal
 // face.  All N_g group systems of MLOQD_FV are solved as one batch by a
 // Jacobi-preconditioned BiCGSTAB; the grey Newton system is a batch of one.
-#include "trt_internal.h"
+#include "qd_view.h"
+
+#include <chrono>
 
 #include <algorithm>
 #include <cmath>
@@ -23,93 +25,21 @@
 
 namespace trt {
 
-enum { EBL, EBB, EBC, EBR, EBT, MXC, MXR, MXB, MXT, MYC, MYT, MYL, MYR, ERHS, MXRHS, MYRHS, NCOEF };
-enum { CPL, CPB, CPR, CPT, BCL, BCB, BCR, BCT, NBND };
-
 static int nb(size_t n) { size_t b = (n + 255) / 256; return (int)std::min<size_t>(std::max<size_t>(b, 1), 65536); }
-
-// ---------------------------------------------------------------------------
-// matrix-free row of the reduced QD system
-// ---------------------------------------------------------------------------
-struct QDView {
-    int nx, ny, nb;
-    size_t nc, nv, nh, nrow;
-    const double *coef;   // [sys][NCOEF][nc]
-    const double *bnd;    // [sys][NBND][nb]
-    const double *cs;     // [sys][nrow] column scales: the row computes (A S y)_row for x = S y
-};
 
 // Row `row` of system `sys` applied to the scaled unknowns y (x = S y, S = diag(cs)):
 // ax = (A S y)_row, diag = A_row,row * cs_row, rhs = b_row.
 __device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, const double *y, double &ax,
                                        double &diag, double &rhs)
 {
-    const int nx = V.nx, ny = V.ny;
-    const double *cf = V.coef + (size_t)sys * NCOEF * V.nc;
-    const double *bd = V.bnd + (size_t)sys * NBND * V.nb;
     const double *s = V.cs + (size_t)sys * V.nrow;
-    const double *xc = y, *xv = y + V.nc, *xh = y + V.nc + V.nv;
-    const double *sc = s, *sv = s + V.nc, *sh = s + V.nc + V.nv;
-#define CF(k, i, j) cf[(size_t)(k) * V.nc + (size_t)(i) + (size_t)nx * (j)]
-#define XC(i, j) (xc[(size_t)(i) + (size_t)nx * (j)] * sc[(size_t)(i) + (size_t)nx * (j)])
-#define XV(i, j) (xv[(size_t)(i) + (size_t)(nx + 1) * (j)] * sv[(size_t)(i) + (size_t)(nx + 1) * (j)])
-#define XH(i, j) (xh[(size_t)(i) + (size_t)nx * (j)] * sh[(size_t)(i) + (size_t)nx * (j)])
-    if (row < V.nc) {                       // energy balance of cell (i,j)
-        const int i = (int)(row % nx), j = (int)(row / nx);
-        ax = CF(EBL, i, j) * XV(i, j) + CF(EBB, i, j) * XH(i, j) + CF(EBC, i, j) * XC(i, j)
-             + CF(EBR, i, j) * XV(i + 1, j) + CF(EBT, i, j) * XH(i, j + 1);
-        diag = CF(EBC, i, j);
-        rhs = CF(ERHS, i, j);
-    } else if (row < V.nc + V.nv) {         // vertical face f of row j
-        const size_t r = row - V.nc;
-        const int f = (int)(r % (nx + 1)), j = (int)(r / (nx + 1));
-        if (f == 0) {                       // left boundary condition (QS:92-97, 262-267)
-            ax = bd[CPL * V.nb + j] * XV(0, j) - CF(MXB, 0, j) * XH(0, j) + CF(MXC, 0, j) * XC(0, j)
-                 - CF(MXT, 0, j) * XH(0, j + 1);
-            diag = bd[CPL * V.nb + j];
-            rhs = bd[BCL * V.nb + j];
-        } else if (f == nx) {               // right boundary condition (QS:198-203, 364-369)
-            const int i = nx - 1;
-            ax = -CF(MXB, i, j) * XH(i, j) - CF(MXC, i, j) * XC(i, j) + bd[CPR * V.nb + j] * XV(nx, j)
-                 - CF(MXT, i, j) * XH(i, j + 1);
-            diag = bd[CPR * V.nb + j];
-            rhs = bd[BCR * V.nb + j];
-        } else {                            // x-momentum between cells f-1 and f (QS:117-125)
-            const int i = f - 1;
-            ax = CF(MXB, i, j) * XH(i, j) + CF(MXC, i, j) * XC(i, j) + CF(MXR, i, j) * XV(f, j)
-                 + CF(MXT, i, j) * XH(i, j + 1) - CF(MXB, f, j) * XH(f, j) + CF(MXC, f, j) * XC(f, j)
-                 - CF(MXT, f, j) * XH(f, j + 1);
-            diag = CF(MXR, i, j);
-            rhs = CF(MXRHS, i, j);
-        }
-    } else {                                // horizontal face f of column i
-        const size_t r = row - V.nc - V.nv;
-        const int i = (int)(r % nx), f = (int)(r / nx);
-        if (f == 0) {                       // bottom boundary condition (QS:100-105, 168-173)
-            ax = -CF(MYL, i, 0) * XV(i, 0) + bd[CPB * V.nb + i] * XH(i, 0) + CF(MYC, i, 0) * XC(i, 0)
-                 - CF(MYR, i, 0) * XV(i + 1, 0);
-            diag = bd[CPB * V.nb + i];
-            rhs = bd[BCB * V.nb + i];
-        } else if (f == ny) {               // top boundary condition (QS:303-309, 388-393)
-            const int j = ny - 1;
-            ax = -CF(MYL, i, j) * XV(i, j) - CF(MYC, i, j) * XC(i, j) - CF(MYR, i, j) * XV(i + 1, j)
-                 + bd[CPT * V.nb + i] * XH(i, ny);
-            diag = bd[CPT * V.nb + i];
-            rhs = bd[BCT * V.nb + i];
-        } else {                            // y-momentum between cells f-1 and f (QS:128-137)
-            const int j = f - 1;
-            ax = CF(MYL, i, j) * XV(i, j) + CF(MYC, i, j) * XC(i, j) + CF(MYR, i, j) * XV(i + 1, j)
-                 + CF(MYT, i, j) * XH(i, f) - CF(MYL, i, f) * XV(i, f) + CF(MYC, i, f) * XC(i, f)
-                 - CF(MYR, i, f) * XV(i + 1, f);
-            diag = CF(MYT, i, j);
-            rhs = CF(MYRHS, i, j);
-        }
-    }
+    double acc = 0.0;
+    qd_row_visit(V, sys, row, [&](int kind, int i, int j, double a) {
+        const size_t k = qd_index(V, kind, i, j);
+        acc += a * (y[k] * s[k]);
+    }, diag, rhs);
+    ax = acc;
     diag *= s[row];
-#undef CF
-#undef XC
-#undef XV
-#undef XH
 }
 
 // ---------------------------------------------------------------------------
@@ -121,7 +51,7 @@ __device__ __forceinline__ void qd_row(const QDView &V, int sys, size_t row, con
 // which a norm of the unscaled residual cannot do.
 // ---------------------------------------------------------------------------
 // per-system scalars
-enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_BNORM, S_RNORM, S_DONE, S_ITS, S_XMAX, S_ACC, S_NSCAL };
+enum { S_RHO, S_ALPHA, S_OMEGA, S_BETA, S_BNORM, S_RNORM, S_DONE, S_ITS, S_XMAX, S_ACC, S_RNORM0, S_NSCAL };
 
 constexpr int RT = 256;   // threads of the vector kernels
 
@@ -185,6 +115,7 @@ __global__ void k_init_fin(KrylovArgs a)
     double *S = a.scal + sys * S_NSCAL;
     S[S_BNORM] = sqrt(bb);
     S[S_RNORM] = sqrt(rr);
+    S[S_RNORM0] = sqrt(rr);
     if (S[S_ACC] != 0.0) {   // accepted in an earlier stage: leave it alone
         S[S_DONE] = 1.0;
         S[S_ITS] = 0.0;
@@ -322,6 +253,8 @@ __global__ void k_rho(KrylovArgs a)
     S[S_ITS] += 1.0;
     S[S_RNORM] = sqrt(rr);
     if (sqrt(rr) <= a.tol * S[S_BNORM]) { S[S_DONE] = 1.0; return; }
+    // diverging: the recurrence residual grew far beyond the initial one
+    if (sqrt(rr) > 1e4 * fmax(S[S_RNORM0], a.tol * S[S_BNORM])) { S[S_DONE] = 2.0; return; }
     if (S[S_OMEGA] == 0.0 || S[S_RHO] == 0.0 || !isfinite(rr)) { S[S_DONE] = 2.0; return; }
     S[S_BETA] = (rhr / S[S_RHO]) * (S[S_ALPHA] / S[S_OMEGA]);
     S[S_RHO] = rhr;
@@ -489,12 +422,23 @@ void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
     hipLaunchKernelGGL(k_refmax, grid, blk, 0, c.stream, a, s.ref);
     hipLaunchKernelGGL(k_refmax_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
     hipLaunchKernelGGL(k_clear_acc, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+    // systems on which BiCGSTAB failed recently skip it: marked accepted, solved directly below
+    if (s.direct_next.size() != (size_t)s.nsys) s.direct_next.assign(s.nsys, 0);
+    std::vector<int> skipped;
+    static const double one = 1.0;
+    for (int k = 0; k < s.nsys; ++k)
+        if (s.direct_next[k] > 0) {
+            skipped.push_back(k);
+            --s.direct_next[k];
+            TRT_CHECK(hipMemcpyAsync(a.scal + (size_t)k * S_NSCAL + S_ACC, &one, 8, hipMemcpyHostToDevice, c.stream));
+        }
     std::vector<double> hs((size_t)s.nsys * S_NSCAL);
     std::vector<int> total_its(s.nsys, 0);
     const double floors[3] = {1e-20, 1e-6, -1.0};
     bool all_ok = false;
     std::vector<double> h(3 * (size_t)s.nsys);
     for (int stage = 0; stage < 3 && !all_ok; ++stage) {
+        if (skipped.size() == (size_t)s.nsys) break;
         if (stage > 0) hipLaunchKernelGGL(k_restore, grid, blk, 0, c.stream, a, s.x0);
         hipLaunchKernelGGL(k_colscale, grid, blk, 0, c.stream, a, s.ref, s.cs, floors[stage]);
         const double thr = (stage == 2 ? 100.0 : 4.0) * c.lin_rtol;
@@ -535,12 +479,47 @@ void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
                 if (h[3 * k + 2] == 0.0) all_ok = false;
                 if (!(h[3 * k] <= h[3 * k + 1])) diverged = true;    // worse than x = 0, or not finite
             }
+            if (c.verbose > 1 || (c.verbose && !all_ok)) {
+                int nacc = 0, worst = 0;
+                for (int k = 0; k < s.nsys; ++k) {
+                    nacc += h[3 * k + 2] != 0.0;
+                    if (h[3 * k] / (h[3 * k + 1] + 1e-300) > h[3 * worst] / (h[3 * worst + 1] + 1e-300)) worst = k;
+                }
+                fprintf(stderr, "[qd_solve nsys=%d] stage %d restart %d its %d accepted %d/%d worst sys %d |r|/|b| %.3e\n",
+                        s.nsys, stage, restart, it, nacc, s.nsys, worst, h[3 * worst] / (h[3 * worst + 1] + 1e-300));
+            }
             if (all_ok || (diverged && stage < 2)) break;
         }
         hipLaunchKernelGGL(k_unscale, grid, blk, 0, c.stream, a);
     }
     TRT_CHECK(hipGetLastError());
-    if (!all_ok) {
+    std::vector<int> which = skipped;
+    for (int k = 0; k < s.nsys; ++k)
+        if (h[3 * k + 2] == 0.0 && !std::binary_search(skipped.begin(), skipped.end(), k)) {
+            which.push_back(k);
+            s.direct_next[k] = 3;
+        }
+    if (!which.empty()) {
+        // the reference falls back to a direct solve when BiCGSTAB fails (QS:492-500)
+        std::sort(which.begin(), which.end());
+        hipLaunchKernelGGL(k_restore, grid, blk, 0, c.stream, a, s.x0);
+        hipLaunchKernelGGL(k_colscale, grid, blk, 0, c.stream, a, s.ref, s.cs, -1.0);   // unit scales
+        hipLaunchKernelGGL(k_unscale, grid, blk, 0, c.stream, a);
+        const auto t0 = std::chrono::steady_clock::now();
+        qd_direct(c, s, which);
+        if (c.verbose) {
+            TRT_CHECK(hipStreamSynchronize(c.stream));
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "[qd_solve nsys=%d] direct solve of %zu systems (%zu skipped BiCGSTAB): %.1f ms\n", s.nsys,
+                    which.size(), skipped.size(), ms);
+        }
+        hipLaunchKernelGGL(k_init, grid, blk, 0, c.stream, a);          // |b| in the unscaled rows
+        hipLaunchKernelGGL(k_init_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+        hipLaunchKernelGGL(k_true_res, grid, blk, 0, c.stream, a);
+        hipLaunchKernelGGL(k_true_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a, c.scratch, 1e4 * c.lin_rtol);
+        TRT_CHECK(hipMemcpyAsync(h.data(), c.scratch, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+        for (int k : which) total_its[k] = -1;     // as the reference flags a direct solve (QS:499)
         for (int k = 0; k < s.nsys; ++k)
             if (!(h[3 * k] <= 1e4 * c.lin_rtol * h[3 * k + 1] || h[3 * k] == 0.0)) {
                 char msg[160];

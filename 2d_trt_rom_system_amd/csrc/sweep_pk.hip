@@ -427,25 +427,39 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 {
     extern __shared__ double dyn[];
     __shared__ int s_next;
+    __shared__ int4 s_item;                          // work-list entry of s_next
+    __shared__ int s_pi;                             // problem whose descriptor is in s_prob
     __shared__ unsigned long long s_prob[PKPROB_WORDS];
     const int tid = threadIdx.x;
     const int epoch = a.queue[1];
     int *err = a.queue + 2;
-    if (tid == 0) s_next = atomicAdd(a.queue, 1);
+    if (tid == 0) {
+        const int c0 = atomicAdd(a.queue, 1);
+        s_next = c0;
+        if (c0 < a.nitems) s_item = a.items[c0];
+        s_pi = -1;
+    }
     __syncthreads();
     int cur = __builtin_amdgcn_readfirstlane(s_next);
     while (cur < a.nitems) {
+        // the next item and its work-list entry are fetched while this one runs
         int nxt = 0;
-        if (tid == 0) nxt = atomicAdd(a.queue, 1);
-        const int4 it = a.items[cur];
-        const int pi = __builtin_amdgcn_readfirstlane(it.x);
-        const int fl = __builtin_amdgcn_readfirstlane(it.y);
-        const int tpos = __builtin_amdgcn_readfirstlane(it.z);
-        const int qg = __builtin_amdgcn_readfirstlane(it.w);
+        int4 nit = make_int4(0, 0, 0, 0);
+        if (tid == 0) {
+            nxt = atomicAdd(a.queue, 1);
+            if (nxt < a.nitems) nit = a.items[nxt];
+        }
+        const int pi = __builtin_amdgcn_readfirstlane(s_item.x);
+        const int fl = __builtin_amdgcn_readfirstlane(s_item.y);
+        const int tpos = __builtin_amdgcn_readfirstlane(s_item.z);
+        const int qg = __builtin_amdgcn_readfirstlane(s_item.w);
         const int txl = tpos & 0xffff, tyl = tpos >> 16, q = qg & 0xff, gb = qg >> 8;
-        const PkProb &Pg = a.probs[pi];
-        if (tid < PKPROB_WORDS) s_prob[tid] = reinterpret_cast<const unsigned long long *>(&Pg)[tid];
-        __syncthreads();
+        if (pi != __builtin_amdgcn_readfirstlane(s_pi)) {
+            const PkProb &Pg = a.probs[pi];
+            if (tid < PKPROB_WORDS) s_prob[tid] = reinterpret_cast<const unsigned long long *>(&Pg)[tid];
+            __syncthreads();
+            if (tid == 0) s_pi = pi;
+        }
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
         double *ctab = dyn;
         TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
@@ -462,6 +476,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
         if (tid == 0) {
             __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_next = nxt;
+            s_item = nit;
         }
         tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
         __syncthreads();

@@ -62,9 +62,13 @@ struct QDSystems {
     double *scal = nullptr;       // per-system scalars
     int *flags = nullptr;         // per-system convergence flags (device)
     int red_blocks = 0;
+    // per system: how many coming solves go straight to the direct solver (set
+    // when BiCGSTAB failed on it; the systems of successive iterations are close)
+    std::vector<int> direct_next;
 };
 
 struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
+struct DirectWork;  // block-LU workspace of the direct QD fallback (qd_direct.hip)
 
 struct Ctx {
     trt_problem p{};
@@ -107,8 +111,10 @@ struct Ctx {
     size_t scratch_n = 0;
     double *hbuf = nullptr;                // pinned host staging for scalars
     int lin_maxit = 0;
+    int verbose = 0;                       // TRT_VERBOSE=1: per-solve Krylov statistics on stderr
     double lin_rtol = 1e-13;
     std::vector<int> last_kits;
+    DirectWork *direct = nullptr;
 
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
@@ -154,6 +160,9 @@ void qd_alloc(Ctx &c, QDSystems &s, int nsys);
 // solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess.
 // The unknowns are scaled by |s.ref| (ref_given) or by |initial guess|.
 void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given);
+// direct block-LU solve of the listed systems of s (their solutions replace s.x)
+void qd_direct(Ctx &c, QDSystems &s, const std::vector<int> &which);
+void direct_destroy(Ctx &c);
 // device copy helper
 void dcopy(Ctx &c, double *dst, const double *src, size_t n);
 void dfill(Ctx &c, double *dst, double v, size_t n);

@@ -49,7 +49,8 @@ typedef struct {
     int precision;                  /* 0 = FP64 sweep, 1 = FP32 sweep with FP64 moment accumulation */
     int tile_x, tile_y;             /* sweep tile shape; 0 = default                 */
     double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
-    int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default */
+    int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default 2000.  A system
+                                       that BiCGSTAB does not solve goes to a block-LU direct solve */
     int sweep_mode;                 /* 0 = default (2); 1 = one launch per tile diagonal with a
                                        per-cell moment reduction; 2 = one persistent launch per
                                        sweep; 3 = one launch per tile diagonal with per-row
@@ -94,7 +95,8 @@ int trt_old_mgqd_coefs(trt_ctx *ctx);
 int trt_old_grey_coefs(trt_ctx *ctx);
 /* MLOQD_FV, MainSrc/MLOQD_SOLVES.f08:78-362, solving the N_g reduced systems of
  * QD_FV (MainSrc/QD_SOLVERS.f08:40-538) as one batch.  kits (may be NULL)
- * receives N_g Krylov iteration counts. */
+ * receives N_g Krylov iteration counts, -1 for a group solved by the block-LU
+ * direct fallback (the reference's dense fallback, QD_SOLVERS.f08:492-500). */
 int trt_mloqd_fv(trt_ctx *ctx, int *kits);
 /* COLLAPSE_MG_EF MainSrc/MLOQD_SOLVES.f08:687-730; Cbar_Calc MainSrc/GLOQD_SOLVES.f08:177-231;
  * GREY_COEFS :47-146; GQD_In_Calc :263-290 */

@@ -74,6 +74,8 @@ def lib():
             getattr(L, f).restype = None
         L.orc_egp_fv_newt.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_lin_noise.argtypes = [C.c_double]
+        L.orc_set_qd_capture.argtypes = [C.c_int, C.c_void_p, C.c_int]
+        L.orc_set_qd_capture.restype = None
         L.orc_set_lin_noise.restype = None
         L.orc_egp_fv_newt.restype = C.c_int
         L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_int * 4)]

@@ -852,6 +852,19 @@ int orc_qd_fv(int nx, int ny, const double *const coef[16], const double *const 
 /* ------------------------------------------------------------------------ */
 /* MLOQD_FV, MLOQD_SOLVES.f08:78-362 (residual branch omitted)              */
 /* ------------------------------------------------------------------------ */
+/* Test hook (off by default): MLOQD_FV copies the coefficient arrays of one
+ * group's reduced QD system (16 x N_c coefficients, then 8 x max(N_x,N_y)
+ * boundary terms, in the QD_COEF_NAMES / QD_BND_NAMES order) to a caller
+ * buffer, and may skip the solves; used to study the linear systems. */
+static double *g_cap_buf = NULL;
+static int g_cap_group = -1, g_cap_skip = 0;
+void orc_set_qd_capture(int group, double *buf, int skip_solve)
+{
+    g_cap_group = group;
+    g_cap_buf = buf;
+    g_cap_skip = skip_solve;
+}
+
 void orc_mloqd_fv(orc_state *s)
 {
     const int nx = s->nx, ny = s->ny, ng = s->ng;
@@ -932,6 +945,14 @@ void orc_mloqd_fv(orc_state *s)
         const double *cbd[8];
         for (int q = 0; q < 16; ++q) ccf[q] = cf[q];
         for (int q = 0; q < 8; ++q) cbd[q] = bd[q];
+        if (g == g_cap_group && g_cap_buf) {
+            for (int q = 0; q < 16; ++q) memcpy(g_cap_buf + (size_t)q * n, cf[q], n * sizeof(double));
+            for (int q = 0; q < 8; ++q) memcpy(g_cap_buf + 16 * n + (size_t)q * nb, bd[q], (size_t)nb * sizeof(double));
+        }
+        if (g_cap_skip) {
+            free(buf);
+            continue;
+        }
         orc_qd_fv(nx, ny, ccf, cbd, Ea, Ev, Eh);
         s->MGQD_Kits[g] = -1;      /* direct solve, as the reference flags it (QS:499) */
         /* flux recovery, ML:242-261 */

@@ -89,6 +89,8 @@ int orc_qd_build_csr(int nx, int ny, const double *const coef[16], const double 
                      int *ia, int *ja, double *a, double *rhs);
 /* solve the reduced QD system by banded LU with partial pivoting and scatter the
  * solution to E_avg(N_x,N_y), E_edgV(N_x+1,N_y), E_edgH(N_x,N_y+1). Returns 0 on success. */
+/* test hook: capture one group's MLOQD system coefficients, optionally skipping the solves */
+void orc_set_qd_capture(int group, double *buf, int skip_solve);
 /* sensitivity studies: relative perturbation of every QD solution (0 = off) */
 void orc_set_lin_noise(double eps);
 int orc_qd_fv(int nx, int ny, const double *const coef[16], const double *const bnd[8],

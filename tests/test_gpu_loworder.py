@@ -18,14 +18,14 @@ def sync_to_gpu(o, ctx, names):
         ctx.push(n, o[n])
 
 
-def _pair(trt, oracle_mod, steps=1, **kw):
+def _pair(trt, oracle_mod, steps=1, lin_maxit=0, **kw):
     prob = trt.problem.small(nx=9, ny=7, order=8, ng=3)
     for k, v in kw.items():
         setattr(prob, k, v)
     o = oracle_mod.Oracle(prob, threads=3)
     for _ in range(steps):
         o.step()
-    ctx = trt.native.Context(prob)
+    ctx = trt.native.Context(prob, lin_maxit=lin_maxit)
     return prob, o, ctx
 
 
@@ -53,12 +53,20 @@ def test_old_coefs_match(trt, oracle_mod):
     ctx.close()
 
 
-def test_mloqd_fv_matches(trt, oracle_mod):
-    prob, o, ctx = _pair(trt, oracle_mod, steps=1)
+@pytest.mark.parametrize("lin_maxit", [0, 1])
+def test_mloqd_fv_matches(trt, oracle_mod, lin_maxit):
+    """lin_maxit = 1 starves BiCGSTAB (one iteration per restart), so groups go
+    through the block-LU direct fallback (reported as -1 iterations)."""
+    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit)
     sync_to_gpu(o, ctx, ML_INPUTS)
+    if lin_maxit == 1:
+        # an initial guess that is not already the solution
+        for n in ("Eg_avg", "Eg_edgV", "Eg_edgH"):
+            ctx.push(n, 1.5 * o[n])
     o.call("mloqd_fv")
     kits = ctx.mloqd_fv()
-    assert len(kits) == prob.ng and all(k >= 0 for k in kits)
+    assert len(kits) == prob.ng, kits
+    assert (-1 in kits) if lin_maxit == 1 else all(k >= 0 for k in kits), kits
     for n in ("Eg_avg", "Eg_edgV", "Eg_edgH", "Fxg_edgV", "Fyg_edgH"):
         assert rel_err(ctx.pull(n), o[n].ravel(order="F")) < 1e-10, n
     ctx.close()
@@ -86,11 +94,12 @@ GREY_STATE = ["E_avg", "E_edgV", "E_edgH", "Fx_edgV", "Fy_edgH", "Temp", "KapE_B
               "F_in_L", "F_in_B", "F_in_R", "F_in_T", "Temp_MGQDold2", "KapE_bar_MGQDold"]
 
 
-@pytest.mark.parametrize("mgqd_it", [1, 2])
-def test_egp_newton_matches(trt, oracle_mod, mgqd_it):
+@pytest.mark.parametrize("mgqd_it,lin_maxit", [(1, 0), (2, 0), (1, 1)])
+def test_egp_newton_matches(trt, oracle_mod, mgqd_it, lin_maxit):
     """Line search off: both Newton iterations reach the same grey fixed point (the line
-    search's accept/reject decisions follow rounding noise once a delta is tiny)."""
-    prob, o, ctx = _pair(trt, oracle_mod, steps=1, use_line_search=0)
+    search's accept/reject decisions follow rounding noise once a delta is tiny).
+    lin_maxit = 1 sends the grey linear solves through the direct fallback."""
+    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit, use_line_search=0)
     o.call("mloqd_fv")
     o.call("collapse_mg_ef")
     o.call("cbar_calc")
