@@ -12,6 +12,11 @@ struct trt_ctx {
     Ctx c;
 };
 
+struct trt_batch {
+    trt::PkBatch *b = nullptr;
+    std::string err;
+};
+
 namespace {
 
 template <typename F>
@@ -111,8 +116,10 @@ const char *trt_last_error(const trt_ctx *ctx)
 size_t trt_ctx_bytes(const trt_problem *p)
 {
     if (!p) return 0;
-    return trt::sweep_bytes(*p, nullptr);
+    return trt::sweep_bytes(*p, nullptr) + trt::state_bytes(*p);
 }
+
+size_t trt_ctx_device_bytes(const trt_ctx *ctx) { return ctx ? ctx->c.bytes : 0; }
 
 size_t trt_array_size(const trt_ctx *ctx, const char *name)
 {
@@ -298,6 +305,117 @@ int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, in
     if (tile_x) *tile_x = ctx->c.sg.TX;
     if (tile_y) *tile_y = ctx->c.sg.TY;
     if (grid) *grid = trt::pk_grid(ctx->c);
+    return 0;
+}
+
+int trt_tgen_step(trt_ctx *ctx)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        trt::tgen_step(c);
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+        trt::pk_check(c);
+    });
+}
+
+int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n)
+{
+    if (!out) return -1;
+    *out = nullptr;
+    try {
+        if (!ctxs || n < 1) throw Error("a batch needs at least one context");
+        std::vector<Ctx *> v;
+        for (int i = 0; i < n; ++i) {
+            if (!ctxs[i]) throw Error("null context in batch");
+            v.push_back(&ctxs[i]->c);
+        }
+        trt_batch *b = new trt_batch();
+        try {
+            b->b = trt::pk_batch_create(v);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+        return 0;
+    } catch (const Error &e) {
+        g_create_error = e.msg;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+    }
+    return -2;
+}
+
+int trt_batch_destroy(trt_batch *b)
+{
+    if (!b) return -1;
+    trt::pk_batch_destroy(b->b);
+    delete b;
+    return 0;
+}
+
+const char *trt_batch_last_error(const trt_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"
+
+namespace {
+template <typename F>
+int bguarded(trt_batch *b, F &&f)
+{
+    if (!b || !b->b) return -1;
+    try {
+        f(*b->b);
+        return 0;
+    } catch (const Error &e) {
+        b->err = e.msg;
+    } catch (const std::exception &e) {
+        b->err = e.what();
+    }
+    return -2;
+}
+
+void sync_members(trt::PkBatch &pb)
+{
+    for (Ctx *c : trt::pk_batch_ctxs(pb)) TRT_CHECK(hipStreamSynchronize(c->stream));
+}
+}  // namespace
+
+extern "C" {
+
+int trt_batch_sweep_timed(trt_batch *b, int nsweeps, double *ms)
+{
+    return bguarded(b, [&](trt::PkBatch &pb) {
+        if (nsweeps < 1) throw Error("nsweeps must be positive");
+        for (int k = 0; k < nsweeps; ++k) trt::pk_batch_sweep(pb, (k == 0 ? 1 : 0) | (k == nsweeps - 1 ? 2 : 0));
+        const double t = trt::pk_batch_elapsed_ms(pb);
+        trt::pk_batch_check(pb);
+        sync_members(pb);
+        for (Ctx *c : trt::pk_batch_ctxs(pb)) c->sweep_ms += t / (double)trt::pk_batch_ctxs(pb).size();
+        if (ms) *ms = t;
+    });
+}
+
+int trt_batch_tgen_step(trt_batch *b, double *sweep_ms)
+{
+    return bguarded(b, [&](trt::PkBatch &pb) {
+        for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_pre(*c);
+        trt::pk_batch_sweep(pb, 3);
+        for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_post(*c);
+        sync_members(pb);
+        const double t = trt::pk_batch_elapsed_ms(pb);
+        trt::pk_batch_check(pb);
+        if (sweep_ms) *sweep_ms = t;
+    });
+}
+
+int trt_batch_use_graph(trt_batch *b, int on)
+{
+    return bguarded(b, [&](trt::PkBatch &pb) { trt::pk_batch_use_graph(pb, on != 0); });
+}
+
+int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems)
+{
+    if (!b || !b->b) return -1;
+    trt::pk_batch_info(*b->b, updates_per_sweep, grid, nitems);
     return 0;
 }
 

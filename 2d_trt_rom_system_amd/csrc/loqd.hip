@@ -1313,4 +1313,30 @@ int step(Ctx &c, int its_out[4])
     return rt_its;
 }
 
+// ---------------------------------------------------------------------------
+// Tgen_QDf, ALGORITHMS.f08:1031-1095: one sweep per step with Temp prescribed
+// (MATERIAL_UPDATE :1042, I_crn_old = I_crn :1046, TRANSPORT_SCB :1047,
+// COLLAPSE_INTENSITIES :1056, fg_Calc :1083, Cg_Calc :1095), no low-order solve.
+// ---------------------------------------------------------------------------
+void tgen_pre(Ctx &c)
+{
+    c.time += c.p.dt;
+    material_update(c);
+    c.iold = c.icur;
+}
+
+void tgen_post(Ctx &c)
+{
+    collapse_intensities(c);
+    fg_calc(c);
+    cg_calc(c);
+}
+
+void tgen_step(Ctx &c)
+{
+    tgen_pre(c);
+    transport_sweep(c);
+    tgen_post(c);
+}
+
 }  // namespace trt

@@ -13,7 +13,18 @@ void *Ctx::dalloc(size_t bytes)
     void *p = nullptr;
     TRT_CHECK(hipMalloc(&p, bytes ? bytes : 8));
     owned.push_back(p);
+    this->bytes += bytes ? bytes : 8;
     return p;
+}
+
+size_t state_bytes(const trt_problem &p)
+{
+    const size_t na = (size_t)p.nx * p.ny, nv = (size_t)(p.nx + 1) * p.ny, nh = (size_t)p.nx * (p.ny + 1);
+    const size_t nbd = (size_t)p.nx + p.ny;
+    size_t n = (25 * na + 12 * nv + 12 * nh + 6 * nbd) * p.ng;       // per-group arrays of init_state
+    n += 40 * na + 6 * nv + 6 * nh + 6 * nbd + 3 * (size_t)p.nm + p.ng + 1 + p.nmat + p.nx + p.ny;
+    n += std::max<size_t>(4096, 8 * (size_t)std::max(p.nx, p.ny) * p.ng + 4096) + 4 * (size_t)p.ng;   // scratch, bcin
+    return n * 8 + na * sizeof(int);
 }
 
 DevArray &Ctx::reg(const std::string &name, int d0, int d1, int d2, int d3)

@@ -12,6 +12,10 @@
 //    completion flags, which carry the sweep's epoch and are never cleared.
 //  * per-diagonal (sweep_mode 3): one launch per tile diagonal; kernel
 //    boundaries order the tiles.
+//  * batched: the persistent form over the items of many problems (uneven
+//    grids, group counts and quadratures, one tile shape), merged by tile
+//    diagonal so the merged list is still a topological order.  A small
+//    problem's fill and drain then overlap the others' work.
 //
 // Tile body.  A thread owns one (quadrant, group, direction).  At the start of
 // a tile it issues every load the tile needs -- the old corner intensities and
@@ -518,7 +522,7 @@ __global__ void pk_reset(int *queue)
 template <typename TI>
 __global__ void pk_boundary_moments(const PkProb *probs)
 {
-    const PkProb &P = probs[0];
+    const PkProb &P = probs[blockIdx.y];
     const int nx = P.nx, ny = P.ny, ng = P.ng, K = P.K;
     const int per = ny + nx;
     const int tot = 4 * ng * per;
@@ -604,6 +608,17 @@ void pk_destroy(Ctx &c)
 
 bool pk_supported(const Ctx &c) { return find_tile_kernels(c.precision, c.sg.TX, c.sg.TY) != nullptr; }
 
+// workgroups that fit on the device at once (the persistent grid), at most nitems
+static int resident_grid(const void *kern, size_t lds, int nitems)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    TRT_CHECK(hipGetDevice(&dev));
+    TRT_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TRT_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PK_THREADS, lds));
+    if (per_cu < 1) throw Error("sweep tile kernel does not fit on a compute unit");
+    return std::max(1, std::min(nitems, per_cu * ncu));
+}
+
 static void pk_setup(Ctx &c)
 {
     const SweepGeom &sg = c.sg;
@@ -658,21 +673,22 @@ static void pk_setup(Ctx &c)
     TRT_CHECK(hipMemset(pk->queue, 0, 4 * sizeof(int)));
 
     pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
-    int dev = 0, ncu = 0, per_cu = 0;
-    TRT_CHECK(hipGetDevice(&dev));
-    TRT_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    TRT_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk->k->pk, PK_THREADS, pk->lds));
-    if (per_cu < 1) throw Error("sweep tile kernel does not fit on a compute unit");
-    pk->grid = std::max(1, std::min(pk->nitems, per_cu * ncu));
+    pk->grid = resident_grid(pk->k->pk, pk->lds, pk->nitems);
+}
+
+// boundary-face moments of nprob problems (table tab), the largest with nbt faces x groups x quadrants
+static void boundary_moments(int precision, const PkProb *tab, int nprob, int nbt, hipStream_t s)
+{
+    const dim3 grid((nbt + 255) / 256, nprob);
+    if (precision)
+        hipLaunchKernelGGL(pk_boundary_moments<float>, grid, dim3(256), 0, s, tab);
+    else
+        hipLaunchKernelGGL(pk_boundary_moments<double>, grid, dim3(256), 0, s, tab);
 }
 
 static void boundary_moments(Ctx &c, const PkProb *tab)
 {
-    const int nbt = 4 * c.ng * (c.nx + c.ny);
-    if (c.precision)
-        hipLaunchKernelGGL(pk_boundary_moments<float>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
-    else
-        hipLaunchKernelGGL(pk_boundary_moments<double>, dim3((nbt + 255) / 256), dim3(256), 0, c.stream, tab);
+    boundary_moments(c.precision, tab, 1, 4 * c.ng * (c.nx + c.ny), c.stream);
 }
 
 // Enqueues one sweep: work-list reset, boundary-face moments, persistent kernel.
@@ -734,5 +750,233 @@ void pk_check(Ctx &c)
 }
 
 int pk_grid(const Ctx &c) { return c.pk ? c.pk->grid : 0; }
+
+// ---------------------------------------------------------------------------
+// Batched persistent sweep: one launch sweeps every problem of a batch.
+// ---------------------------------------------------------------------------
+struct PkBatch {
+    std::vector<Ctx *> ctxs;
+    const TileKernels *k = nullptr;
+    int precision = 0, tx = 0, ty = 0;
+    PkProb *probs = nullptr;        // device table, one descriptor per problem
+    std::vector<PkProb> host;       // the table as last uploaded
+    std::vector<int> up_iold;       // I_crn_old buffer of each problem at that upload
+    PkProb *staging = nullptr;      // pinned copy of the table for the upload
+    hipEvent_t uploaded = nullptr;  // the last upload has read the staging copy
+    bool upload_pending = false;
+    int4 *items = nullptr;
+    int nitems = 0;
+    int *queue = nullptr;           // [0] counter, [1] epoch, [2] error word
+    int *flags = nullptr;
+    size_t lds = 0;
+    int grid = 0, nbt = 0;
+    long long updates = 0;          // cell-angle-group updates per batched sweep
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> ready;  // per problem: its stream's work before the sweep
+    std::vector<void *> owned;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void *batch_alloc(PkBatch &b, size_t bytes)
+{
+    void *p = nullptr;
+    TRT_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    b.owned.push_back(p);
+    return p;
+}
+
+void pk_batch_destroy(PkBatch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void *p : b->owned) (void)hipFree(p);
+    for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->done, b->t0, b->t1, b->uploaded})
+        if (e) (void)hipEventDestroy(e);
+    if (b->staging) (void)hipHostFree(b->staging);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs)
+{
+    if (ctxs.empty()) throw Error("a batch needs at least one context");
+    Ctx &c0 = *ctxs[0];
+    for (Ctx *c : ctxs) {
+        if (c->precision != c0.precision) throw Error("batched contexts must share the sweep precision");
+        if (c->sweep_mode < 2) throw Error("batched contexts must use the tile engine (sweep_mode 0, 2 or 3)");
+        if (c->sg.TX != c0.sg.TX || c->sg.TY != c0.sg.TY) throw Error("batched contexts must share the tile shape");
+    }
+    for (size_t i = 0; i < ctxs.size(); ++i)
+        for (size_t j = 0; j < i; ++j)
+            if (ctxs[i] == ctxs[j]) throw Error("a context appears twice in the batch");
+    PkBatch *b = new PkBatch();
+    try {
+        b->ctxs = ctxs;
+        b->precision = c0.precision;
+        b->tx = c0.sg.TX;
+        b->ty = c0.sg.TY;
+        b->k = find_tile_kernels(b->precision, b->tx, b->ty);
+        if (!b->k) throw Error("no tile kernel for this tile shape and precision");
+        const int n = (int)ctxs.size();
+        b->host.resize(n);
+        b->up_iold.assign(n, -1);
+        std::vector<int> base(n + 1, 0);
+        int maxd = 0, maxk = 0;
+        for (int i = 0; i < n; ++i) {
+            Ctx &c = *ctxs[i];
+            if (!c.pk) pk_setup(c);
+            const SweepGeom &sg = c.sg;
+            base[i + 1] = base[i] + 4 * sg.nblk_g * sg.NTX * sg.NTY;
+            maxd = std::max(maxd, sg.NTX + sg.NTY - 2);
+            maxk = std::max(maxk, sg.K);
+            b->nbt = std::max(b->nbt, 4 * c.ng * (c.nx + c.ny));
+            b->updates += (long long)c.nx * c.ny * c.nm * c.ng;
+        }
+        // items of all problems, merged by tile diagonal
+        std::vector<int4> items;
+        items.reserve((size_t)base[n]);
+        for (int d = 0; d <= maxd; ++d)
+            for (int i = 0; i < n; ++i) {
+                const SweepGeom &sg = ctxs[i]->sg;
+                if (d > sg.NTX + sg.NTY - 2) continue;
+                const int lo = std::max(0, d - (sg.NTY - 1)), hi = std::min(d, sg.NTX - 1);
+                for (int txl = lo; txl <= hi; ++txl) {
+                    const int tyl = d - txl;
+                    for (int q = 0; q < 4; ++q)
+                        for (int gb = 0; gb < sg.nblk_g; ++gb) {
+                            const int flag = base[i] + ((q * sg.nblk_g + gb) * sg.NTY + tyl) * sg.NTX + txl;
+                            items.push_back(make_int4(i, flag, txl | (tyl << 16), q | (gb << 8)));
+                        }
+                }
+            }
+        b->nitems = (int)items.size();
+        if ((size_t)b->nitems != (size_t)base[n]) throw Error("batched work list does not cover every tile");
+        b->items = static_cast<int4 *>(batch_alloc(*b, items.size() * sizeof(int4)));
+        TRT_CHECK(hipMemcpy(b->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+        b->flags = static_cast<int *>(batch_alloc(*b, (size_t)base[n] * sizeof(int)));
+        TRT_CHECK(hipMemset(b->flags, 0, (size_t)base[n] * sizeof(int)));
+        b->queue = static_cast<int *>(batch_alloc(*b, 4 * sizeof(int)));
+        TRT_CHECK(hipMemset(b->queue, 0, 4 * sizeof(int)));
+        b->probs = static_cast<PkProb *>(batch_alloc(*b, (size_t)n * sizeof(PkProb)));
+        const size_t ti = b->precision ? 4 : 8;
+        b->lds = (size_t)maxk * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti;
+        b->grid = resident_grid(b->k->pk, b->lds, b->nitems);
+        TRT_CHECK(hipStreamCreate(&b->stream));
+        TRT_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+        TRT_CHECK(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
+        TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&b->staging), (size_t)n * sizeof(PkProb), hipHostMallocDefault));
+        TRT_CHECK(hipEventCreate(&b->t0));
+        TRT_CHECK(hipEventCreate(&b->t1));
+        b->ready.resize(n, nullptr);
+        for (auto &e : b->ready) TRT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    } catch (...) {
+        pk_batch_destroy(b);
+        throw;
+    }
+    return b;
+}
+
+static void batch_enqueue(PkBatch &b)
+{
+    hipLaunchKernelGGL(pk_reset, dim3(1), dim3(64), 0, b.stream, b.queue);
+    boundary_moments(b.precision, b.probs, (int)b.ctxs.size(), b.nbt, b.stream);
+    PkArgs a{};
+    a.probs = b.probs;
+    a.items = b.items;
+    a.nitems = b.nitems;
+    a.queue = b.queue;
+    a.flags = b.flags;
+    a.spin_limit = 1 << 20;
+    void *args[] = {&a};
+    TRT_CHECK(hipLaunchKernel(b.k->pk, dim3(b.grid), dim3(PK_THREADS), args, b.lds, b.stream));
+}
+
+// One batched sweep: every problem's TRANSPORT_SCB (I_crn from I_crn_old) with
+// its moments, after the work already enqueued on each problem's stream; each
+// problem's stream waits for the sweep.  Events t0 (mark & 1) and t1 (mark & 2)
+// are recorded on the batch stream before and after the sweep.
+void pk_batch_sweep(PkBatch &b, int mark)
+{
+    const int n = (int)b.ctxs.size();
+    bool dirty = false;
+    for (int i = 0; i < n; ++i) {
+        Ctx &c = *b.ctxs[i];
+        if (b.up_iold[i] != c.iold) {
+            b.host[i] = c.pk->host[c.iold];
+            b.up_iold[i] = c.iold;
+            dirty = true;
+        }
+    }
+    if (dirty) {
+        if (b.upload_pending) TRT_CHECK(hipEventSynchronize(b.uploaded));   // staging copy free again
+        std::copy(b.host.begin(), b.host.end(), b.staging);
+        TRT_CHECK(hipMemcpyAsync(b.probs, b.staging, (size_t)n * sizeof(PkProb), hipMemcpyHostToDevice, b.stream));
+        TRT_CHECK(hipEventRecord(b.uploaded, b.stream));
+        b.upload_pending = true;
+    }
+    for (int i = 0; i < n; ++i) {
+        TRT_CHECK(hipEventRecord(b.ready[i], b.ctxs[i]->stream));
+        TRT_CHECK(hipStreamWaitEvent(b.stream, b.ready[i], 0));
+    }
+    if (mark & 1) TRT_CHECK(hipEventRecord(b.t0, b.stream));
+    if (b.use_graph) {
+        if (!b.exec) {
+            TRT_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+            batch_enqueue(b);
+            TRT_CHECK(hipStreamEndCapture(b.stream, &b.graph));
+            TRT_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+        }
+        TRT_CHECK(hipGraphLaunch(b.exec, b.stream));
+    } else {
+        batch_enqueue(b);
+    }
+    if (mark & 2) TRT_CHECK(hipEventRecord(b.t1, b.stream));
+    TRT_CHECK(hipGetLastError());
+    TRT_CHECK(hipEventRecord(b.done, b.stream));
+    for (int i = 0; i < n; ++i) {
+        Ctx &c = *b.ctxs[i];
+        TRT_CHECK(hipStreamWaitEvent(c.stream, b.done, 0));
+        c.icur = 1 - c.iold;
+        c.sweeps += 1;
+    }
+}
+
+void pk_batch_use_graph(PkBatch &b, bool on) { b.use_graph = on; }
+
+// waits for the batch stream; throws if a dependency wait of the last sweeps timed out
+void pk_batch_check(PkBatch &b)
+{
+    int e = 0;
+    TRT_CHECK(hipMemcpyAsync(&e, b.queue + 2, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+    TRT_CHECK(hipStreamSynchronize(b.stream));
+    if (e) {
+        const int zero = 0;
+        TRT_CHECK(hipMemcpy(b.queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+        throw Error("batched sweep: a dependency wait exceeded its spin limit");
+    }
+}
+
+double pk_batch_elapsed_ms(PkBatch &b)
+{
+    float ms = 0.f;
+    TRT_CHECK(hipEventSynchronize(b.t1));
+    TRT_CHECK(hipEventElapsedTime(&ms, b.t0, b.t1));
+    return ms;
+}
+
+void pk_batch_info(const PkBatch &b, long long *updates, int *grid, int *nitems)
+{
+    if (updates) *updates = b.updates;
+    if (grid) *grid = b.grid;
+    if (nitems) *nitems = b.nitems;
+}
+
+const std::vector<Ctx *> &pk_batch_ctxs(const PkBatch &b) { return b.ctxs; }
 
 }  // namespace trt

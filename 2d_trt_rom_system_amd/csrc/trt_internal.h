@@ -119,6 +119,7 @@ struct Ctx {
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
     void *dalloc(size_t bytes);
+    size_t bytes = 0;             // device bytes allocated by dalloc
 };
 
 // --- launchers (each enqueues on ctx.stream) ---
@@ -155,6 +156,23 @@ void pk_check(Ctx &c);
 void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
 void pk_destroy(Ctx &c);
 int pk_grid(const Ctx &c);
+// batched persistent sweep over many contexts (one precision and tile shape)
+struct PkBatch;
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs);
+void pk_batch_destroy(PkBatch *b);
+void pk_batch_sweep(PkBatch &b, int mark);
+void pk_batch_use_graph(PkBatch &b, bool on);
+void pk_batch_check(PkBatch &b);
+double pk_batch_elapsed_ms(PkBatch &b);
+void pk_batch_info(const PkBatch &b, long long *updates, int *grid, int *nitems);
+const std::vector<Ctx *> &pk_batch_ctxs(const PkBatch &b);
+// one step of the sweep-only entry Tgen_QDf with Temp prescribed: the part
+// before the sweep, the part after it, and the whole step
+void tgen_pre(Ctx &c);
+void tgen_post(Ctx &c);
+void tgen_step(Ctx &c);
+// device bytes a context allocates at creation (sweep and state arrays)
+size_t state_bytes(const trt_problem &p);
 size_t sweep_bytes(const trt_problem &p, SweepGeom *out);
 void qd_alloc(Ctx &c, QDSystems &s, int nsys);
 // solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess.

@@ -43,7 +43,8 @@ EXPORTS = [
     "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_mloqd_fv", "trt_collapse_mg_ef",
     "trt_cbar_calc", "trt_grey_coefs", "trt_gqd_in_calc", "trt_egp_fv_newt", "trt_convergence", "trt_step",
     "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
-    "trt_sweep_config",
+    "trt_sweep_config", "trt_ctx_device_bytes", "trt_tgen_step", "trt_batch_create", "trt_batch_destroy",
+    "trt_batch_last_error", "trt_batch_sweep_timed", "trt_batch_tgen_step", "trt_batch_use_graph", "trt_batch_info",
 ]
 
 _lib = None
@@ -90,6 +91,17 @@ def load(path: str = LIBPATH):
     L.trt_sweep_config.argtypes = [vp, ip, ip, ip, ip]
     L.trt_sweep_kernel_timed.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
     L.trt_calib_copy.argtypes = [C.c_size_t, C.c_int, _dp]
+    L.trt_ctx_device_bytes.argtypes = [vp]
+    L.trt_ctx_device_bytes.restype = C.c_size_t
+    L.trt_tgen_step.argtypes = [vp]
+    L.trt_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
+    L.trt_batch_destroy.argtypes = [vp]
+    L.trt_batch_last_error.argtypes = [vp]
+    L.trt_batch_last_error.restype = C.c_char_p
+    L.trt_batch_sweep_timed.argtypes = [vp, C.c_int, _dp]
+    L.trt_batch_tgen_step.argtypes = [vp, _dp]
+    L.trt_batch_use_graph.argtypes = [vp, C.c_int]
+    L.trt_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), ip, ip]
     _lib = L
     return L
 
@@ -247,3 +259,62 @@ class Context:
         n, ms = C.c_longlong(), C.c_double()
         self.L.trt_sweep_stats(self.h, C.byref(n), C.byref(ms))
         return n.value, ms.value
+
+    def tgen_step(self) -> None:
+        """One Tgen_QDf step (Temp prescribed, one sweep, QD factors)."""
+        self._check(self.L.trt_tgen_step(self.h), "tgen_step")
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.L.trt_ctx_device_bytes(self.h))
+
+
+class Batch:
+    """Contexts swept together in one persistent launch (a trt_batch).  The
+    contexts must stay open while the batch exists."""
+
+    def __init__(self, ctxs, graph: bool = False):
+        self.L = load()
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = C.c_void_p()
+        rc = self.L.trt_batch_create(C.byref(h), arr, len(self.ctxs))
+        if rc != 0:
+            raise TrtError(f"trt_batch_create failed ({rc}): {self.L.trt_batch_last_error(None).decode()}")
+        self.h = h
+        if graph:
+            self._check(self.L.trt_batch_use_graph(self.h, 1), "batch_use_graph")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.trt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise TrtError(f"{what} failed ({rc}): {self.L.trt_batch_last_error(self.h).decode()}")
+
+    def sweep_timed(self, nsweeps: int) -> float:
+        ms = C.c_double()
+        self._check(self.L.trt_batch_sweep_timed(self.h, int(nsweeps), C.byref(ms)), "batch_sweep_timed")
+        return ms.value
+
+    def tgen_step(self) -> float:
+        ms = C.c_double()
+        self._check(self.L.trt_batch_tgen_step(self.h, C.byref(ms)), "batch_tgen_step")
+        return ms.value
+
+    def info(self):
+        """(updates per batched sweep, persistent grid, work items)."""
+        u, g, n = C.c_longlong(), C.c_int(), C.c_int()
+        self._check(self.L.trt_batch_info(self.h, C.byref(u), C.byref(g), C.byref(n)), "batch_info")
+        return u.value, g.value, n.value

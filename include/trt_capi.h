@@ -63,8 +63,11 @@ typedef struct {
 int trt_ctx_create(trt_ctx **ctx, const trt_problem *p);
 int trt_ctx_destroy(trt_ctx *ctx);
 const char *trt_last_error(const trt_ctx *ctx);
-/* device bytes held by a context for problem p (without creating it) */
+/* device bytes a context for problem p allocates at creation (without creating
+ * it); full time steps add the QD systems on first use */
 size_t trt_ctx_bytes(const trt_problem *p);
+/* device bytes a context holds now */
+size_t trt_ctx_device_bytes(const trt_ctx *ctx);
 
 /* host <-> device copies by reference array name, Fortran shapes.  n is the
  * number of doubles the caller's buffer holds (must equal the array size).
@@ -131,6 +134,32 @@ int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
 /* The sweep configuration in use: sweep_mode (1, 2 or 3), tile shape, and the
  * persistent launch's grid (workgroups; 0 before the first persistent sweep). */
 int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, int *grid);
+
+/* One step of the sweep-only entry Tgen_QDf (MainSrc/ALGORITHMS.f08:1031-1095)
+ * with the Temp array prescribed: MATERIAL_UPDATE, I_crn_old = I_crn,
+ * TRANSPORT_SCB, COLLAPSE_INTENSITIES, fg_Calc, Cg_Calc.  No low-order solve. */
+int trt_tgen_step(trt_ctx *ctx);
+
+/* ---- Batches ----------------------------------------------------------------
+ * No counterpart in the reference, which sweeps one problem per run.  A batch
+ * sweeps many contexts (uneven grids, group counts and quadratures; one sweep
+ * precision and tile shape) in one persistent launch.  The contexts stay owned
+ * by the caller, must outlive the batch, and keep their own streams: each
+ * batched sweep starts after the work already enqueued on every member's
+ * stream, and later work on a member's stream waits for it. */
+typedef struct trt_batch trt_batch;
+int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n);
+int trt_batch_destroy(trt_batch *b);
+const char *trt_batch_last_error(const trt_batch *b);
+/* nsweeps batched TRANSPORT_SCB sweeps (each as trt_transport_scb on every
+ * member); ms receives their time on the batch stream */
+int trt_batch_sweep_timed(trt_batch *b, int nsweeps, double *ms);
+/* one Tgen_QDf step of every member, the sweeps in one launch; sweep_ms (may
+ * be NULL) receives the batched sweep's time */
+int trt_batch_tgen_step(trt_batch *b, double *sweep_ms);
+/* on = 1: the batched sweep is captured once in a HIP graph and replayed */
+int trt_batch_use_graph(trt_batch *b, int on);
+int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems);
 
 #ifdef __cplusplus
 }
