@@ -8,10 +8,6 @@
 using trt::Ctx;
 using trt::Error;
 
-struct trt_ctx {
-    Ctx c;
-};
-
 struct trt_batch {
     trt::PkBatch *b = nullptr;
     std::string err;

@@ -1,0 +1,367 @@
+// Request loop with continuous batching (no counterpart in the reference, which
+// runs one problem per process).
+//
+// A request is a problem (trt_problem), an optional prescribed temperature
+// field and a number of Tgen_QDf steps (ALGORITHMS.f08:1031-1095: one sweep per
+// step with the factors recomputed, no low-order solve).  Requests wait in
+// arrival order.  At each tick the loop
+//   1. admits waiting requests that have arrived, in order, while the device
+//      bytes of the active set stay within the HBM budget and the active count
+//      within its cap (a request larger than the whole budget fails);
+//   2. runs one Tgen_QDf step of every active request, all sweeps in ONE
+//      persistent launch (the batch is rebuilt only when membership changes);
+//   3. retires finished requests: their outputs are copied to host memory and
+//      their device state is freed, so waiting requests can be admitted.
+// Each request keeps its own device state and stream; the batched sweep does
+// not change any problem's arithmetic, so a request's results are the same
+// alone and in any mix.
+#include "trt_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+namespace trt {
+
+namespace {
+
+// Outputs kept for a finished request (what Tgen_QDf writes per step).
+const char *const kOutputs[] = {"Temp",      "HO_E_avg",   "HO_Eg_avg",  "HO_Eg_edgV", "HO_Eg_edgH",
+                                "fg_avg_xx", "fg_avg_yy",  "fg_avg_xy",  "fg_edgV_xx", "fg_edgV_xy",
+                                "fg_edgH_yy", "fg_edgH_xy", "Cg_L",      "Cg_B",       "Cg_R",
+                                "Cg_T"};
+
+struct Request {
+    // owned copy of the problem and its vectors
+    trt_problem p{};
+    std::vector<double> dx, dy, ox, oy, w, nu, kmult, temp;
+    std::vector<int> mat;
+    int steps = 0, done_steps = 0;
+    double arrival = 0.0, admit = -1.0, done = -1.0;
+    int status = 1;            // 1 waiting, 2 active, 0 done, -1 failed
+    std::string error;
+    size_t bytes = 0;
+    long long updates = 0;
+    trt_ctx *ctx = nullptr;
+    std::vector<std::vector<double>> out;
+};
+
+}  // namespace
+
+struct Server {
+    size_t budget = 0;
+    int max_active = 0;
+    std::vector<Request> reqs;
+    // statistics of the last run
+    double wall = 0.0;
+    long long ticks = 0, updates = 0;
+    size_t peak_bytes = 0;
+    int peak_active = 0;
+    double sweep_ms = 0.0;
+};
+
+}  // namespace trt
+
+struct trt_server {
+    trt::Server s;
+    std::string err;
+};
+
+namespace {
+
+using trt::Error;
+using trt::Request;
+using trt::kOutputs;
+
+template <typename T>
+std::vector<T> copy_vec(const T *p, size_t n, const char *what)
+{
+    if (!p) throw Error(std::string("null problem vector ") + what);
+    return std::vector<T>(p, p + n);
+}
+
+void rebind(Request &r)
+{
+    r.p.dx = r.dx.data(); r.p.dy = r.dy.data();
+    r.p.ox = r.ox.data(); r.p.oy = r.oy.data(); r.p.w = r.w.data();
+    r.p.nu = r.nu.data(); r.p.kmult = r.kmult.data(); r.p.mat = r.mat.data();
+}
+
+// device bytes the active requests hold now
+size_t active_bytes(const trt::Server &s)
+{
+    size_t b = 0;
+    for (const Request &r : s.reqs)
+        if (r.status == 2 && r.ctx) b += r.ctx->c.bytes;
+    return b;
+}
+
+// admission estimate: the creation bytes plus room for the sweep's work list
+size_t admission_bytes(const trt_problem &p)
+{
+    const size_t b = trt_ctx_bytes(&p);
+    return b + b / 64 + ((size_t)1 << 20);
+}
+
+void retire(Request &r, double now)
+{
+    trt::Ctx &c = r.ctx->c;
+    r.out.clear();
+    for (const char *name : kOutputs) {
+        const size_t n = trt_array_size(r.ctx, name);
+        std::vector<double> v(n);
+        if (trt_pull(r.ctx, name, v.data(), n) != 0) throw Error(std::string("pull ") + name + ": " + c.err);
+        r.out.push_back(std::move(v));
+    }
+    trt_ctx_destroy(r.ctx);
+    r.ctx = nullptr;
+    r.status = 0;
+    r.done = now;
+}
+
+void fail(Request &r, const std::string &msg, double now)
+{
+    if (r.ctx) trt_ctx_destroy(r.ctx);
+    r.ctx = nullptr;
+    r.status = -1;
+    r.error = msg;
+    r.done = now;
+}
+
+void run(trt::Server &s)
+{
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    std::vector<size_t> order(s.reqs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return s.reqs[a].arrival < s.reqs[b].arrival; });
+    size_t next = 0;                       // first request in arrival order not yet admitted
+    while (next < order.size() && s.reqs[order[next]].status != 1) ++next;
+    s.ticks = 0; s.updates = 0; s.peak_bytes = 0; s.peak_active = 0; s.sweep_ms = 0.0;
+    std::vector<size_t> active;
+    trt::PkBatch *batch = nullptr;
+    bool changed = true;
+    try {
+        while (next < order.size() || !active.empty()) {
+            // 1. admission in arrival order, within the budget and the active cap
+            while (next < order.size()) {
+                Request &r = s.reqs[order[next]];
+                if (r.arrival > now()) break;
+                const size_t need = admission_bytes(r.p);
+                if (need > s.budget) {
+                    fail(r, "request needs more device memory than the whole budget", now());
+                    ++next;
+                    continue;
+                }
+                if ((int)active.size() >= s.max_active || active_bytes(s) + need > s.budget) break;
+                trt_ctx *ctx = nullptr;
+                if (trt_ctx_create(&ctx, &r.p) != 0) {
+                    fail(r, std::string("context creation: ") + trt_last_error(nullptr), now());
+                    ++next;
+                    continue;
+                }
+                r.ctx = ctx;
+                if (!r.temp.empty() && trt_push(ctx, "Temp", r.temp.data(), r.temp.size()) != 0)
+                    throw Error("push Temp: " + ctx->c.err);
+                r.bytes = ctx->c.bytes;
+                r.admit = now();
+                r.status = 2;
+                active.push_back(order[next]);
+                ++next;
+                changed = true;
+            }
+            if (active.empty()) {
+                if (next < order.size()) {
+                    const double wait = s.reqs[order[next]].arrival - now();
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+                }
+                continue;
+            }
+            s.peak_active = std::max(s.peak_active, (int)active.size());
+            s.peak_bytes = std::max(s.peak_bytes, active_bytes(s));
+            // 2. one Tgen_QDf step of every active request, the sweeps batched
+            if (changed) {
+                trt::pk_batch_destroy(batch);
+                batch = nullptr;
+                std::vector<trt::Ctx *> cs;
+                for (size_t i : active) cs.push_back(&s.reqs[i].ctx->c);
+                batch = trt::pk_batch_create(cs);
+                changed = false;
+            }
+            for (size_t i : active) trt::tgen_pre(s.reqs[i].ctx->c);
+            trt::pk_batch_sweep(*batch, 3);
+            for (size_t i : active) trt::tgen_post(s.reqs[i].ctx->c);
+            for (size_t i : active) TRT_CHECK(hipStreamSynchronize(s.reqs[i].ctx->c.stream));
+            s.sweep_ms += trt::pk_batch_elapsed_ms(*batch);
+            trt::pk_batch_check(*batch);
+            ++s.ticks;
+            // 3. retire finished requests
+            std::vector<size_t> still;
+            s.peak_bytes = std::max(s.peak_bytes, active_bytes(s));
+            for (size_t i : active) {
+                Request &r = s.reqs[i];
+                const trt::Ctx &c = r.ctx->c;
+                r.bytes = c.bytes;
+                r.done_steps += 1;
+                const long long u = (long long)c.nx * c.ny * c.nm * c.ng;
+                r.updates += u;
+                s.updates += u;
+                if (r.done_steps >= r.steps) {
+                    retire(r, now());
+                    changed = true;
+                } else {
+                    still.push_back(i);
+                }
+            }
+            if (changed) {
+                // the batch refers to contexts that are gone
+                trt::pk_batch_destroy(batch);
+                batch = nullptr;
+            }
+            active.swap(still);
+        }
+    } catch (...) {
+        trt::pk_batch_destroy(batch);
+        for (size_t i : active) fail(s.reqs[i], "aborted by an error of the request loop", now());
+        s.wall = now();
+        throw;
+    }
+    trt::pk_batch_destroy(batch);
+    s.wall = now();
+}
+
+template <typename F>
+int sguarded(trt_server *sv, F &&f)
+{
+    if (!sv) return -1;
+    try {
+        f(sv->s);
+        return 0;
+    } catch (const Error &e) {
+        sv->err = e.msg;
+    } catch (const std::exception &e) {
+        sv->err = e.what();
+    }
+    return -2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trt_server_create(trt_server **out, size_t hbm_budget, int max_active)
+{
+    if (!out) return -1;
+    *out = nullptr;
+    if (hbm_budget == 0 || max_active < 1) return -1;
+    trt_server *sv = new trt_server();
+    sv->s.budget = hbm_budget;
+    sv->s.max_active = max_active;
+    *out = sv;
+    return 0;
+}
+
+int trt_server_destroy(trt_server *sv)
+{
+    if (!sv) return -1;
+    for (Request &r : sv->s.reqs)
+        if (r.ctx) trt_ctx_destroy(r.ctx);
+    delete sv;
+    return 0;
+}
+
+const char *trt_server_last_error(const trt_server *sv) { return sv ? sv->err.c_str() : ""; }
+
+int trt_server_submit(trt_server *sv, const trt_problem *p, const double *temp, int steps, double arrival_s,
+                      int *id)
+{
+    return sguarded(sv, [&](trt::Server &s) {
+        if (!p) throw Error("null problem");
+        if (steps < 1) throw Error("steps must be positive");
+        if (p->sweep_mode == 1) throw Error("served requests use the tile engine (sweep_mode 0, 2 or 3)");
+        Request r;
+        r.p = *p;
+        const size_t na = (size_t)p->nx * p->ny;
+        if (p->nx < 2 || p->ny < 2 || p->nm < 4 || p->ng < 1 || p->nmat < 1) throw Error("invalid problem sizes");
+        r.dx = copy_vec(p->dx, p->nx, "dx");
+        r.dy = copy_vec(p->dy, p->ny, "dy");
+        r.ox = copy_vec(p->ox, p->nm, "ox");
+        r.oy = copy_vec(p->oy, p->nm, "oy");
+        r.w = copy_vec(p->w, p->nm, "w");
+        r.nu = copy_vec(p->nu, (size_t)p->ng + 1, "nu");
+        r.kmult = copy_vec(p->kmult, p->nmat, "kmult");
+        r.mat = copy_vec(p->mat, na, "mat");
+        if (temp) r.temp.assign(temp, temp + na);
+        r.steps = steps;
+        r.arrival = arrival_s;
+        s.reqs.push_back(std::move(r));
+        rebind(s.reqs.back());
+        if (id) *id = (int)s.reqs.size() - 1;
+    });
+}
+
+int trt_server_run(trt_server *sv)
+{
+    return sguarded(sv, [&](trt::Server &s) {
+        for (Request &r : s.reqs) rebind(r);
+        run(s);
+    });
+}
+
+const char *trt_server_request_error(const trt_server *sv, int id)
+{
+    if (!sv || id < 0 || id >= (int)sv->s.reqs.size()) return "";
+    return sv->s.reqs[id].error.c_str();
+}
+
+int trt_server_request(const trt_server *sv, int id, trt_request_stats *out)
+{
+    if (!sv || !out || id < 0 || id >= (int)sv->s.reqs.size()) return -1;
+    const Request &r = sv->s.reqs[id];
+    out->arrival_s = r.arrival;
+    out->admit_s = r.admit;
+    out->done_s = r.done;
+    out->updates = r.updates;
+    out->device_bytes = r.bytes;
+    out->steps = r.done_steps;
+    out->status = r.status;
+    return 0;
+}
+
+int trt_server_summary(const trt_server *sv, trt_server_stats *out)
+{
+    if (!sv || !out) return -1;
+    const trt::Server &s = sv->s;
+    out->wall_s = s.wall;
+    out->ticks = s.ticks;
+    out->updates = s.updates;
+    out->sweep_ms = s.sweep_ms;
+    out->peak_bytes = s.peak_bytes;
+    out->peak_active = s.peak_active;
+    return 0;
+}
+
+size_t trt_server_output_size(const trt_server *sv, int id, const char *name)
+{
+    if (!sv || !name || id < 0 || id >= (int)sv->s.reqs.size()) return 0;
+    const Request &r = sv->s.reqs[id];
+    if (r.status != 0) return 0;
+    for (size_t k = 0; k < sizeof(kOutputs) / sizeof(kOutputs[0]); ++k)
+        if (!strcmp(kOutputs[k], name)) return r.out[k].size();
+    return 0;
+}
+
+int trt_server_output(const trt_server *sv, int id, const char *name, double *host, size_t n)
+{
+    const size_t m = trt_server_output_size(sv, id, name);
+    if (m == 0 || !host || n != m) return -1;
+    const Request &r = sv->s.reqs[id];
+    for (size_t k = 0; k < sizeof(kOutputs) / sizeof(kOutputs[0]); ++k)
+        if (!strcmp(kOutputs[k], name)) std::memcpy(host, r.out[k].data(), m * sizeof(double));
+    return 0;
+}
+
+}  // extern "C"

@@ -122,6 +122,15 @@ struct Ctx {
     size_t bytes = 0;             // device bytes allocated by dalloc
 };
 
+}  // namespace trt
+
+// the C ABI's context handle
+struct trt_ctx {
+    trt::Ctx c;
+};
+
+namespace trt {
+
 // --- launchers (each enqueues on ctx.stream) ---
 void init_state(Ctx &c);
 void material_update(Ctx &c);

@@ -45,6 +45,9 @@ EXPORTS = [
     "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
     "trt_sweep_config", "trt_ctx_device_bytes", "trt_tgen_step", "trt_batch_create", "trt_batch_destroy",
     "trt_batch_last_error", "trt_batch_sweep_timed", "trt_batch_tgen_step", "trt_batch_use_graph", "trt_batch_info",
+    "trt_server_create", "trt_server_destroy", "trt_server_last_error", "trt_server_submit", "trt_server_run",
+    "trt_server_request", "trt_server_request_error", "trt_server_summary", "trt_server_output_size",
+    "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
 ]
 
 _lib = None
@@ -52,6 +55,16 @@ _lib = None
 
 class TrtError(RuntimeError):
     pass
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("updates", C.c_longlong), ("device_bytes", C.c_size_t), ("steps", C.c_int), ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("ticks", C.c_longlong), ("updates", C.c_longlong), ("sweep_ms", C.c_double),
+                ("peak_bytes", C.c_size_t), ("peak_active", C.c_int)]
 
 
 def load(path: str = LIBPATH):
@@ -102,6 +115,22 @@ def load(path: str = LIBPATH):
     L.trt_batch_tgen_step.argtypes = [vp, _dp]
     L.trt_batch_use_graph.argtypes = [vp, C.c_int]
     L.trt_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), ip, ip]
+    L.trt_server_create.argtypes = [C.POINTER(vp), C.c_size_t, C.c_int]
+    L.trt_server_destroy.argtypes = [vp]
+    L.trt_server_last_error.argtypes = [vp]
+    L.trt_server_last_error.restype = C.c_char_p
+    L.trt_server_submit.argtypes = [vp, C.POINTER(TrtProblem), _dp, C.c_int, C.c_double, ip]
+    L.trt_server_run.argtypes = [vp]
+    L.trt_server_request.argtypes = [vp, C.c_int, C.POINTER(RequestStats)]
+    L.trt_server_request_error.argtypes = [vp, C.c_int]
+    L.trt_server_request_error.restype = C.c_char_p
+    L.trt_server_summary.argtypes = [vp, C.POINTER(ServerStats)]
+    L.trt_server_output_size.argtypes = [vp, C.c_int, C.c_char_p]
+    L.trt_server_output_size.restype = C.c_size_t
+    L.trt_server_output.argtypes = [vp, C.c_int, C.c_char_p, _dp, C.c_size_t]
+    L.trt_autotune_tile.argtypes = [C.POINTER(TrtProblem), C.c_int, C.c_int, C.c_int, ip, ip, _dp, _dp]
+    L.trt_autotune_shapes.argtypes = [C.c_int, ip, ip, C.c_int]
+    L.trt_autotune_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -318,3 +347,31 @@ class Batch:
         u, g, n = C.c_longlong(), C.c_int(), C.c_int()
         self._check(self.L.trt_batch_info(self.h, C.byref(u), C.byref(g), C.byref(n)), "batch_info")
         return u.value, g.value, n.value
+
+
+def autotune_shapes(precision: int):
+    """Tile shapes the tile engine offers for the precision."""
+    L = load()
+    n = L.trt_autotune_shapes(int(precision), None, None, 0)
+    tx, ty = (C.c_int * n)(), (C.c_int * n)()
+    L.trt_autotune_shapes(int(precision), tx, ty, n)
+    return list(zip(tx, ty))
+
+
+def autotune_tile(probs, precision: int = 1, sweeps: int = 3):
+    """Fastest tile shape for a batched sweep of probs: (tile, best ms, {shape: ms})."""
+    L = load()
+    structs, keep = [], []
+    for p in probs:
+        s, k = make_problem_struct(p, precision)
+        structs.append(s)
+        keep.append(k)
+    arr = (TrtProblem * len(structs))(*structs)
+    shapes = autotune_shapes(precision)
+    tx, ty, best = C.c_int(), C.c_int(), C.c_double()
+    all_ms = (C.c_double * len(shapes))()
+    rc = L.trt_autotune_tile(arr, len(structs), int(precision), int(sweeps), C.byref(tx), C.byref(ty), C.byref(best),
+                             all_ms)
+    if rc != 0:
+        raise TrtError(f"trt_autotune_tile failed ({rc}): {L.trt_autotune_last_error().decode()}")
+    return (tx.value, ty.value), best.value, dict(zip(shapes, list(all_ms)))

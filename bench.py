@@ -76,6 +76,41 @@ def traffic_from_profiles(precision: int):
     return d.get(key, {}).get("hbm_bytes_per_launch")
 
 
+def batch_mix(trt, budget_gib: float, base_seed: int = 20240611):
+    """Problems of the batched distribution (grid 64..512, 16..64 groups, S8..S16), in
+    draw order, that fit the FP64 device budget together."""
+    probs, tot = [], 0
+    for i in range(64):
+        p = trt.problem.cfg3_problem(base_seed, i)
+        b = trt.native.device_bytes(p, precision=0, tile=(2, 4))
+        if tot + b <= budget_gib * 2**30:
+            probs.append(p)
+            tot += b
+    return probs
+
+
+def measure_batch(trt, probs, precision: int, steps: int):
+    """Batched persistent sweep of probs vs the same problems swept one at a time."""
+    tile = (4, 2) if precision else (2, 4)
+    ctxs = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]
+    try:
+        upd = sum(p.updates_per_sweep() for p in probs)
+        alone_ms = 0.0
+        for c in ctxs:
+            c.sweep_timed(1)
+            alone_ms += c.sweep_timed(2) / 2
+        with trt.native.Batch(ctxs, graph=True) as b:
+            b.sweep_timed(1)
+            ms = b.sweep_timed(steps) / steps
+            _, grid, nitems = b.info()
+    finally:
+        for c in ctxs:
+            c.close()
+    return {"updates_per_sweep": upd, "ms_per_sweep": ms, "updates_per_s": upd / (ms * 1e-3),
+            "one_at_a_time_updates_per_s": upd / (alone_ms * 1e-3), "tile": list(tile), "grid": grid,
+            "work_items": nitems}
+
+
 def cpu_baseline(trt, budget_s: float = 12.0):
     """The FP64 CPU restatement (oracle) of TRANSPORT_SCB + COLLAPSE_INTENSITIES on the same
     256x256 S16 grid, OpenMP over groups as the reference, one group per thread."""
@@ -115,6 +150,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
                     help="measure one form only (profiling runs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop load")
+    ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
+    ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
@@ -167,6 +207,24 @@ def main(argv=None):
         out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
+    if not args.no_batch and not args.only:
+        probs = batch_mix(trt, args.batch_gib)
+        print(f"[bench] batch of {len(probs)} problems", file=sys.stderr, flush=True)
+        bt = {"problems": len(probs),
+              "mix": [f"{p.nx}x{p.ny} S{p.order} G{p.ng}" for p in probs],
+              "note": "problems of the batched distribution (grid 64..512, 16..64 groups, S8..S16) in draw order "
+                      f"that fit {args.batch_gib:g} GiB in FP64; one persistent launch per sweep, HIP-graph replay"}
+        for prec, key in ((1, "fp32"), (0, "fp64")):
+            bt[key] = measure_batch(trt, probs, prec, max(2, args.steps // 2))
+        bt["fp32_over_fp64"] = bt["fp32"]["updates_per_s"] / bt["fp64"]["updates_per_s"]
+        out["batched"] = bt
+    if not args.no_serve and not args.only:
+        print(f"[bench] request loop, {args.serve_requests} requests at {args.serve_rate}/s", file=sys.stderr,
+              flush=True)
+        sv = trt.serve.run_load(args.serve_rate, args.serve_requests, steps=20, budget_gib=200.0, precision=1)
+        sv["note"] = ("requests drawn from the batched distribution, 20 Tgen_QDf steps each (one sweep per step), "
+                      "Poisson arrivals, HBM budget 200 GiB, FP32 sweep; latency = completion - arrival")
+        out["serving"] = sv
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(trt)

@@ -161,6 +161,56 @@ int trt_batch_tgen_step(trt_batch *b, double *sweep_ms);
 int trt_batch_use_graph(trt_batch *b, int on);
 int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems);
 
+/* Tile-shape autotuner (no counterpart in the reference).  Creates contexts
+ * for the n problems with each tile shape the tile engine offers for the
+ * precision (trt_autotune_shapes lists them; their count is the return value),
+ * times `sweeps` batched sweeps of each and returns the fastest shape.
+ * all_ms (may be NULL) receives the ms per sweep of every shape in list order.
+ * The problems' own precision, tile and sweep_mode fields are ignored. */
+int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps, int *tile_x, int *tile_y,
+                      double *best_ms, double *all_ms);
+int trt_autotune_shapes(int precision, int *tile_x, int *tile_y, int cap);
+const char *trt_autotune_last_error(void);
+
+/* ---- Request loop -----------------------------------------------------------
+ * No counterpart in the reference.  Requests (a problem, an optional
+ * prescribed Temp(N_x,N_y) field, a number of Tgen_QDf steps, an arrival time in
+ * seconds from the start of trt_server_run) are served with continuous
+ * batching: at every tick the loop admits arrived requests in arrival order
+ * while the active set's device bytes stay within hbm_budget and its size
+ * within max_active, runs one Tgen_QDf step of every active request with all
+ * sweeps in one persistent launch, and retires finished requests (outputs
+ * copied to host, device state freed).  A request larger than the whole budget
+ * fails (status -1).  The problem's vectors are copied at submission. */
+typedef struct trt_server trt_server;
+typedef struct {
+    double arrival_s, admit_s, done_s;  /* seconds from the start of the run; -1 if never */
+    long long updates;                  /* sweep cell-angle-group updates performed */
+    size_t device_bytes;                /* device bytes held while active */
+    int steps;                          /* steps completed */
+    int status;                         /* 0 done, 1 waiting, 2 active, -1 failed */
+} trt_request_stats;
+typedef struct {
+    double wall_s;                      /* duration of the last run */
+    long long ticks, updates;
+    double sweep_ms;                    /* summed batched-sweep time */
+    size_t peak_bytes;                  /* largest device bytes of the active set */
+    int peak_active;                    /* largest active set */
+} trt_server_stats;
+int trt_server_create(trt_server **out, size_t hbm_budget, int max_active);
+int trt_server_destroy(trt_server *s);
+const char *trt_server_last_error(const trt_server *s);
+int trt_server_submit(trt_server *s, const trt_problem *p, const double *temp, int steps, double arrival_s,
+                      int *id);
+int trt_server_run(trt_server *s);
+int trt_server_request(const trt_server *s, int id, trt_request_stats *out);
+const char *trt_server_request_error(const trt_server *s, int id);
+int trt_server_summary(const trt_server *s, trt_server_stats *out);
+/* outputs of a finished request: Temp, HO_E_avg, HO_Eg_avg, HO_Eg_edgV,
+ * HO_Eg_edgH, the seven fg_* factors and Cg_L/B/R/T, Fortran shapes */
+size_t trt_server_output_size(const trt_server *s, int id, const char *name);
+int trt_server_output(const trt_server *s, int id, const char *name, double *host, size_t n);
+
 #ifdef __cplusplus
 }
 #endif
