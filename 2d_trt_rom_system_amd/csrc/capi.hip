@@ -54,52 +54,87 @@ void validate(const trt_problem *p)
 
 extern "C" {
 
-int trt_ctx_create(trt_ctx **out, const trt_problem *p)
+}  // extern "C"
+
+namespace trt {
+
+trt_ctx *ctx_create(const trt_problem *p, bool pooled, Arena *arena)
 {
-    if (!out) return -1;
-    *out = nullptr;
     trt_ctx *ctx = nullptr;
     try {
         validate(p);
         ctx = new trt_ctx();
         Ctx &c = ctx->c;
+        c.pooled = pooled;
+        c.arena = arena;
         c.p = *p;
         c.nx = p->nx; c.ny = p->ny; c.nm = p->nm; c.ng = p->ng; c.nmat = p->nmat;
         c.precision = p->precision;
         c.lin_rtol = p->lin_rtol > 0 ? p->lin_rtol : 1e-13;
         c.lin_maxit = p->lin_maxit > 0 ? p->lin_maxit : 2000;
         if (const char *v = getenv("TRT_VERBOSE")) c.verbose = atoi(v);
-        TRT_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        if (pooled) c.stream = cache_stream_get();
+        else TRT_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         TRT_CHECK(hipEventCreate(&c.ev0));
         TRT_CHECK(hipEventCreate(&c.ev1));
         trt::init_state(c);
         // the problem's host vectors are not kept
         c.p.dx = c.p.dy = c.p.ox = c.p.oy = c.p.w = c.p.nu = c.p.kmult = nullptr;
         c.p.mat = nullptr;
-        *out = ctx;
+        return ctx;
+    } catch (...) {
+        if (ctx) ctx_destroy(ctx);
+        throw;
+    }
+}
+
+void ctx_destroy(trt_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    trt::pk_destroy(c);
+    trt::direct_destroy(c);
+    for (size_t k = 0; k < c.owned.size(); ++k) {
+        if (c.arena) arena_release(c.arena, c.owned[k], c.owned_bytes[k]);
+        else if (c.pooled) cache_release(c.owned[k], c.owned_bytes[k]);
+        else (void)hipFree(c.owned[k]);
+    }
+    if (c.hbuf) {
+        if (c.pooled) cache_pinned_put(c.hbuf, 4096);
+        else (void)hipHostFree(c.hbuf);
+    }
+    if (c.ev0) (void)hipEventDestroy(c.ev0);
+    if (c.ev1) (void)hipEventDestroy(c.ev1);
+    if (c.stream) {
+        if (c.pooled) cache_stream_put(c.stream);
+        else (void)hipStreamDestroy(c.stream);
+    }
+    delete ctx;
+}
+
+}  // namespace trt
+
+extern "C" {
+
+int trt_ctx_create(trt_ctx **out, const trt_problem *p)
+{
+    if (!out) return -1;
+    *out = nullptr;
+    try {
+        *out = trt::ctx_create(p, false);
         return 0;
     } catch (const Error &e) {
         g_create_error = e.msg;
     } catch (const std::exception &e) {
         g_create_error = e.what();
     }
-    if (ctx) trt_ctx_destroy(ctx);
     return -2;
 }
 
 int trt_ctx_destroy(trt_ctx *ctx)
 {
     if (!ctx) return -1;
-    Ctx &c = ctx->c;
-    if (c.stream) (void)hipStreamSynchronize(c.stream);
-    trt::pk_destroy(c);
-    trt::direct_destroy(c);
-    for (void *p : c.owned) (void)hipFree(p);
-    if (c.hbuf) (void)hipHostFree(c.hbuf);
-    if (c.ev0) (void)hipEventDestroy(c.ev0);
-    if (c.ev1) (void)hipEventDestroy(c.ev1);
-    if (c.stream) (void)hipStreamDestroy(c.stream);
-    delete ctx;
+    trt::ctx_destroy(ctx);
     return 0;
 }
 

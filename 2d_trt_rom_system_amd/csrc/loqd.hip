@@ -1262,8 +1262,7 @@ int step(Ctx &c, int its_out[4])
         if (rt_its >= rt_start) {
             transport_sweep(c);              // AL:541-542
             sweeps++;
-            collapse_intensities(c);         // AL:551-553
-            fg_calc(c);                      // AL:568-569
+            collapse_with_fg(c);             // AL:551-553 and fg_Calc AL:568-569
             cg_calc(c);                      // AL:572
         }
         dfill(c, c.arr("MGQD_E_avg_MGQDold2"), 0.0, nc);   // AL:676-679
@@ -1327,8 +1326,7 @@ void tgen_pre(Ctx &c)
 
 void tgen_post(Ctx &c)
 {
-    collapse_intensities(c);
-    fg_calc(c);
+    collapse_with_fg(c);
     cg_calc(c);
 }
 

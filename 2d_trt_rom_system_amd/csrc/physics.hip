@@ -66,54 +66,84 @@ void material_update(Ctx &c)
 }
 
 // ---------------------------------------------------------------------------
-// COLLAPSE_INTENSITIES: sum the 4 quadrant partials of every moment
+// COLLAPSE_INTENSITIES: sum the 4 quadrant partials of every moment, and
+// optionally fg_Calc's ratios (MLOQD_SOLVES.f08:646-674) in the same pass.
+//
+// The partials are [location][quadrant][group][moment]; the outputs are the
+// reference's (location, group) arrays with the location fastest.  A block
+// takes 64 locations x 4 groups of one location kind: it reads the partials
+// with 4 adjacent lanes covering 4 adjacent groups (128 contiguous bytes per
+// location and quadrant), stages the results in LDS, and writes each output
+// as 64 consecutive locations per group.
 // ---------------------------------------------------------------------------
 struct CollArgs {
     int nx, ny, ng;
-    double cinv;
+    double cinv, c;
     const double *Pavg, *PV, *PH;
     double *Hxx, *Hyy, *Hxy, *Ea;           // cell averages
     double *Vxx, *Vxy, *Fx, *Ev;            // V faces
     double *Hyy2, *Hxy2, *Fy, *Eh;          // H faces
+    double *fxx, *fyy, *fxy, *fvxx, *fvxy, *fhyy, *fhxy;   // fg_Calc outputs, or null
+    int nbl[3];                             // location blocks of each kind
 };
 
-__global__ void collapse_kernel(CollArgs a)
+constexpr int CL_LOC = 64, CL_G = 4, CL_OUT = 7;
+
+__global__ __launch_bounds__(256) void collapse_kernel(CollArgs a)
 {
-    const size_t na = (size_t)a.nx * a.ny, nv = (size_t)(a.nx + 1) * a.ny, nh = (size_t)a.nx * (a.ny + 1);
-    const size_t total = (na + nv + nh) * a.ng;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        size_t r = idx;
-        if (r < na * a.ng) {
-            const size_t loc = r % na; const int g = (int)(r / na);
-            double s[4] = {0, 0, 0, 0};
-            for (int q = 0; q < 4; ++q) {
-                const double *p = a.Pavg + ((loc * 4 + q) * a.ng + g) * 4;
-                s[0] += p[0]; s[1] += p[1]; s[2] += p[2]; s[3] += p[3];
+    __shared__ double st[CL_OUT][CL_G][CL_LOC];
+    const int ngb = (a.ng + CL_G - 1) / CL_G;
+    int b = blockIdx.x;
+    int kind = 0;
+    while (kind < 2 && b >= a.nbl[kind] * ngb) { b -= a.nbl[kind] * ngb; ++kind; }
+    const int gb = b % ngb, lb = b / ngb;
+    const size_t nloc = kind == 0 ? (size_t)a.nx * a.ny : kind == 1 ? (size_t)(a.nx + 1) * a.ny
+                                                                    : (size_t)a.nx * (a.ny + 1);
+    const int t = threadIdx.x;
+    const bool fg = a.fxx != nullptr;
+    {
+        const int gl = t & (CL_G - 1), cl = t >> 2;
+        const size_t loc = (size_t)lb * CL_LOC + cl;
+        const int g = gb * CL_G + gl;
+        if (loc < nloc && g < a.ng) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            if (kind < 2) {
+                const double *P = kind == 0 ? a.Pavg : a.PV;
+                for (int q = 0; q < 4; ++q) {
+                    const double4 v = *reinterpret_cast<const double4 *>(P + ((loc * 4 + q) * a.ng + g) * 4);
+                    s0 += v.x; s1 += v.y; s2 += v.z; s3 += v.w;
+                }
+            } else {
+                for (int q = 0; q < 4; ++q) {
+                    const double *p = a.PH + ((loc * 4 + q) * a.ng + g) * 5;
+                    s0 += p[0]; s1 += p[1]; s2 += p[2]; s3 += p[4];
+                }
             }
-            a.Hxx[r] = s[0]; a.Hyy[r] = s[1]; a.Hxy[r] = s[2]; a.Ea[r] = s[3] * a.cinv;
-            continue;
-        }
-        r -= na * a.ng;
-        if (r < nv * a.ng) {
-            const size_t loc = r % nv; const int g = (int)(r / nv);
-            double s[4] = {0, 0, 0, 0};
-            for (int q = 0; q < 4; ++q) {
-                const double *p = a.PV + ((loc * 4 + q) * a.ng + g) * 4;
-                s[0] += p[0]; s[1] += p[1]; s[2] += p[2]; s[3] += p[3];
+            const double e = s3 * a.cinv;
+            st[0][gl][cl] = s0; st[1][gl][cl] = s1; st[2][gl][cl] = s2; st[3][gl][cl] = e;
+            if (fg) {
+                const double ce = a.c * e;
+                st[4][gl][cl] = s0 / ce;
+                st[5][gl][cl] = s1 / ce;
+                if (kind == 0) st[6][gl][cl] = s2 / ce;
             }
-            a.Vxx[r] = s[0]; a.Vxy[r] = s[1]; a.Fx[r] = s[2]; a.Ev[r] = s[3] * a.cinv;
-            continue;
         }
-        r -= nv * a.ng;
-        {
-            const size_t loc = r % nh; const int g = (int)(r / nh);
-            double s[5] = {0, 0, 0, 0, 0};
-            for (int q = 0; q < 4; ++q) {
-                const double *p = a.PH + ((loc * 4 + q) * a.ng + g) * 5;
-                s[0] += p[0]; s[1] += p[1]; s[2] += p[2]; s[4] += p[4];
-            }
-            a.Hyy2[r] = s[0]; a.Hxy2[r] = s[1]; a.Fy[r] = s[2]; a.Eh[r] = s[4] * a.cinv;
-        }
+    }
+    __syncthreads();
+    const int cl = t & (CL_LOC - 1), gl = t >> 6;
+    const size_t loc = (size_t)lb * CL_LOC + cl;
+    const int g = gb * CL_G + gl;
+    if (loc >= nloc || g >= a.ng) return;
+    const size_t o = loc + nloc * g;
+    if (kind == 0) {
+        a.Hxx[o] = st[0][gl][cl]; a.Hyy[o] = st[1][gl][cl]; a.Hxy[o] = st[2][gl][cl]; a.Ea[o] = st[3][gl][cl];
+        if (fg) { a.fxx[o] = st[4][gl][cl]; a.fyy[o] = st[5][gl][cl]; a.fxy[o] = st[6][gl][cl]; }
+    } else if (kind == 1) {
+        a.Vxx[o] = st[0][gl][cl]; a.Vxy[o] = st[1][gl][cl]; a.Fx[o] = st[2][gl][cl]; a.Ev[o] = st[3][gl][cl];
+        if (fg) { a.fvxx[o] = st[4][gl][cl]; a.fvxy[o] = st[5][gl][cl]; }
+    } else {
+        a.Hyy2[o] = st[0][gl][cl]; a.Hxy2[o] = st[1][gl][cl]; a.Fy[o] = st[2][gl][cl]; a.Eh[o] = st[3][gl][cl];
+        if (fg) { a.fhyy[o] = st[4][gl][cl]; a.fhxy[o] = st[5][gl][cl]; }
     }
 }
 
@@ -159,21 +189,38 @@ void grey_sums(Ctx &c, const char *ea, const char *ev, const char *eh, const cha
     TRT_CHECK(hipGetLastError());
 }
 
-void collapse_intensities(Ctx &c)
+static void collapse(Ctx &c, bool with_fg)
 {
     CollArgs a{};
     a.nx = c.nx; a.ny = c.ny; a.ng = c.ng;
     a.cinv = 1.0 / C_CONST;     // TRANSPORT_SOLVES.f08:403,475-477 (hard-coded c)
+    a.c = c.p.c;                // fg_Calc divides by c*E with the input c (MLOQD_SOLVES.f08:646-674)
     a.Pavg = c.Pavg; a.PV = c.PV; a.PH = c.PH;
     a.Hxx = c.arr("Hg_avg_xx"); a.Hyy = c.arr("Hg_avg_yy"); a.Hxy = c.arr("Hg_avg_xy"); a.Ea = c.arr("HO_Eg_avg");
     a.Vxx = c.arr("Hg_edgV_xx"); a.Vxy = c.arr("Hg_edgV_xy"); a.Fx = c.arr("HO_Fxg_edgV"); a.Ev = c.arr("HO_Eg_edgV");
     a.Hyy2 = c.arr("Hg_edgH_yy"); a.Hxy2 = c.arr("Hg_edgH_xy"); a.Fy = c.arr("HO_Fyg_edgH"); a.Eh = c.arr("HO_Eg_edgH");
-    const size_t n = ((size_t)c.nx * c.ny + (size_t)(c.nx + 1) * c.ny + (size_t)c.nx * (c.ny + 1)) * c.ng;
-    hipLaunchKernelGGL(collapse_kernel, dim3(nblocks_for(n)), dim3(256), 0, c.stream, a);
+    if (with_fg) {
+        a.fxx = c.arr("fg_avg_xx"); a.fyy = c.arr("fg_avg_yy"); a.fxy = c.arr("fg_avg_xy");
+        a.fvxx = c.arr("fg_edgV_xx"); a.fvxy = c.arr("fg_edgV_xy");
+        a.fhyy = c.arr("fg_edgH_yy"); a.fhxy = c.arr("fg_edgH_xy");
+    }
+    const size_t nl[3] = {(size_t)c.nx * c.ny, (size_t)(c.nx + 1) * c.ny, (size_t)c.nx * (c.ny + 1)};
+    const int ngb = (c.ng + CL_G - 1) / CL_G;
+    size_t nblocks = 0;
+    for (int k = 0; k < 3; ++k) {
+        a.nbl[k] = (int)((nl[k] + CL_LOC - 1) / CL_LOC);
+        nblocks += (size_t)a.nbl[k] * ngb;
+    }
+    if (nblocks > 0x7fffffff) throw Error("grid too large for the collapse kernel");
+    hipLaunchKernelGGL(collapse_kernel, dim3((unsigned)nblocks), dim3(256), 0, c.stream, a);
     TRT_CHECK(hipGetLastError());
     grey_sums(c, "HO_Eg_avg", "HO_Eg_edgV", "HO_Eg_edgH", "HO_Fxg_edgV", "HO_Fyg_edgH",
               "HO_E_avg", "HO_E_edgV", "HO_E_edgH", "HO_Fx_edgV", "HO_Fy_edgH");
 }
+
+void collapse_intensities(Ctx &c) { collapse(c, false); }
+
+void collapse_with_fg(Ctx &c) { collapse(c, true); }
 
 // fg_Calc, MLOQD_SOLVES.f08:646-674
 __global__ void ratio_kernel(const double *H, const double *E, double *f, double c, size_t n)

@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -44,7 +46,12 @@ struct Request {
     size_t bytes = 0;
     long long updates = 0;
     trt_ctx *ctx = nullptr;
-    std::vector<std::vector<double>> out;
+    // outputs, in pinned host memory (copied there while draining; kept until
+    // the server is destroyed): offsets of each output in pin
+    double *pin = nullptr;
+    size_t pin_n = 0;
+    std::vector<size_t> out_off;
+    hipEvent_t drained = nullptr;
 };
 
 }  // namespace
@@ -59,6 +66,12 @@ struct Server {
     size_t peak_bytes = 0;
     int peak_active = 0;
     double sweep_ms = 0.0;
+    double t_admit = 0.0, t_rebuild = 0.0, t_step = 0.0, t_retire = 0.0;   // host seconds per phase
+    ~Server()
+    {
+        for (Request &r : reqs)
+            if (r.pin) (void)hipHostFree(r.pin);
+    }
 };
 
 }  // namespace trt
@@ -88,41 +101,61 @@ void rebind(Request &r)
     r.p.nu = r.nu.data(); r.p.kmult = r.kmult.data(); r.p.mat = r.mat.data();
 }
 
-// device bytes the active requests hold now
-size_t active_bytes(const trt::Server &s)
-{
-    size_t b = 0;
-    for (const Request &r : s.reqs)
-        if (r.status == 2 && r.ctx) b += r.ctx->c.bytes;
-    return b;
-}
-
-// admission estimate: the creation bytes plus room for the sweep's work list
+// admission estimate: the creation bytes plus room for block rounding and the
+// sweep's work list.  Creation itself fails when the arena has no room.
 size_t admission_bytes(const trt_problem &p)
 {
     const size_t b = trt_ctx_bytes(&p);
-    return b + b / 64 + ((size_t)1 << 20);
+    return b + b / 16 + ((size_t)8 << 20);
 }
 
-void retire(Request &r, double now)
+// Retirement, in two halves so the copies overlap the next ticks: the outputs
+// are copied to a pinned buffer on the request's stream (drain_start); once the
+// copy has completed they move to host vectors and the request's device blocks
+// go back to the block cache (drain_finish).
+void drain_start(trt::Server &s, Request &r)
 {
     trt::Ctx &c = r.ctx->c;
-    r.out.clear();
-    for (const char *name : kOutputs) {
-        const size_t n = trt_array_size(r.ctx, name);
-        std::vector<double> v(n);
-        if (trt_pull(r.ctx, name, v.data(), n) != 0) throw Error(std::string("pull ") + name + ": " + c.err);
-        r.out.push_back(std::move(v));
+    const size_t nout = sizeof(kOutputs) / sizeof(kOutputs[0]);
+    r.out_off.assign(nout + 1, 0);
+    for (size_t k = 0; k < nout; ++k) r.out_off[k + 1] = r.out_off[k] + trt_array_size(r.ctx, kOutputs[k]);
+    r.pin_n = r.out_off[nout];
+    TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&r.pin), r.pin_n * sizeof(double), hipHostMallocDefault));
+    for (size_t k = 0; k < nout; ++k)
+        TRT_CHECK(hipMemcpyAsync(r.pin + r.out_off[k], c.arr(kOutputs[k]),
+                                 (r.out_off[k + 1] - r.out_off[k]) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (!r.drained) TRT_CHECK(hipEventCreateWithFlags(&r.drained, hipEventDisableTiming));
+    TRT_CHECK(hipEventRecord(r.drained, c.stream));
+    r.status = 3;
+}
+
+// true when the request's copy had completed (or wait) and it is now done
+bool drain_finish(Request &r, bool wait, double now)
+{
+    if (wait) {
+        TRT_CHECK(hipEventSynchronize(r.drained));
+    } else {
+        const hipError_t e = hipEventQuery(r.drained);
+        if (e == hipErrorNotReady) return false;
+        TRT_CHECK(e);
     }
-    trt_ctx_destroy(r.ctx);
+    (void)hipEventDestroy(r.drained);
+    r.drained = nullptr;
+    trt::ctx_destroy(r.ctx);
     r.ctx = nullptr;
     r.status = 0;
     r.done = now;
+    return true;
 }
 
 void fail(Request &r, const std::string &msg, double now)
 {
-    if (r.ctx) trt_ctx_destroy(r.ctx);
+    if (r.drained) {
+        (void)hipEventSynchronize(r.drained);
+        (void)hipEventDestroy(r.drained);
+        r.drained = nullptr;
+    }
+    if (r.ctx) trt::ctx_destroy(r.ctx);
     r.ctx = nullptr;
     r.status = -1;
     r.error = msg;
@@ -141,68 +174,117 @@ void run(trt::Server &s)
     size_t next = 0;                       // first request in arrival order not yet admitted
     while (next < order.size() && s.reqs[order[next]].status != 1) ++next;
     s.ticks = 0; s.updates = 0; s.peak_bytes = 0; s.peak_active = 0; s.sweep_ms = 0.0;
-    std::vector<size_t> active;
+    s.t_admit = s.t_rebuild = s.t_step = s.t_retire = 0.0;
+    std::vector<size_t> active, draining;
     trt::PkBatch *batch = nullptr;
     bool changed = true;
-    try {
-        while (next < order.size() || !active.empty()) {
-            // 1. admission in arrival order, within the budget and the active cap
-            while (next < order.size()) {
-                Request &r = s.reqs[order[next]];
-                if (r.arrival > now()) break;
-                const size_t need = admission_bytes(r.p);
-                if (need > s.budget) {
-                    fail(r, "request needs more device memory than the whole budget", now());
-                    ++next;
-                    continue;
-                }
-                if ((int)active.size() >= s.max_active || active_bytes(s) + need > s.budget) break;
-                trt_ctx *ctx = nullptr;
-                if (trt_ctx_create(&ctx, &r.p) != 0) {
-                    fail(r, std::string("context creation: ") + trt_last_error(nullptr), now());
-                    ++next;
-                    continue;
-                }
-                r.ctx = ctx;
-                if (!r.temp.empty() && trt_push(ctx, "Temp", r.temp.data(), r.temp.size()) != 0)
-                    throw Error("push Temp: " + ctx->c.err);
-                r.bytes = ctx->c.bytes;
-                r.admit = now();
-                r.status = 2;
-                active.push_back(order[next]);
+    // every request's device state lives in one arena of the budget's size
+    trt::Arena *arena = trt::arena_create(s.budget);
+    const char *venv = getenv("TRT_VERBOSE");
+    const bool trace = venv && atoi(venv) > 0;
+    auto active_bytes = [&]() { return trt::arena_used(arena); };
+
+    // admits arrived requests in order while they fit; their initialisation
+    // runs on their own streams
+    auto admit = [&]() {
+        while (next < order.size()) {
+            Request &r = s.reqs[order[next]];
+            if (r.arrival > now()) break;
+            const size_t need = admission_bytes(r.p);
+            if (need > s.budget) {
+                fail(r, "request needs more device memory than the whole budget", now());
                 ++next;
-                changed = true;
+                continue;
             }
+            if ((int)active.size() >= s.max_active || active_bytes() + need > s.budget) break;
+            const double ta = now();
+            trt_ctx *ctx = nullptr;
+            try {
+                ctx = trt::ctx_create(&r.p, true, arena);
+            } catch (const Error &e) {
+                const bool nomem = e.msg.find("arena exhausted") != std::string::npos;
+                if (nomem && !(active.empty() && draining.empty())) break;   // wait for memory
+                fail(r, nomem ? std::string("request needs more device memory than the whole budget")
+                              : "context creation: " + e.msg, now());
+                ++next;
+                continue;
+            }
+            r.ctx = ctx;
+            if (!r.temp.empty() && trt_push(ctx, "Temp", r.temp.data(), r.temp.size()) != 0)
+                throw Error("push Temp: " + ctx->c.err);
+            r.bytes = ctx->c.bytes;
+            r.admit = now();
+            s.t_admit += r.admit - ta;
+            r.status = 2;
+            active.push_back(order[next]);
+            ++next;
+            changed = true;
+        }
+    };
+    // completes the drains whose copies have finished (all of them with wait)
+    auto drain = [&](bool wait) {
+        const double tr = now();
+        std::vector<size_t> still;
+        for (size_t i : draining)
+            if (!drain_finish(s.reqs[i], wait, now())) still.push_back(i);
+        draining.swap(still);
+        s.t_retire += now() - tr;
+    };
+
+    try {
+        while (next < order.size() || !active.empty() || !draining.empty()) {
+            drain(false);
+            admit();
             if (active.empty()) {
-                if (next < order.size()) {
+                if (!draining.empty()) {
+                    drain(true);
+                } else if (next < order.size()) {
                     const double wait = s.reqs[order[next]].arrival - now();
                     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
                 }
                 continue;
             }
             s.peak_active = std::max(s.peak_active, (int)active.size());
-            s.peak_bytes = std::max(s.peak_bytes, active_bytes(s));
-            // 2. one Tgen_QDf step of every active request, the sweeps batched
-            if (changed) {
+            s.peak_bytes = std::max(s.peak_bytes, active_bytes());
+            // one Tgen_QDf step of every active request, the sweeps batched
+            if (changed || !batch) {
+                const double tb = now();
                 trt::pk_batch_destroy(batch);
                 batch = nullptr;
                 std::vector<trt::Ctx *> cs;
                 for (size_t i : active) cs.push_back(&s.reqs[i].ctx->c);
-                batch = trt::pk_batch_create(cs);
+                batch = trt::pk_batch_create(cs, true);
                 changed = false;
+                s.t_rebuild += now() - tb;
             }
-            for (size_t i : active) trt::tgen_pre(s.reqs[i].ctx->c);
+            const double ts = now();
+            const std::vector<size_t> ticked = active;
+            for (size_t i : ticked) trt::tgen_pre(s.reqs[i].ctx->c);
             trt::pk_batch_sweep(*batch, 3);
-            for (size_t i : active) trt::tgen_post(s.reqs[i].ctx->c);
-            for (size_t i : active) TRT_CHECK(hipStreamSynchronize(s.reqs[i].ctx->c.stream));
+            for (size_t i : ticked) trt::tgen_post(s.reqs[i].ctx->c);
+            // host work while the tick runs: arrivals join the next tick, finished copies complete
+            const double th = now();
+            admit();
+            drain(false);
+            const double host_during = now() - th;
+            for (size_t i : ticked) TRT_CHECK(hipStreamSynchronize(s.reqs[i].ctx->c.stream));
             s.sweep_ms += trt::pk_batch_elapsed_ms(*batch);
             trt::pk_batch_check(*batch);
             ++s.ticks;
-            // 3. retire finished requests
+            s.t_step += now() - ts - host_during;
+            if (trace)
+                fprintf(stderr, "[serve] t=%.3f tick %lld: %zu active, %zu draining, %zu waiting, tick %.1f ms (host %.1f ms), arena %.1f GiB\n",
+                        now(), s.ticks, ticked.size(), draining.size(), order.size() - next, 1e3 * (now() - ts),
+                        1e3 * host_during, arena_used(arena) / 1073741824.0);
+            s.peak_bytes = std::max(s.peak_bytes, active_bytes());
+            // requests that finished start draining; the rest stay active
             std::vector<size_t> still;
-            s.peak_bytes = std::max(s.peak_bytes, active_bytes(s));
             for (size_t i : active) {
                 Request &r = s.reqs[i];
+                if (std::find(ticked.begin(), ticked.end(), i) == ticked.end()) {
+                    still.push_back(i);        // admitted during this tick
+                    continue;
+                }
                 const trt::Ctx &c = r.ctx->c;
                 r.bytes = c.bytes;
                 r.done_steps += 1;
@@ -210,27 +292,34 @@ void run(trt::Server &s)
                 r.updates += u;
                 s.updates += u;
                 if (r.done_steps >= r.steps) {
-                    retire(r, now());
+                    const double tr = now();
+                    drain_start(s, r);
+                    s.t_retire += now() - tr;
+                    draining.push_back(i);
                     changed = true;
                 } else {
                     still.push_back(i);
                 }
             }
+            active.swap(still);
             if (changed) {
-                // the batch refers to contexts that are gone
                 trt::pk_batch_destroy(batch);
                 batch = nullptr;
             }
-            active.swap(still);
         }
     } catch (...) {
         trt::pk_batch_destroy(batch);
         for (size_t i : active) fail(s.reqs[i], "aborted by an error of the request loop", now());
+        for (size_t i : draining) fail(s.reqs[i], "aborted by an error of the request loop", now());
         s.wall = now();
+        trt::arena_destroy(arena);
+        trt::cache_trim();
         throw;
     }
     trt::pk_batch_destroy(batch);
     s.wall = now();
+    trt::arena_destroy(arena);
+    trt::cache_trim();
 }
 
 template <typename F>
@@ -268,7 +357,7 @@ int trt_server_destroy(trt_server *sv)
 {
     if (!sv) return -1;
     for (Request &r : sv->s.reqs)
-        if (r.ctx) trt_ctx_destroy(r.ctx);
+        if (r.ctx) trt::ctx_destroy(r.ctx);
     delete sv;
     return 0;
 }
@@ -341,6 +430,10 @@ int trt_server_summary(const trt_server *sv, trt_server_stats *out)
     out->sweep_ms = s.sweep_ms;
     out->peak_bytes = s.peak_bytes;
     out->peak_active = s.peak_active;
+    out->admit_s = s.t_admit;
+    out->rebuild_s = s.t_rebuild;
+    out->step_s = s.t_step;
+    out->retire_s = s.t_retire;
     return 0;
 }
 
@@ -350,7 +443,7 @@ size_t trt_server_output_size(const trt_server *sv, int id, const char *name)
     const Request &r = sv->s.reqs[id];
     if (r.status != 0) return 0;
     for (size_t k = 0; k < sizeof(kOutputs) / sizeof(kOutputs[0]); ++k)
-        if (!strcmp(kOutputs[k], name)) return r.out[k].size();
+        if (!strcmp(kOutputs[k], name)) return r.out_off[k + 1] - r.out_off[k];
     return 0;
 }
 
@@ -360,7 +453,7 @@ int trt_server_output(const trt_server *sv, int id, const char *name, double *ho
     if (m == 0 || !host || n != m) return -1;
     const Request &r = sv->s.reqs[id];
     for (size_t k = 0; k < sizeof(kOutputs) / sizeof(kOutputs[0]); ++k)
-        if (!strcmp(kOutputs[k], name)) std::memcpy(host, r.out[k].data(), m * sizeof(double));
+        if (!strcmp(kOutputs[k], name)) std::memcpy(host, r.pin + r.out_off[k], m * sizeof(double));
     return 0;
 }
 

@@ -5,16 +5,241 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace trt {
 
 void *Ctx::dalloc(size_t bytes)
 {
     void *p = nullptr;
-    TRT_CHECK(hipMalloc(&p, bytes ? bytes : 8));
+    size_t b = bytes ? bytes : 8;
+    if (arena) {
+        p = arena_alloc(arena, b);
+        if (!p) throw Error("device arena exhausted");
+    } else if (pooled) {
+        p = cache_alloc(b);
+    } else {
+        TRT_CHECK(hipMalloc(&p, b));
+    }
     owned.push_back(p);
-    this->bytes += bytes ? bytes : 8;
+    owned_bytes.push_back(b);
+    this->bytes += b;
     return p;
+}
+
+void Ctx::dfree_last()
+{
+    if (owned.empty()) return;
+    void *p = owned.back();
+    const size_t b = owned_bytes.back();
+    owned.pop_back();
+    owned_bytes.pop_back();
+    this->bytes -= b;
+    if (arena) arena_release(arena, p, b);
+    else if (pooled) cache_release(p, b);
+    else (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// device arena
+// ---------------------------------------------------------------------------
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    std::map<size_t, size_t> free;      // offset -> length, address order, no two adjacent
+    std::mutex m;
+};
+
+Arena *arena_create(size_t bytes)
+{
+    Arena *a = new Arena();
+    a->size = bytes / 256 * 256;
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&a->base), a->size);
+    if (e != hipSuccess) {
+        delete a;
+        throw Error(std::string("device arena of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    a->free.emplace(0, a->size);
+    return a;
+}
+
+void arena_destroy(Arena *a)
+{
+    if (!a) return;
+    (void)hipFree(a->base);
+    delete a;
+}
+
+void *arena_alloc(Arena *a, size_t &bytes)
+{
+    const size_t want = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    std::lock_guard<std::mutex> lk(a->m);
+    auto best = a->free.end();
+    for (auto it = a->free.begin(); it != a->free.end(); ++it)
+        if (it->second >= want && (best == a->free.end() || it->second < best->second)) best = it;
+    if (best == a->free.end()) return nullptr;
+    const size_t off = best->first, len = best->second;
+    a->free.erase(best);
+    // large blocks (intensity buffers) are cut from the top of the range, small
+    // ones from the bottom, so the two kinds do not interleave and freed
+    // contexts leave large contiguous ranges behind
+    const bool large = want >= ((size_t)256 << 20);
+    const size_t at = large ? off + len - want : off;
+    if (len > want) a->free.emplace(large ? off : off + want, len - want);
+    a->used += want;
+    bytes = want;
+    return a->base + at;
+}
+
+void arena_release(Arena *a, void *p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(a->m);
+    size_t off = static_cast<size_t>(static_cast<char *>(p) - a->base), len = bytes;
+    a->used -= bytes;
+    auto nx = a->free.lower_bound(off);
+    if (nx != a->free.end() && off + len == nx->first) {       // merge with the following range
+        len += nx->second;
+        nx = a->free.erase(nx);
+    }
+    if (nx != a->free.begin()) {                              // merge with the preceding range
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += len;
+            return;
+        }
+    }
+    a->free.emplace(off, len);
+}
+
+size_t arena_used(const Arena *a) { return a ? a->used : 0; }
+size_t arena_size(const Arena *a) { return a ? a->size : 0; }
+
+// ---------------------------------------------------------------------------
+// device block cache
+// ---------------------------------------------------------------------------
+namespace {
+struct BlockCache {
+    std::mutex m;
+    std::multimap<size_t, void *> free;   // block size -> block
+    size_t cached = 0;
+    std::multimap<size_t, double *> pinned;   // pinned host buffers by capacity (doubles)
+    std::vector<hipStream_t> streams;         // idle streams
+};
+BlockCache &block_cache()
+{
+    static BlockCache *bc = new BlockCache();   // outlives every context
+    return *bc;
+}
+// blocks of 1 MiB and more are rounded to 2 MiB, smaller ones to 4 KiB
+size_t round_block(size_t b)
+{
+    const size_t g = b >= ((size_t)1 << 20) ? ((size_t)2 << 20) : 4096;
+    return (b + g - 1) / g * g;
+}
+}  // namespace
+
+void *cache_alloc(size_t &bytes)
+{
+    const size_t want = round_block(bytes);
+    BlockCache &bc = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(bc.m);
+        auto it = bc.free.lower_bound(want);
+        // reuse a cached block at most 1/16 larger than asked for
+        if (it != bc.free.end() && it->first <= want + want / 16) {
+            void *p = it->second;
+            bytes = it->first;
+            bc.cached -= it->first;
+            bc.free.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_trim();
+        e = hipMalloc(&p, want);
+    }
+    if (e != hipSuccess) throw Error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = want;
+    return p;
+}
+
+void cache_release(void *p, size_t bytes)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.m);
+    bc.free.emplace(bytes, p);
+    bc.cached += bytes;
+}
+
+void cache_trim()
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.m);
+    for (auto &kv : bc.free) (void)hipFree(kv.second);
+    bc.free.clear();
+    bc.cached = 0;
+    for (auto &kv : bc.pinned) (void)hipHostFree(kv.second);
+    bc.pinned.clear();
+    for (hipStream_t s : bc.streams) (void)hipStreamDestroy(s);
+    bc.streams.clear();
+}
+
+double *cache_pinned_get(size_t n)
+{
+    BlockCache &bc = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(bc.m);
+        auto it = bc.pinned.lower_bound(n);
+        if (it != bc.pinned.end() && it->first == n) {
+            double *p = it->second;
+            bc.pinned.erase(it);
+            return p;
+        }
+    }
+    double *p = nullptr;
+    TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&p), n * sizeof(double), hipHostMallocDefault));
+    return p;
+}
+
+void cache_pinned_put(double *p, size_t n)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.m);
+    bc.pinned.emplace(n, p);
+}
+
+hipStream_t cache_stream_get()
+{
+    BlockCache &bc = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(bc.m);
+        if (!bc.streams.empty()) {
+            hipStream_t s = bc.streams.back();
+            bc.streams.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    TRT_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void cache_stream_put(hipStream_t s)
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.m);
+    bc.streams.push_back(s);
+}
+
+size_t cache_bytes()
+{
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lk(bc.m);
+    return bc.cached;
 }
 
 size_t state_bytes(const trt_problem &p)
@@ -243,7 +468,8 @@ void init_state(Ctx &c)
     }
     c.scratch_n = std::max<size_t>(4096, 8 * (size_t)std::max(nx, ny) * ng + 4096);
     c.scratch = static_cast<double *>(c.dalloc(c.scratch_n * sizeof(double)));
-    TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c.hbuf), 4096 * sizeof(double), hipHostMallocDefault));
+    if (c.pooled) c.hbuf = cache_pinned_get(4096);
+    else TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c.hbuf), 4096 * sizeof(double), hipHostMallocDefault));
 
     sweep_setup(c);
 

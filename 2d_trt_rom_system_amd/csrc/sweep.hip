@@ -436,8 +436,7 @@ void pull_icrn(Ctx &c, int which, double *host)
                            static_cast<const double *>(c.Icrn[which]), tmp, c.nx, c.ny, c.nm, c.ng);
     TRT_CHECK(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c.stream));
     TRT_CHECK(hipStreamSynchronize(c.stream));
-    TRT_CHECK(hipFree(tmp));
-    c.owned.pop_back();
+    c.dfree_last();
 }
 
 void push_icrn(Ctx &c, int which, const double *host)
@@ -452,8 +451,7 @@ void push_icrn(Ctx &c, int which, const double *host)
         hipLaunchKernelGGL(icrn_from_fortran<double>, dim3(1024), dim3(256), 0, c.stream, tmp,
                            static_cast<double *>(c.Icrn[which]), c.nx, c.ny, c.nm, c.ng);
     TRT_CHECK(hipStreamSynchronize(c.stream));
-    TRT_CHECK(hipFree(tmp));
-    c.owned.pop_back();
+    c.dfree_last();
 }
 
 
@@ -542,8 +540,7 @@ void pull_derived(Ctx &c, const char *name, double *host)
                            static_cast<const double *>(c.Icrn[c.icur]), c.bcin, tmp, kind, c.nx, c.ny, c.nm, c.ng);
     TRT_CHECK(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c.stream));
     TRT_CHECK(hipStreamSynchronize(c.stream));
-    TRT_CHECK(hipFree(tmp));
-    c.owned.pop_back();
+    c.dfree_last();
 }
 
 // (KapE + 1/(c dt), RT_Src) per (cell, group) for the sweep, from the named arrays

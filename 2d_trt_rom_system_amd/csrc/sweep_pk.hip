@@ -775,6 +775,8 @@ struct PkBatch {
     hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;
     std::vector<hipEvent_t> ready;  // per problem: its stream's work before the sweep
     std::vector<void *> owned;
+    std::vector<size_t> owned_bytes;
+    bool pooled = false;            // device blocks, staging and stream from the process caches
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -783,8 +785,11 @@ struct PkBatch {
 static void *batch_alloc(PkBatch &b, size_t bytes)
 {
     void *p = nullptr;
-    TRT_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    size_t n = std::max<size_t>(bytes, 16);
+    if (b.pooled) p = cache_alloc(n);
+    else TRT_CHECK(hipMalloc(&p, n));
     b.owned.push_back(p);
+    b.owned_bytes.push_back(n);
     return p;
 }
 
@@ -794,16 +799,26 @@ void pk_batch_destroy(PkBatch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void *p : b->owned) (void)hipFree(p);
+    for (size_t k = 0; k < b->owned.size(); ++k) {
+        if (b->pooled) cache_release(b->owned[k], b->owned_bytes[k]);
+        else (void)hipFree(b->owned[k]);
+    }
     for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
     for (hipEvent_t e : {b->done, b->t0, b->t1, b->uploaded})
         if (e) (void)hipEventDestroy(e);
-    if (b->staging) (void)hipHostFree(b->staging);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    const size_t sw = b->ctxs.size() * sizeof(PkProb) / sizeof(double);
+    if (b->staging) {
+        if (b->pooled) cache_pinned_put(reinterpret_cast<double *>(b->staging), sw);
+        else (void)hipHostFree(b->staging);
+    }
+    if (b->stream) {
+        if (b->pooled) cache_stream_put(b->stream);
+        else (void)hipStreamDestroy(b->stream);
+    }
     delete b;
 }
 
-PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs)
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
 {
     if (ctxs.empty()) throw Error("a batch needs at least one context");
     Ctx &c0 = *ctxs[0];
@@ -818,6 +833,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs)
     PkBatch *b = new PkBatch();
     try {
         b->ctxs = ctxs;
+        b->pooled = pooled;
         b->precision = c0.precision;
         b->tx = c0.sg.TX;
         b->ty = c0.sg.TY;
@@ -867,10 +883,14 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs)
         const size_t ti = b->precision ? 4 : 8;
         b->lds = (size_t)maxk * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti;
         b->grid = resident_grid(b->k->pk, b->lds, b->nitems);
-        TRT_CHECK(hipStreamCreate(&b->stream));
+        if (pooled) b->stream = cache_stream_get();
+        else TRT_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         TRT_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
         TRT_CHECK(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
-        TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&b->staging), (size_t)n * sizeof(PkProb), hipHostMallocDefault));
+        static_assert(sizeof(PkProb) % sizeof(double) == 0, "table staged as doubles");
+        const size_t sw = (size_t)n * sizeof(PkProb) / sizeof(double);
+        if (pooled) b->staging = reinterpret_cast<PkProb *>(cache_pinned_get(sw));
+        else TRT_CHECK(hipHostMalloc(reinterpret_cast<void **>(&b->staging), sw * sizeof(double), hipHostMallocDefault));
         TRT_CHECK(hipEventCreate(&b->t0));
         TRT_CHECK(hipEventCreate(&b->t1));
         b->ready.resize(n, nullptr);

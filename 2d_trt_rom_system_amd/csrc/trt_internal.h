@@ -68,6 +68,7 @@ struct QDSystems {
 };
 
 struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
+struct Arena;     // one device allocation carved into contexts' blocks (state.hip)
 struct DirectWork;  // block-LU workspace of the direct QD fallback (qd_direct.hip)
 
 struct Ctx {
@@ -83,7 +84,10 @@ struct Ctx {
 
     // registry of named FP64 device arrays (reference names)
     std::map<std::string, DevArray> arrays;
-    std::vector<void *> owned;       // every hipMalloc of this context
+    std::vector<void *> owned;       // every device block of this context
+    std::vector<size_t> owned_bytes; // and its size
+    bool pooled = false;             // stream and pinned buffer from the process caches
+    Arena *arena = nullptr;          // when set, device blocks come from (and return to) this arena
 
     // problem vectors on device
     double *dx = nullptr, *dy = nullptr, *A = nullptr, *ox = nullptr, *oy = nullptr, *w = nullptr;
@@ -119,6 +123,7 @@ struct Ctx {
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
     void *dalloc(size_t bytes);
+    void dfree_last();            // frees the block of the latest dalloc
     size_t bytes = 0;             // device bytes allocated by dalloc
 };
 
@@ -141,6 +146,8 @@ void pull_derived(Ctx &c, const char *name, double *host);
 void rebuild_sc(Ctx &c);
 void calib_copy(size_t nbytes, int reps, double *ms);
 void collapse_intensities(Ctx &c);
+// COLLAPSE_INTENSITIES followed by fg_Calc, in one pass over the partial moments
+void collapse_with_fg(Ctx &c);
 void fg_calc(Ctx &c);
 void cg_calc(Ctx &c);
 void mgqd_in_calc(Ctx &c);
@@ -167,7 +174,7 @@ void pk_destroy(Ctx &c);
 int pk_grid(const Ctx &c);
 // batched persistent sweep over many contexts (one precision and tile shape)
 struct PkBatch;
-PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs);
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled = false);
 void pk_batch_destroy(PkBatch *b);
 void pk_batch_sweep(PkBatch &b, int mark);
 void pk_batch_use_graph(PkBatch &b, bool on);
@@ -182,6 +189,32 @@ void tgen_post(Ctx &c);
 void tgen_step(Ctx &c);
 // device bytes a context allocates at creation (sweep and state arrays)
 size_t state_bytes(const trt_problem &p);
+// Process-wide cache of device blocks for contexts created and destroyed at a
+// high rate (the request loop): freed blocks are kept and reused for requests
+// of a similar size instead of going back through hipFree/hipMalloc.
+void *cache_alloc(size_t &bytes);          // rounds bytes up to the block actually used
+void cache_release(void *p, size_t bytes);
+void cache_trim();                         // returns every cached block, buffer and stream
+size_t cache_bytes();
+// pinned host buffers and streams for pooled contexts: hipHostFree and hipFree
+// wait for all work on the device, hipStreamCreate takes milliseconds
+double *cache_pinned_get(size_t n);        // n doubles
+void cache_pinned_put(double *p, size_t n);
+hipStream_t cache_stream_get();
+void cache_stream_put(hipStream_t s);
+// A device arena: one hipMalloc of a fixed size, carved into blocks (best fit,
+// coalescing frees), so contexts can be created and destroyed at a high rate
+// without hipMalloc/hipFree (hipFree waits for all work on the device).
+Arena *arena_create(size_t bytes);
+void arena_destroy(Arena *a);
+void *arena_alloc(Arena *a, size_t &bytes);   // nullptr when no free range fits; bytes rounded up
+void arena_release(Arena *a, void *p, size_t bytes);
+size_t arena_used(const Arena *a);
+size_t arena_size(const Arena *a);
+// context creation behind trt_ctx_create; pooled contexts take their stream and
+// pinned buffer from the process caches and, with an arena, their device blocks
+trt_ctx *ctx_create(const trt_problem *p, bool pooled, Arena *arena = nullptr);
+void ctx_destroy(trt_ctx *ctx);
 size_t sweep_bytes(const trt_problem &p, SweepGeom *out);
 void qd_alloc(Ctx &c, QDSystems &s, int nsys);
 // solve the batched reduced QD systems stored in s (coef, bnd filled); x holds the guess.

@@ -132,6 +132,7 @@ def run_load(rate: float, n: int, steps: int = 20, budget_gib: float = 200.0, pr
         "sweep_updates_per_s": summ["updates"] / (summ["sweep_ms"] * 1e-3) if summ["sweep_ms"] > 0 else math.nan,
         "completed_per_s": len(ok) / summ["wall_s"] if summ["wall_s"] > 0 else math.nan,
         "peak_active": summ["peak_active"], "peak_device_gib": summ["peak_bytes"] / 2**30,
+        "host_s": {k: summ[k] for k in ("admit_s", "rebuild_s", "step_s", "retire_s")},
         "budget_gib": budget_gib,
     }
 

@@ -196,6 +196,8 @@ typedef struct {
     double sweep_ms;                    /* summed batched-sweep time */
     size_t peak_bytes;                  /* largest device bytes of the active set */
     int peak_active;                    /* largest active set */
+    double admit_s, rebuild_s, step_s, retire_s;   /* host seconds in admission (context creation),
+                                                      batch rebuilds, steps and retirement */
 } trt_server_stats;
 int trt_server_create(trt_server **out, size_t hbm_budget, int max_active);
 int trt_server_destroy(trt_server *s);

@@ -148,3 +148,21 @@ def test_batch_rejects_mixed_tiles(trt):
         trt.native.Batch([c1, c3])
     for c in (c1, c2, c3):
         c.close()
+
+
+def test_fused_collapse_equals_separate_routines(trt):
+    """The step's collapse (COLLAPSE_INTENSITIES with fg_Calc fused) gives bit for bit
+    what trt_collapse_intensities followed by trt_fg_calc gives."""
+    prob = _probs(trt)[2]
+    temp = np.random.default_rng(9).uniform(200.0, 1200.0, (prob.nx, prob.ny))
+    a = trt.native.Context(prob, precision=1, tile=(4, 2))
+    b = trt.native.Context(prob, precision=1, tile=(4, 2))
+    for c in (a, b):
+        c.push("Temp", temp)
+    a.tgen_step()
+    for r in ("material_update", "transport_scb", "collapse_intensities", "fg_calc", "cg_calc"):
+        b.call(r)
+    for n in TGEN_OUT + ["Hg_avg_xy", "Hg_edgV_xx", "HO_Fxg_edgV", "HO_Fyg_edgH", "HO_E_edgV"]:
+        assert np.array_equal(a.pull(n), b.pull(n)), n
+    a.close()
+    b.close()

@@ -48,9 +48,9 @@ def test_request_same_alone_and_in_mix(trt):
 
 
 def test_admission_honours_budget_and_cap(trt):
-    p = _mix(trt)[0]
+    p = trt.problem.small(nx=96, ny=96, order=8, ng=16, seed=41)
     need = trt.native.device_bytes(p, precision=1, tile=(4, 2))
-    admit = need + need // 64 + (1 << 20)             # the loop's admission estimate
+    admit = need + need // 16 + (8 << 20)             # the loop's admission estimate
     budget = 2 * admit + need // 2                    # room for two requests, not three
     with trt.serve.Server(budget, max_active=8) as srv:
         ids = [srv.submit(p, steps=3, arrival_s=0.0) for _ in range(5)]
@@ -72,7 +72,8 @@ def test_admission_honours_budget_and_cap(trt):
 def test_oversized_request_fails_alone(trt):
     small, big = _mix(trt)[1], _mix(trt)[0]
     need = trt.native.device_bytes(small, precision=1, tile=(4, 2))
-    budget = 2 * need + (2 << 20)
+    budget = 2 * need + need // 16 + (8 << 20)       # the small request's admission estimate, with room
+    assert trt.native.device_bytes(big, precision=1, tile=(4, 2)) > 2 * need
     with trt.serve.Server(budget) as srv:
         b = srv.submit(big, 2)
         s = srv.submit(small, 2)
