@@ -127,9 +127,15 @@ __device__ __forceinline__ T xload(GLOBAL_AS const T *p) { return __hip_atomic_l
 template <typename T>
 __device__ __forceinline__ void xstore(GLOBAL_AS T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// 1/x: hardware reciprocal, refined for FP64 by two Newton-Raphson steps
-// (r <- r + r (1 - x r)), each of which squares the relative error.
-__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1/x: hardware reciprocal, refined by Newton-Raphson steps (r <- r + r (1 - x r)),
+// each of which squares the relative error: one step for FP32, two for FP64.
+// The refinement removes the bias of the hardware estimate, which would
+// otherwise accumulate along the sweep's characteristics.
+__device__ __forceinline__ float recip(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
 __device__ __forceinline__ double recip(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
@@ -174,7 +180,7 @@ __device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q)
 // that do not depend on other tiles (old intensities, coefficients, geometry)
 // are issued before sync(), which waits for the upwind tiles; the faces those
 // tiles wrote are read after it.
-template <typename TI, int TX, int TY, bool px, bool py, typename Sync>
+template <typename TI, typename TC, int TX, int TY, bool px, bool py, typename Sync>
 __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
 {
     const int tid = threadIdx.x;
@@ -197,7 +203,7 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     GLOBAL_AS TI *XH = gptr(static_cast<TI *>(P.XH));
     typedef typename Vec2<TI>::type TI2;
     GLOBAL_AS const TI2 *scv = reinterpret_cast<GLOBAL_AS const TI2 *>(gptr(P.sc));
-    const TI inv_cdt = (TI)P.inv_cdt;
+    const TC inv_cdt = P.inv_cdt;
     const int i0 = txl * TX, iw = min(TX, nx - i0);
     const int j0 = tyl * TY, jh = min(TY, ny - j0);
     const int gg = act ? g : 0;
@@ -263,70 +269,74 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
 
     // ---- geometry of the tile's rows and columns for this direction ----
     // aa = |Omega_x| dy / 4, bb = |Omega_y| dx / 4, A/4 = dx dy / 4 (SCB_BGEN, :332-371)
-    TI aar[TY], dyr[TY], bbc[TX], dx4c[TX];
+    TC aar[TY], dyr[TY], bbc[TX], dx4c[TX];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
-        dyr[r] = (TI)dyv[r];
-        aar[r] = (TI)(axn * dyv[r] * 0.25);
+        dyr[r] = (TC)dyv[r];
+        aar[r] = (TC)(axn * dyv[r] * 0.25);
     }
 #pragma unroll
     for (int ii = 0; ii < TX; ++ii) {
-        dx4c[ii] = (TI)(dxv[ii] * 0.25);
-        bbc[ii] = (TI)(ayn * dxv[ii] * 0.25);
+        dx4c[ii] = (TC)(dxv[ii] * 0.25);
+        bbc[ii] = (TC)(ayn * dxv[ii] * 0.25);
     }
+    TC hlc[TX], hrc[TX];
+#pragma unroll
+    for (int ii = 0; ii < TX; ++ii) { hlc[ii] = (TC)hl[ii]; hrc[ii] = (TC)hr[ii]; }
 
-    // ---- cells in upwind order ----
+    // ---- cells in upwind order; the corner solve runs in the compute type TC,
+    // values are rounded to TI where they are stored ----
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
         if (r >= jh) continue;
         const int j = py ? j0 + r : ny - 1 - (j0 + r);
-        TI vb = vbr[r], vt = vtr[r];
-        const TI aa = aar[r];
+        TC vb = (TC)vbr[r], vt = (TC)vtr[r];
+        const TC aa = aar[r];
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             if (ii >= iw) continue;
             const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
             const size_t c = (size_t)i + (size_t)nx * j;
-            const TI bb = bbc[ii];
-            const TI A4 = dx4c[ii] * dyr[r];
-            const TI gam = aa + bb + kt[r][ii] * A4;
-            const TI src = sr[r][ii];
+            const TC bb = bbc[ii];
+            const TC A4 = dx4c[ii] * dyr[r];
+            const TC gam = aa + bb + (TC)kt[r][ii] * A4;
+            const TC src = (TC)sr[r][ii];
             // S = RT_Src + I_old / (c dt)  (TRANSPORT_SOLVES.f08:48); Q = B x (:270-294)
-            const TI s0 = src + io[r][ii].v[perm[0]] * inv_cdt;
-            const TI s1 = src + io[r][ii].v[perm[1]] * inv_cdt;
-            const TI s2 = src + io[r][ii].v[perm[2]] * inv_cdt;
-            const TI s3 = src + io[r][ii].v[perm[3]] * inv_cdt;
-            const TI Q0 = A4 * s0 + (TI)2 * aa * vb + (TI)2 * bb * hl[ii];
-            const TI Q1 = A4 * s1 + (TI)2 * bb * hr[ii];
-            const TI Q2 = A4 * s2;
-            const TI Q3 = A4 * s3 + (TI)2 * aa * vt;
-            const TI g2 = gam * gam;
-            const TI al = g2 + aa * aa + bb * bb, be = (TI)2 * aa * bb;
-            const TI y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
-            const TI y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
-            const TI dm = aa - bb, dp = aa + bb;
-            const TI rD = recip((g2 + dm * dm) * (g2 + dp * dp));
-            const TI x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
-            const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
-            const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
-            const TI x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
+            const TC s0 = src + (TC)io[r][ii].v[perm[0]] * inv_cdt;
+            const TC s1 = src + (TC)io[r][ii].v[perm[1]] * inv_cdt;
+            const TC s2 = src + (TC)io[r][ii].v[perm[2]] * inv_cdt;
+            const TC s3 = src + (TC)io[r][ii].v[perm[3]] * inv_cdt;
+            const TC Q0 = A4 * s0 + (TC)2 * aa * vb + (TC)2 * bb * hlc[ii];
+            const TC Q1 = A4 * s1 + (TC)2 * bb * hrc[ii];
+            const TC Q2 = A4 * s2;
+            const TC Q3 = A4 * s3 + (TC)2 * aa * vt;
+            const TC g2 = gam * gam;
+            const TC al = g2 + aa * aa + bb * bb, be = (TC)2 * aa * bb;
+            const TC y0 = al * Q0 + be * Q2, y1 = al * Q1 - be * Q3;
+            const TC y2 = al * Q2 + be * Q0, y3 = al * Q3 - be * Q1;
+            const TC dm = aa - bb, dp = aa + bb;
+            const TC rD = recip((g2 + dm * dm) * (g2 + dp * dp));
+            const TC x0 = (gam * y0 - aa * y1 - bb * y3) * rD;
+            const TC x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
+            const TC x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
+            const TC x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
             if (act) {
                 V4<TI> o;
-                o.v[perm[0]] = x0; o.v[perm[1]] = x1; o.v[perm[2]] = x2; o.v[perm[3]] = x3;
+                o.v[perm[0]] = (TI)x0; o.v[perm[1]] = (TI)x1; o.v[perm[2]] = (TI)x2; o.v[perm[3]] = (TI)x3;
                 o.store(Inew + ((c * 4 + q) * NL + L) * 4);
             }
             // outgoing corner edges (:86-89)
-            vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
+            vb = x1; vt = x2; hlc[ii] = x3; hrc[ii] = x2;
             // corner averages for COLLAPSE_INTENSITIES (:447-470)
             const int cidx = r * TX + ii;
-            st[(0 * TX * TY + cidx) * PK_THREADS + tid] = (x0 + x1 + x2 + x3) * (TI)0.25;
-            st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (x1 + x2) * (TI)0.5;
-            st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (x3 + x2) * (TI)0.5;
+            st[(0 * TX * TY + cidx) * PK_THREADS + tid] = (TI)((x0 + x1 + x2 + x3) * (TC)0.25);
+            st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (TI)((x1 + x2) * (TC)0.5);
+            st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (TI)((x3 + x2) * (TC)0.5);
         }
         if (txl < P.NTX - 1 && act) {
             const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * (j0 + r)) * NL + L;
-            xstore(XV + base, vb);
-            xstore(XV + base + NL, vt);
+            xstore(XV + base, (TI)vb);
+            xstore(XV + base + NL, (TI)vt);
         }
     }
     if (tyl < P.NTY - 1 && act) {
@@ -334,8 +344,8 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         for (int ii = 0; ii < TX; ++ii) {
             if (ii < iw) {
                 const size_t base = ((size_t)(q * (P.NTY - 1) + tyl) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-                xstore(XH + base, hl[ii]);
-                xstore(XH + base + NL, hr[ii]);
+                xstore(XH + base, (TI)hlc[ii]);
+                xstore(XH + base + NL, (TI)hrc[ii]);
             }
         }
     }
@@ -397,14 +407,14 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const
     }
 }
 
-template <typename TI, int TX, int TY, typename Sync>
+template <typename TI, typename TC, int TX, int TY, typename Sync>
 __device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
 {
     switch (q) {
-    case 0: tile_body<TI, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync); break;
-    case 1: tile_body<TI, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync); break;
-    case 2: tile_body<TI, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync); break;
-    default: tile_body<TI, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync); break;
+    case 0: tile_body<TI, TC, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync); break;
+    case 1: tile_body<TI, TC, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync); break;
+    case 2: tile_body<TI, TC, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync); break;
+    default: tile_body<TI, TC, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync); break;
     }
 }
 
@@ -426,7 +436,7 @@ __device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, const
 constexpr int PKPROB_WORDS = (int)(sizeof(PkProb) / 8);
 static_assert(sizeof(PkProb) % 8 == 0, "PkProb is copied as 8-byte words");
 
-template <typename TI, int TX, int TY>
+template <typename TI, typename TC, int TX, int TY>
 __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 {
     extern __shared__ double dyn[];
@@ -467,7 +477,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
         double *ctab = dyn;
         TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
-        tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
+        tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
             if (tid == 0) {
                 if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
                 if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
 }
 
 // One launch per tile diagonal d: the tiles of d of every quadrant and group block.
-template <typename TI, int TX, int TY>
+template <typename TI, typename TC, int TX, int TY>
 __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, int tx_lo, int nt_d, int nblk_g)
 {
     extern __shared__ double dyn[];
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, 
     const int txl = tx_lo + t, tyl = d - txl;
     double *ctab = dyn;
     TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
-    tile_sweep<TI, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
+    tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
         load_ctab(P, ctab, q);
         __syncthreads();
     });
@@ -567,18 +577,21 @@ struct TileKernels {
     const void *pk, *diag;
 };
 
-template <typename TI, int TX, int TY>
+// TI: intensity storage type, TC: arithmetic of the corner solve
+template <typename TI, typename TC, int TX, int TY>
 TileKernels tk(int precision)
 {
-    return TileKernels{TX, TY, precision, reinterpret_cast<const void *>(&sweep_pk<TI, TX, TY>),
-                       reinterpret_cast<const void *>(&sweep_diag<TI, TX, TY>)};
+    return TileKernels{TX, TY, precision, reinterpret_cast<const void *>(&sweep_pk<TI, TC, TX, TY>),
+                       reinterpret_cast<const void *>(&sweep_diag<TI, TC, TX, TY>)};
 }
 
 const TileKernels *find_tile_kernels(int precision, int tx, int ty)
 {
     static const TileKernels table[] = {
-        tk<float, 2, 2>(1), tk<float, 2, 4>(1), tk<float, 4, 2>(1), tk<float, 4, 4>(1), tk<float, 8, 2>(1),
-        tk<float, 2, 8>(1), tk<double, 2, 2>(0), tk<double, 2, 4>(0), tk<double, 4, 2>(0),
+        tk<float, float, 2, 2>(1), tk<float, float, 2, 4>(1), tk<float, float, 4, 2>(1),
+        tk<float, float, 4, 4>(1), tk<float, float, 8, 2>(1), tk<float, float, 2, 8>(1),
+        tk<double, double, 2, 2>(0), tk<double, double, 2, 4>(0), tk<double, double, 4, 2>(0),
+        tk<float, double, 2, 2>(2), tk<float, double, 2, 4>(2), tk<float, double, 4, 2>(2),
     };
     for (const TileKernels &t : table)
         if (t.tx == tx && t.ty == ty && t.precision == precision) return &t;

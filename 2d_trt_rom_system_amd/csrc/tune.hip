@@ -11,7 +11,8 @@ struct Shape {
     int tx, ty;
 };
 
-// tile shapes with a kernel in the tile engine, per precision
+// tile shapes with a kernel in the tile engine, per precision (FP64 arithmetic:
+// the FP64 shapes, also for FP32 storage)
 const Shape kFp32[] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}, {8, 2}, {2, 8}};
 const Shape kFp64[] = {{2, 2}, {2, 4}, {4, 2}};
 
@@ -59,10 +60,10 @@ extern "C" {
 int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps, int *tile_x, int *tile_y,
                       double *best_ms, double *all_ms)
 {
-    if (!probs || n < 1 || !tile_x || !tile_y || (precision != 0 && precision != 1)) return -1;
+    if (!probs || n < 1 || !tile_x || !tile_y || precision < 0 || precision > 2) return -1;
     try {
-        const Shape *shapes = precision ? kFp32 : kFp64;
-        const int ns = precision ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
+        const Shape *shapes = precision == 1 ? kFp32 : kFp64;
+        const int ns = precision == 1 ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
         double best = std::numeric_limits<double>::infinity();
         for (int s = 0; s < ns; ++s) {
             const double ms = time_shape(probs, n, precision, shapes[s], sweeps > 0 ? sweeps : 3);
@@ -85,8 +86,8 @@ int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps
 
 int trt_autotune_shapes(int precision, int *tile_x, int *tile_y, int cap)
 {
-    const Shape *shapes = precision ? kFp32 : kFp64;
-    const int ns = precision ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
+    const Shape *shapes = precision == 1 ? kFp32 : kFp64;
+    const int ns = precision == 1 ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
     for (int s = 0; s < ns && s < cap; ++s) {
         if (tile_x) tile_x[s] = shapes[s].tx;
         if (tile_y) tile_y[s] = shapes[s].ty;

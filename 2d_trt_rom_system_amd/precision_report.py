@@ -1,0 +1,110 @@
+"""FP32-sweep accuracy against the FP64 oracle over full time steps.
+
+Runs the same problem four ways -- the FP64 oracle (CPU), and on the GPU the
+FP64 sweep (fp64), the FP32 sweep (fp32: FP32 intensities and corner-solve
+arithmetic) and FP32 storage with FP64 arithmetic (fp32s); all with FP64
+moments and FP64 low-order solves -- for a number of MLQD time steps, and
+reports after every step the relative L2 error of E (grey, cell averages), T
+and the Eddington tensor components f_xx, f_yy, f_xy of each GPU form against
+the oracle, plus the f errors weighted by the group energy density:
+
+    python -m 2d_trt_rom_system_amd.precision_report --problem map48 --steps 10
+
+The oracle is the checker here, never the thing measured.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import native, problem
+
+QUANTITIES = {"E": "E_avg", "T": "Temp", "f_xx": "fg_avg_xx", "f_yy": "fg_avg_yy", "f_xy": "fg_avg_xy"}
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    d = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (d if d > 0 else 1.0)
+
+
+def build_problem(name: str, tight: bool = False):
+    if name == "cfg0":
+        p = problem.cfg0()
+    elif name.startswith("map"):
+        # S8 and 16 groups keep the oracle's band solves to seconds per step
+        p = problem.mapped(nx=int(name[3:]), order=8, ng=16, sigma=1.0)
+    else:
+        raise ValueError(name)
+    if tight:
+        # iterations converged far below the sweep's FP32 rounding, so that the
+        # FP32 and FP64 runs are compared at (nearly) the same fixed point rather
+        # than at iterates that stopped at different distances from it
+        p.conv_ho, p.conv_lo, p.conv_gr1, p.conv_gr2 = 1e-11, 1e-12, 1e-13, 1e-13
+    return p
+
+
+def weighted_rel(a, b, w) -> float:
+    """sum w |a - b| / sum w |b|: errors of a ratio weighted by the energy it carries."""
+    a, b, w = (np.asarray(x, dtype=np.float64).ravel() for x in (a, b, w))
+    den = float(np.sum(np.abs(w * b)))
+    return float(np.sum(np.abs(w * (a - b)))) / (den if den > 0 else 1.0)
+
+
+def run(name: str, steps: int, threads: int = 8, tight: bool = False) -> dict:
+    import oracle
+    prob = build_problem(name, tight)
+    o = oracle.Oracle(prob, threads=threads)
+    forms = {"fp64": native.Context(prob, precision=0), "fp32": native.Context(prob, precision=1),
+             "fp32s": native.Context(prob, precision=2)}
+    rows = []
+    try:
+        for s in range(1, steps + 1):
+            t0 = time.perf_counter()
+            o.step()
+            t1 = time.perf_counter()
+            row = {"step": s, "oracle_s": t1 - t0}
+            for k, g in forms.items():
+                t2 = time.perf_counter()
+                g.step()
+                row[f"{k}_gpu_s"] = time.perf_counter() - t2
+                print(f"[precision] step {s} {k} done", file=sys.stderr, flush=True)
+            eg = np.abs(o["HO_Eg_avg"].ravel(order="F"))
+            for k, g in forms.items():
+                for q, arr in QUANTITIES.items():
+                    row[f"{k}_{q}"] = rel_l2(g.pull(arr), o[arr].ravel(order="F"))
+                # f weighted by the group energy density: cells and groups where E_g
+                # is near zero (or changes sign) make f = H/E meaningless there
+                for q in ("f_xx", "f_yy", "f_xy"):
+                    arr = QUANTITIES[q]
+                    row[f"{k}_{q}_Ew"] = weighted_rel(g.pull(arr), o[arr].ravel(order="F"), eg)
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+    finally:
+        for g in forms.values():
+            g.close()
+    keys = [k for k in rows[0] if k.startswith("fp") and not k.endswith("gpu_s")]
+    worst = {k: max(r[k] for r in rows) for k in keys}
+    return {"problem": name, "nx": prob.nx, "ny": prob.ny, "order": prob.order, "ng": prob.ng, "steps": steps,
+            "tight": tight, "conv": [prob.conv_ho, prob.conv_lo, prob.conv_gr1, prob.conv_gr2], "rows": rows,
+            "worst": worst}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--problem", default="map48")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--tight", action="store_true", help="converge the iterations to 1e-11..1e-13")
+    a = ap.parse_args(argv)
+    print(json.dumps(run(a.problem, a.steps, a.threads, a.tight)))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -214,6 +214,19 @@ def cfg1(seed: int = 20240611, nx: int = 256) -> Problem:
     return _conv_cascade(p, 1e-8)
 
 
+def mapped(nx: int = 48, order: int = 8, ng: int = 16, sigma: float = 1.0, seed: int = 20240611,
+           nsteps: int = 10) -> Problem:
+    """The 256x256 problem's physics at a smaller size with a milder multiplier map:
+    an 8x8-region lognormal(0, sigma) map, groups 1e-3..1e2 keV.  With sigma = 1.5
+    (multipliers 0.04..53) the grey Newton iteration of EGP_FV_NEWT diverges within
+    the first step (NaN in the state, also in the oracle); sigma = 1 stays finite."""
+    rng = SplitMix64(seed)
+    kmult = np.array([rng.lognormal(0.0, sigma) for _ in range(64)])
+    p = Problem(nx=nx, ny=nx, order=order, ng=ng, nu=log_group_edges(ng, 1e-3, 1e2), nsteps=nsteps,
+                mat=region_map(nx, nx), kmult=kmult, name=f"mapped{nx}")
+    return _conv_cascade(p, 1e-8)
+
+
 CFG3_GRIDS = (64, 128, 256, 384, 512)
 CFG3_GROUPS = (16, 32, 64)
 CFG3_ORDERS = (8, 12, 16)

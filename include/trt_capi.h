@@ -46,7 +46,9 @@ typedef struct {
     double E_bound_low, T_bound_low;
     int threads;                    /* accepted and ignored (reference Open_Threads) */
     /* extensions of this implementation */
-    int precision;                  /* 0 = FP64 sweep, 1 = FP32 sweep with FP64 moment accumulation */
+    int precision;                  /* 0 = FP64 sweep; 1 = FP32 sweep (storage and arithmetic) with FP64
+                                       moment accumulation; 2 = FP32 intensity storage with FP64
+                                       arithmetic in the corner solve (tile engine only) */
     int tile_x, tile_y;             /* sweep tile shape; 0 = default                 */
     double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
     int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default 2000.  A system

@@ -99,6 +99,23 @@ def test_sweep_fp32_within_tolerance(trt, oracle_mod, nx, ny, order, ng, tile, m
     ctx.close()
 
 
+@pytest.mark.parametrize("tile,mode", [((2, 4), 2), ((4, 2), 3), ((2, 2), 2)])
+def test_sweep_fp32_storage_fp64_arithmetic(trt, oracle_mod, tile, mode):
+    """precision 2: intensities stored in FP32, corner solve in FP64.  The only
+    rounding is the storage of I_crn and of the tile faces: the FP32 relative
+    rounding (6e-8) bounds the error of every stored value."""
+    prob, o, ctx = _randomized_pair(trt, oracle_mod, 13, 11, 8, 5, tile, precision=2, mode=mode)
+    o.call("transport_scb")
+    ctx.call("transport_scb")
+    assert rel_err(ctx.pull("I_crn", o["I_crn"].shape), o["I_crn"]) < 3e-7
+    for c in (o, ctx):
+        c.call("collapse_intensities")
+        c.call("fg_calc")
+    for name in ("HO_Eg_avg", "fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgH_yy"):
+        assert rel_err(ctx.pull(name), o[name].ravel(order="F")) < 3e-7, name
+    ctx.close()
+
+
 def test_equilibrium_sweep_is_exact(trt):
     """T = Tini everywhere with Planckian boundaries at Tini: the sweep returns B_g in every corner."""
     prob = trt.problem.small(nx=8, ny=8, order=8, ng=3)
