@@ -45,8 +45,9 @@ void validate(const trt_problem *p)
         if (p->bc_type[k] != 0) throw Error("only incoming-Planckian boundaries (BC_Type 0) are supported");
     if (p->precision < 0 || p->precision > 2)
         throw Error("precision must be 0 (FP64), 1 (FP32 sweep) or 2 (FP32 storage, FP64 arithmetic)");
-    if (p->precision == 2 && p->sweep_mode == 1) throw Error("precision 2 runs on the tile engine (sweep_mode 0, 2, 3)");
-    if (p->sweep_mode < 0 || p->sweep_mode > 3) throw Error("sweep_mode must be 0, 1, 2 or 3");
+    if (p->precision == 2 && p->sweep_mode == 1)
+        throw Error("precision 2 runs on the tile engine (sweep_mode 0, 2, 3 or 4)");
+    if (p->sweep_mode < 0 || p->sweep_mode > 4) throw Error("sweep_mode must be 0, 1, 2, 3 or 4");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
     for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
         if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");

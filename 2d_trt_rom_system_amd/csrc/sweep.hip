@@ -321,7 +321,7 @@ void sweep_setup(Ctx &c)
     if (c.sweep_mode == 1 && sg.TX != 4 && sg.TX != 8 && sg.TX != 16)
         throw Error("tile_x must be 4, 8 or 16 for sweep_mode 1");
     if (c.sweep_mode >= 2 && !pk_supported(c))
-        throw Error("sweep_mode 2 and 3 take tiles 2x2, 2x4, 4x2 (FP64) and also 4x4, 8x2, 2x8 (FP32)");
+        throw Error("sweep_mode 2, 3 and 4 take tiles 2x2, 2x4, 4x2 (FP64) and also 4x4, 8x2, 2x8 (FP32)");
     const size_t nc = (size_t)c.nx * c.ny, NL = (size_t)c.ng * sg.K;
     const size_t ti = c.precision ? 4 : 8;
     c.icrn_bytes = nc * 4 * NL * 4 * ti;
@@ -351,7 +351,7 @@ void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     const SweepGeom &sg = c.sg;
     if (c.sweep_mode >= 2) {
         const int out = 1 - c.iold;
-        if (c.sweep_mode == 2) pk_sweep(c, ev);
+        if (c.sweep_mode == 2 || c.sweep_mode == 4) pk_sweep(c, ev);
         else diag_sweep(c, ev);
         TRT_CHECK(hipGetLastError());
         c.icur = out;
@@ -566,7 +566,10 @@ void rebuild_sc(Ctx &c)
     TRT_CHECK(hipGetLastError());
 }
 
-int sweep_launches(const Ctx &c) { return c.sweep_mode == 2 ? 1 : c.sg.NTX + c.sg.NTY - 1; }
+int sweep_launches(const Ctx &c)
+{
+    return c.sweep_mode == 2 || c.sweep_mode == 4 ? 1 : c.sg.NTX + c.sg.NTY - 1;
+}
 
 // plain device copy of n doubles, the calibration kernel for HBM byte counters
 __global__ void calib_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)

@@ -51,6 +51,7 @@ constexpr int NMOM = 13;
 struct PkProb {
     int nx, ny, ng, K, gblk, NTX, NTY;
     int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per tile
+    int seg, nseg;         // column segments: tiles per segment, segments per tile column
     double inv_cdt, eps;
     const double *dx, *dy, *ox, *oy, *w;
     const void *sc;        // [cell][g] (KapE + 1/(c dt), RT_Src) in the sweep precision
@@ -180,8 +181,13 @@ __device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q)
 // that do not depend on other tiles (old intensities, coefficients, geometry)
 // are issued before sync(), which waits for the upwind tiles; the faces those
 // tiles wrote are read after it.
+// hc carries the H faces between consecutive tiles of one column swept by the same
+// workgroup: with hin_regs the tile's incoming H faces come from hc instead of the
+// exchange buffer, and with hout_xh false its outgoing ones stay in hc only.  The
+// carried values are rounded to TI exactly as the exchange buffer rounds them.
 template <typename TI, typename TC, int TX, int TY, bool px, bool py, typename Sync>
-__device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
+__device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync,
+                                          TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
 {
     const int tid = threadIdx.x;
     const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
@@ -261,9 +267,14 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     for (int ii = 0; ii < TX; ++ii) {
         hl[ii] = bcy; hr[ii] = bcy;
         if (tyl > 0 && ii < iw && act) {
-            const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-            hl[ii] = xload(XH + base);
-            hr[ii] = xload(XH + base + NL);
+            if (hin_regs) {
+                hl[ii] = hc[ii][0];
+                hr[ii] = hc[ii][1];
+            } else {
+                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
+                hl[ii] = xload(XH + base);
+                hr[ii] = xload(XH + base + NL);
+            }
         }
     }
 
@@ -339,7 +350,7 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             xstore(XV + base + NL, (TI)vt);
         }
     }
-    if (tyl < P.NTY - 1 && act) {
+    if (tyl < P.NTY - 1 && act && hout_xh) {
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             if (ii < iw) {
@@ -348,6 +359,11 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
                 xstore(XH + base + NL, (TI)hrc[ii]);
             }
         }
+    }
+#pragma unroll
+    for (int ii = 0; ii < TX; ++ii) {
+        hc[ii][0] = (TI)hlc[ii];
+        hc[ii][1] = (TI)hrc[ii];
     }
 }
 
@@ -408,13 +424,14 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const
 }
 
 template <typename TI, typename TC, int TX, int TY, typename Sync>
-__device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync)
+__device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync,
+                                           TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
 {
     switch (q) {
-    case 0: tile_body<TI, TC, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync); break;
-    case 1: tile_body<TI, TC, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync); break;
-    case 2: tile_body<TI, TC, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync); break;
-    default: tile_body<TI, TC, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync); break;
+    case 0: tile_body<TI, TC, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
+    case 1: tile_body<TI, TC, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
+    case 2: tile_body<TI, TC, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
+    default: tile_body<TI, TC, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
     }
 }
 
@@ -477,6 +494,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
         double *ctab = dyn;
         TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        TI hc[TX][2];
         tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
             if (tid == 0) {
                 if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
@@ -484,7 +502,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
             }
             load_ctab(P, ctab, q);
             __syncthreads();
-        });
+        }, hc, false, true);
         wait_vm_stores();                 // this wave's face stores have completed
         __syncthreads();
         if (tid == 0) {
@@ -511,12 +529,87 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, 
     const int txl = tx_lo + t, tyl = d - txl;
     double *ctab = dyn;
     TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+    TI hc[TX][2];
     tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
         load_ctab(P, ctab, q);
         __syncthreads();
-    });
+    }, hc, false, true);
     __syncthreads();
     tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+}
+
+// Column-segment form: a work item is (quadrant, group block, tile column, segment
+// of seg consecutive tiles of that column).  The workgroup sweeps the segment's
+// tiles bottom to top and keeps the H faces between them in registers, so only
+// the segment's last tile writes the H exchange buffer and only its first tile
+// waits for the tile below; every tile still waits for its left neighbour and
+// publishes its own flag.  Items are listed by segment diagonal (tile column +
+// segment index), a topological order of the segments.
+template <typename TI, typename TC, int TX, int TY>
+__global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
+{
+    extern __shared__ double dyn[];
+    __shared__ int s_next;
+    __shared__ int4 s_item;
+    __shared__ int s_pi;
+    __shared__ unsigned long long s_prob[PKPROB_WORDS];
+    const int tid = threadIdx.x;
+    const int epoch = a.queue[1];
+    int *err = a.queue + 2;
+    if (tid == 0) {
+        const int c0 = atomicAdd(a.queue, 1);
+        s_next = c0;
+        if (c0 < a.nitems) s_item = a.items[c0];
+        s_pi = -1;
+    }
+    __syncthreads();
+    int cur = __builtin_amdgcn_readfirstlane(s_next);
+    while (cur < a.nitems) {
+        int nxt = 0;
+        int4 nit = make_int4(0, 0, 0, 0);
+        if (tid == 0) {
+            nxt = atomicAdd(a.queue, 1);
+            if (nxt < a.nitems) nit = a.items[nxt];
+        }
+        const int pi = __builtin_amdgcn_readfirstlane(s_item.x);
+        const int fl0 = __builtin_amdgcn_readfirstlane(s_item.y);
+        const int tpos = __builtin_amdgcn_readfirstlane(s_item.z);
+        const int qg = __builtin_amdgcn_readfirstlane(s_item.w);
+        const int txl = tpos & 0xffff, sg = tpos >> 16, q = qg & 0xff, gb = qg >> 8;
+        if (pi != __builtin_amdgcn_readfirstlane(s_pi)) {
+            const PkProb &Pg = a.probs[pi];
+            if (tid < PKPROB_WORDS) s_prob[tid] = reinterpret_cast<const unsigned long long *>(&Pg)[tid];
+            __syncthreads();
+            if (tid == 0) s_pi = pi;
+        }
+        const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
+        double *ctab = dyn;
+        TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        const int t0 = sg * P.seg, t1 = min(t0 + P.seg, P.NTY);
+        TI hc[TX][2];
+        for (int tyl = t0; tyl < t1; ++tyl) {
+            const int fl = fl0 + (tyl - t0) * P.NTX;
+            tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
+                if (tid == 0) {
+                    if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
+                    if (tyl == t0 && tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
+                }
+                if (tyl == t0) load_ctab(P, ctab, q);
+                __syncthreads();
+            }, hc, tyl > t0, tyl == t1 - 1);
+            wait_vm_stores();             // this wave's face stores have completed
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            s_next = nxt;
+            s_item = nit;
+        }
+        __syncthreads();
+        cur = __builtin_amdgcn_readfirstlane(s_next);
+    }
 }
 
 __global__ void pk_reset(int *queue)
@@ -574,7 +667,7 @@ __global__ void pk_boundary_moments(const PkProb *probs)
 // (3 TX TY 256 values) fit 48 KiB of LDS and the tile's loads fit in registers.
 struct TileKernels {
     int tx, ty, precision;
-    const void *pk, *diag;
+    const void *pk, *diag, *seg;
 };
 
 // TI: intensity storage type, TC: arithmetic of the corner solve
@@ -582,7 +675,8 @@ template <typename TI, typename TC, int TX, int TY>
 TileKernels tk(int precision)
 {
     return TileKernels{TX, TY, precision, reinterpret_cast<const void *>(&sweep_pk<TI, TC, TX, TY>),
-                       reinterpret_cast<const void *>(&sweep_diag<TI, TC, TX, TY>)};
+                       reinterpret_cast<const void *>(&sweep_diag<TI, TC, TX, TY>),
+                       reinterpret_cast<const void *>(&sweep_seg<TI, TC, TX, TY>)};
 }
 
 const TileKernels *find_tile_kernels(int precision, int tx, int ty)
@@ -610,6 +704,7 @@ struct PkState {
     size_t lds = 0;
     int grid = 0;
     const TileKernels *k = nullptr;
+    const void *kern = nullptr;  // the persistent kernel: tile items or column segments
     PkProb host[2];              // host copies of the two tables
 };
 
@@ -632,6 +727,42 @@ static int resident_grid(const void *kern, size_t lds, int nitems)
     return std::max(1, std::min(nitems, per_cu * ncu));
 }
 
+// Work-list entries of diagonal d of one problem: tiles (txl, tyl = d - txl), or
+// with segs column segments (txl, s = d - txl) of seg tiles.
+static void diag_items(std::vector<int4> &items, int prob, int base, const SweepGeom &sg, int seg, int nseg, int d,
+                       bool segs)
+{
+    const int nrow = segs ? nseg : sg.NTY;
+    const int lo = std::max(0, d - (nrow - 1)), hi = std::min(d, sg.NTX - 1);
+    for (int txl = lo; txl <= hi; ++txl) {
+        const int r = d - txl;
+        const int tyl = segs ? r * seg : r;
+        for (int q = 0; q < 4; ++q)
+            for (int gb = 0; gb < sg.nblk_g; ++gb) {
+                const int flag = base + ((q * sg.nblk_g + gb) * sg.NTY + tyl) * sg.NTX + txl;
+                items.push_back(make_int4(prob, flag, txl | (r << 16), q | (gb << 8)));
+            }
+    }
+}
+
+static int num_diags(const SweepGeom &sg, int nseg, bool segs) { return sg.NTX + (segs ? nseg : sg.NTY) - 1; }
+
+// Tiles per column segment: the longest power of two that still leaves about
+// eight work items per resident workgroup (long segments keep more H faces in
+// registers; enough items keep the device busy).
+static int seg_length(const SweepGeom &sg, int grid)
+{
+    if (const char *v = getenv("TRT_SEG_LEN")) {
+        const int s = atoi(v);
+        if (s >= 1) return std::min(s, sg.NTY);
+    }
+    int seg = 1;
+    while (seg * 2 <= sg.NTY &&
+           (long long)sg.NTX * ((sg.NTY + seg * 2 - 1) / (seg * 2)) * 4 * sg.nblk_g >= 8LL * grid)
+        seg *= 2;
+    return seg;
+}
+
 static void pk_setup(Ctx &c)
 {
     const SweepGeom &sg = c.sg;
@@ -639,7 +770,6 @@ static void pk_setup(Ctx &c)
     c.pk = pk;
     pk->k = find_tile_kernels(c.precision, sg.TX, sg.TY);
     if (!pk->k) throw Error("no tile kernel for this tile shape and precision");
-    const size_t ti = c.precision ? 4 : 8;
     PkProb P{};
     P.nx = c.nx; P.ny = c.ny; P.ng = c.ng; P.K = sg.K; P.gblk = sg.gblk; P.NTX = sg.NTX; P.NTY = sg.NTY;
     P.npair = sg.TX * sg.TY * sg.gblk;
@@ -654,6 +784,12 @@ static void pk_setup(Ctx &c)
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
     P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
+    const bool segs = c.sweep_mode == 4;
+    const size_t ti = c.precision ? 4 : 8;
+    pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
+    const int full_grid = resident_grid(segs ? pk->k->seg : pk->k->pk, pk->lds, 1 << 30);
+    P.seg = segs ? seg_length(sg, full_grid) : 1;
+    P.nseg = (sg.NTY + P.seg - 1) / P.seg;
     PkProb tab[2] = {P, P};
     tab[0].Iold = c.Icrn[0]; tab[0].Inew = c.Icrn[1];
     tab[1].Iold = c.Icrn[1]; tab[1].Inew = c.Icrn[0];
@@ -662,21 +798,11 @@ static void pk_setup(Ctx &c)
     pk->probs = static_cast<PkProb *>(c.dalloc(sizeof(tab)));
     TRT_CHECK(hipMemcpy(pk->probs, tab, sizeof(tab), hipMemcpyHostToDevice));
 
-    // items in tile-diagonal order
+    // items in tile-diagonal (or segment-diagonal) order
     std::vector<int4> items;
     const int nchain = 4 * sg.nblk_g;
     items.reserve((size_t)nchain * sg.NTX * sg.NTY);
-    for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
-        const int lo = std::max(0, d - (sg.NTY - 1)), hi = std::min(d, sg.NTX - 1);
-        for (int txl = lo; txl <= hi; ++txl) {
-            const int tyl = d - txl;
-            for (int q = 0; q < 4; ++q)
-                for (int gb = 0; gb < sg.nblk_g; ++gb) {
-                    const int flag = ((q * sg.nblk_g + gb) * sg.NTY + tyl) * sg.NTX + txl;
-                    items.push_back(make_int4(0, flag, txl | (tyl << 16), q | (gb << 8)));
-                }
-        }
-    }
+    for (int d = 0; d < num_diags(sg, P.nseg, segs); ++d) diag_items(items, 0, 0, sg, P.seg, P.nseg, d, segs);
     pk->nitems = (int)items.size();
     pk->items = static_cast<int4 *>(c.dalloc(items.size() * sizeof(int4)));
     TRT_CHECK(hipMemcpy(pk->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -684,9 +810,8 @@ static void pk_setup(Ctx &c)
     TRT_CHECK(hipMemset(pk->flags, 0, (size_t)nchain * sg.NTX * sg.NTY * sizeof(int)));
     pk->queue = static_cast<int *>(c.dalloc(4 * sizeof(int)));
     TRT_CHECK(hipMemset(pk->queue, 0, 4 * sizeof(int)));
-
-    pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
-    pk->grid = resident_grid(pk->k->pk, pk->lds, pk->nitems);
+    pk->kern = segs ? pk->k->seg : pk->k->pk;
+    pk->grid = std::min(full_grid, pk->nitems);
 }
 
 // boundary-face moments of nprob problems (table tab), the largest with nbt faces x groups x quadrants
@@ -722,7 +847,7 @@ void pk_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     a.spin_limit = 1 << 20;
     void *args[] = {&a};
     if (ev) TRT_CHECK(hipEventRecord((*ev)[0], c.stream));
-    TRT_CHECK(hipLaunchKernel(pk.k->pk, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
+    TRT_CHECK(hipLaunchKernel(pk.kern, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
     if (ev) TRT_CHECK(hipEventRecord((*ev)[1], c.stream));
 }
 
@@ -790,6 +915,9 @@ struct PkBatch {
     std::vector<void *> owned;
     std::vector<size_t> owned_bytes;
     bool pooled = false;            // device blocks, staging and stream from the process caches
+    bool segs = false;              // column-segment items
+    std::vector<int> seg_of;        // tiles per segment of each problem
+    const void *kern = nullptr;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -837,7 +965,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
     Ctx &c0 = *ctxs[0];
     for (Ctx *c : ctxs) {
         if (c->precision != c0.precision) throw Error("batched contexts must share the sweep precision");
-        if (c->sweep_mode < 2) throw Error("batched contexts must use the tile engine (sweep_mode 0, 2 or 3)");
+        if (c->sweep_mode < 2) throw Error("batched contexts must use the tile engine (sweep_mode 0, 2, 3 or 4)");
         if (c->sg.TX != c0.sg.TX || c->sg.TY != c0.sg.TY) throw Error("batched contexts must share the tile shape");
     }
     for (size_t i = 0; i < ctxs.size(); ++i)
@@ -855,37 +983,54 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
         const int n = (int)ctxs.size();
         b->host.resize(n);
         b->up_iold.assign(n, -1);
+        // The batch picks its own item form (the results are bitwise the same): column
+        // segments of up to 4 tiles when the segment wavefronts of all members still
+        // offer twice the resident workgroups, else single tiles.
+        {
+            const size_t ti0 = b->precision ? 4 : 8;
+            int maxk0 = 0;
+            for (Ctx *c : ctxs) maxk0 = std::max(maxk0, c->sg.K);
+            const size_t lds0 = (size_t)maxk0 * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti0;
+            const long long g2 = 2LL * resident_grid(b->k->seg, lds0, 1 << 30);
+            int seg = 1;
+            if (const char *v = getenv("TRT_SEG_LEN")) seg = std::max(1, atoi(v));
+            else
+                for (int s = 4; s >= 2 && seg == 1; s /= 2) {
+                    long long par = 0;
+                    for (Ctx *c : ctxs)
+                        par += (long long)std::min(c->sg.NTX, (c->sg.NTY + s - 1) / s) * 4 * c->sg.nblk_g;
+                    if (par >= g2) seg = s;
+                }
+            b->segs = seg > 1;
+            b->seg_of.resize(n);
+            for (int i = 0; i < n; ++i) b->seg_of[i] = std::min(seg, ctxs[i]->sg.NTY);
+        }
         std::vector<int> base(n + 1, 0);
         int maxd = 0, maxk = 0;
+        size_t expect = 0;
         for (int i = 0; i < n; ++i) {
             Ctx &c = *ctxs[i];
             if (!c.pk) pk_setup(c);
             const SweepGeom &sg = c.sg;
             base[i + 1] = base[i] + 4 * sg.nblk_g * sg.NTX * sg.NTY;
-            maxd = std::max(maxd, sg.NTX + sg.NTY - 2);
+            const int nseg = (sg.NTY + b->seg_of[i] - 1) / b->seg_of[i];
+            maxd = std::max(maxd, num_diags(sg, nseg, b->segs) - 1);
+            expect += (size_t)4 * sg.nblk_g * sg.NTX * (b->segs ? nseg : sg.NTY);
             maxk = std::max(maxk, sg.K);
             b->nbt = std::max(b->nbt, 4 * c.ng * (c.nx + c.ny));
             b->updates += (long long)c.nx * c.ny * c.nm * c.ng;
         }
-        // items of all problems, merged by tile diagonal
+        // items of all problems, merged by tile (or segment) diagonal
         std::vector<int4> items;
         items.reserve((size_t)base[n]);
         for (int d = 0; d <= maxd; ++d)
             for (int i = 0; i < n; ++i) {
                 const SweepGeom &sg = ctxs[i]->sg;
-                if (d > sg.NTX + sg.NTY - 2) continue;
-                const int lo = std::max(0, d - (sg.NTY - 1)), hi = std::min(d, sg.NTX - 1);
-                for (int txl = lo; txl <= hi; ++txl) {
-                    const int tyl = d - txl;
-                    for (int q = 0; q < 4; ++q)
-                        for (int gb = 0; gb < sg.nblk_g; ++gb) {
-                            const int flag = base[i] + ((q * sg.nblk_g + gb) * sg.NTY + tyl) * sg.NTX + txl;
-                            items.push_back(make_int4(i, flag, txl | (tyl << 16), q | (gb << 8)));
-                        }
-                }
+                const int seg = b->seg_of[i], nseg = (sg.NTY + seg - 1) / seg;
+                if (d < num_diags(sg, nseg, b->segs)) diag_items(items, i, base[i], sg, seg, nseg, d, b->segs);
             }
         b->nitems = (int)items.size();
-        if ((size_t)b->nitems != (size_t)base[n]) throw Error("batched work list does not cover every tile");
+        if ((size_t)b->nitems != expect) throw Error("batched work list does not cover every tile");
         b->items = static_cast<int4 *>(batch_alloc(*b, items.size() * sizeof(int4)));
         TRT_CHECK(hipMemcpy(b->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
         b->flags = static_cast<int *>(batch_alloc(*b, (size_t)base[n] * sizeof(int)));
@@ -895,7 +1040,8 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
         b->probs = static_cast<PkProb *>(batch_alloc(*b, (size_t)n * sizeof(PkProb)));
         const size_t ti = b->precision ? 4 : 8;
         b->lds = (size_t)maxk * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti;
-        b->grid = resident_grid(b->k->pk, b->lds, b->nitems);
+        b->kern = b->segs ? b->k->seg : b->k->pk;
+        b->grid = resident_grid(b->kern, b->lds, b->nitems);
         if (pooled) b->stream = cache_stream_get();
         else TRT_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         TRT_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
@@ -927,7 +1073,7 @@ static void batch_enqueue(PkBatch &b)
     a.flags = b.flags;
     a.spin_limit = 1 << 20;
     void *args[] = {&a};
-    TRT_CHECK(hipLaunchKernel(b.k->pk, dim3(b.grid), dim3(PK_THREADS), args, b.lds, b.stream));
+    TRT_CHECK(hipLaunchKernel(b.kern, dim3(b.grid), dim3(PK_THREADS), args, b.lds, b.stream));
 }
 
 // One batched sweep: every problem's TRANSPORT_SCB (I_crn from I_crn_old) with
@@ -942,6 +1088,8 @@ void pk_batch_sweep(PkBatch &b, int mark)
         Ctx &c = *b.ctxs[i];
         if (b.up_iold[i] != c.iold) {
             b.host[i] = c.pk->host[c.iold];
+            b.host[i].seg = b.seg_of[i];
+            b.host[i].nseg = (b.host[i].NTY + b.seg_of[i] - 1) / b.seg_of[i];
             b.up_iold[i] = c.iold;
             dirty = true;
         }

@@ -35,8 +35,13 @@ def _randomize(ctx, prob, seed):
     ctx.call("material_update")
 
 
-@pytest.mark.parametrize("precision,tile", [(1, (4, 2)), (0, (2, 4)), (1, (2, 2))])
-def test_batch_sweep_equals_single(trt, precision, tile):
+@pytest.mark.parametrize("precision,tile,seg", [(1, (4, 2), None), (0, (2, 4), None), (1, (2, 2), None),
+                                               (1, (4, 2), "2"), (0, (2, 4), "3"), (2, (2, 2), "4")])
+def test_batch_sweep_equals_single(trt, monkeypatch, precision, tile, seg):
+    """seg forces the batch's column-segment length (TRT_SEG_LEN); by default the
+    batch picks tiles or segments from the available parallelism."""
+    if seg:
+        monkeypatch.setenv("TRT_SEG_LEN", seg)
     probs = _probs(trt)
     alone = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]
     mixed = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]

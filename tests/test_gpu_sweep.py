@@ -157,15 +157,17 @@ def test_push_kape_and_source_reaches_the_sweep(trt, oracle_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("precision,tile", [(0, (4, 2)), (0, (2, 2)), (1, (4, 4)), (1, (2, 8))])
+@pytest.mark.parametrize("precision,tile", [(0, (4, 2)), (0, (2, 2)), (1, (4, 4)), (1, (2, 8)), (2, (2, 2))])
 def test_persistent_equals_per_diagonal_bitwise(trt, precision, tile):
-    """The persistent launch and the per-diagonal launches run the same tile body on
-    the same data, so any stale or early read of an exchanged face would change bits."""
+    """The persistent launch (tile items), the column-segment launch and the
+    per-diagonal launches run the same tile body on the same data, so any stale or
+    early read of an exchanged face (or of a face carried in registers) would change
+    bits."""
     prob = trt.problem.small(nx=37, ny=29, order=8, ng=9, seed=5)
     rng = np.random.default_rng(5)
     T = rng.uniform(0.5, 800.0, size=(prob.nx, prob.ny))
     outs = []
-    for mode in (2, 3):
+    for mode in (2, 3, 4):
         ctx = trt.native.Context(prob, precision=precision, tile=tile, sweep_mode=mode)
         ctx.push("Temp", T)
         ctx.call("material_update")
@@ -176,6 +178,7 @@ def test_persistent_equals_per_diagonal_bitwise(trt, precision, tile):
             res.append([ctx.pull(n) for n in ("I_crn", "HO_Eg_avg", "HO_Fxg_edgV", "HO_Fyg_edgH", "Hg_avg_xy")])
         outs.append(res)
         ctx.close()
-    for a, b in zip(*outs):
-        for x, y in zip(a, b):
-            assert np.array_equal(x, y)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
