@@ -284,13 +284,12 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     sg.gblk = SWEEP_THREADS / sg.K;
     if (sg.gblk > p.ng) sg.gblk = p.ng;
     sg.nblk_g = (p.ng + sg.gblk - 1) / sg.gblk;
-    // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x2 (FP32) / 2x4 (FP64),
-    // the fastest measured on the 256x256, S16, 32-group problem
+    // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x2 (FP32), 2x4 (FP64) and
+    // 2x2 (FP32 storage with FP64 arithmetic), the fastest measured on the 256x256, S16,
+    // 32-group problem
     const bool engine = p.sweep_mode != 1;
-    // (precision 2, FP32 storage with FP64 arithmetic, takes the FP64 tiles)
-    const bool f32c = p.precision == 1;
-    sg.TX = p.tile_x > 0 ? p.tile_x : (engine && !f32c ? 2 : 4);
-    sg.TY = p.tile_y > 0 ? p.tile_y : (engine ? (f32c ? 2 : 4) : 4);
+    sg.TX = p.tile_x > 0 ? p.tile_x : (!engine ? 4 : p.precision == 1 ? 4 : 2);
+    sg.TY = p.tile_y > 0 ? p.tile_y : (!engine ? 4 : p.precision == 0 ? 4 : 2);
     sg.NTX = (p.nx + sg.TX - 1) / sg.TX;
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }

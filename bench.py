@@ -28,8 +28,8 @@ BASELINE_METRIC = "sweep cell\u00b7angle\u00b7group updates/s, FP32-batched vs p
 def alg_bytes(prob, precision: int) -> int:
     """Algorithmic bytes of one sweep: read 4 old corners + write 4 new corners per
     (cell, direction, group), read KapE+1/(c dt) and the isotropic source per (cell, group),
-    write 12 FP64 moments per (cell, group)."""
-    s_i = 4 if precision else 8
+    write 12 FP64 moments per (cell, group).  Precisions 1 and 2 store intensities in FP32."""
+    s_i = 4 if precision in (1, 2) else 8
     nc = prob.nx * prob.ny
     return nc * prob.nm * prob.ng * 8 * s_i + nc * prob.ng * (2 * 8 + 12 * 8)
 
@@ -59,10 +59,17 @@ def measure(trt, prob, precision: int, steps: int, warmup: int):
         "launches_per_sweep": nl // nsw,
         "alg_bytes_per_sweep": b,
         "achieved_gbs": b / (kern / nsw * 1e-3) / 1e9,
-        "kernel": (f"sweep_pk<{'float' if precision else 'double'},{tx},{ty}>" if mode == 2 else
-                   f"sweep_diag<{'float' if precision else 'double'},{tx},{ty}>" if mode == 3 else "sweep_kernel"),
+        "kernel": kernel_name(precision, mode, tx, ty),
         "sweep_mode": mode, "tile": [tx, ty], "grid": grid,
     }
+
+
+def kernel_name(precision: int, mode: int, tx: int, ty: int) -> str:
+    """Template name of the sweep kernel a context runs (as rocprofv3 reports it)."""
+    ti = "double" if precision == 0 else "float"
+    tc = "double" if precision != 1 else "float"
+    form = {2: "sweep_pk", 3: "sweep_diag", 4: "sweep_seg"}.get(mode)
+    return f"{form}<{ti}, {tc}, {tx}, {ty}>" if form else "sweep_kernel"
 
 
 def traffic_from_profiles(precision: int):
@@ -72,7 +79,7 @@ def traffic_from_profiles(precision: int):
         return None
     with open(path) as f:
         d = json.load(f)
-    key = "fp32" if precision else "fp64"
+    key = {0: "fp64", 1: "fp32", 2: "fp32s"}[precision]
     return d.get(key, {}).get("hbm_bytes_per_launch")
 
 
@@ -148,7 +155,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "fp32s"], default=None,
                     help="measure one form only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop load")
@@ -163,14 +170,15 @@ def main(argv=None):
     trt = importlib.import_module("2d_trt_rom_system_amd")
     prob = trt.problem.cfg1()
     res = {}
-    for prec, key in ((1, "fp32"), (0, "fp64")):
+    # fp32: FP32 intensities and arithmetic; fp64: plain FP64; fp32s: FP32 storage, FP64 arithmetic
+    for prec, key in ((1, "fp32"), (0, "fp64"), (2, "fp32s")):
         if args.only and args.only != key:
             continue
         print(f"[bench] {key} sweep, {args.warmup} warmup + {args.steps} timed", file=sys.stderr, flush=True)
         res[key] = measure(trt, prob, prec, args.steps, args.warmup)
-    head_key = "fp32" if "fp32" in res else "fp64"
+    head_key = "fp32" if "fp32" in res else ("fp64" if "fp64" in res else "fp32s")
     head = res[head_key]
-    prec = 1 if head_key == "fp32" else 0
+    prec = {"fp32": 1, "fp64": 0, "fp32s": 2}[head_key]
     out = {
         "metric": BASELINE_METRIC,
         "value": head["updates_per_s"],
@@ -182,7 +190,8 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (FP64 moment accumulation)" if head_key == "fp32" else "fp64",
+        "dtype": {"fp32": "fp32 (FP64 moment accumulation)", "fp64": "fp64",
+                  "fp32s": "fp32 storage, fp64 arithmetic"}[head_key],
         "data": "synthetic",
         "config": {
             "workload": "TRANSPORT_SCB sweep + fused angular moments; 256x256 cells, S16 (144 directions), "
@@ -207,6 +216,8 @@ def main(argv=None):
         out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
+    for k in res:
+        res[k]["traffic_bytes_per_launch"] = traffic_from_profiles({"fp32": 1, "fp64": 0, "fp32s": 2}[k])
     if not args.no_batch and not args.only:
         probs = batch_mix(trt, args.batch_gib)
         print(f"[bench] batch of {len(probs)} problems", file=sys.stderr, flush=True)
