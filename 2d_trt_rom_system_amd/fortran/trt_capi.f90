@@ -138,6 +138,11 @@ MODULE TRT_CAPI
       IMPORT :: C_DOUBLE, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION
+    ! one step of the sweep-only entry Tgen_QDf (ALGORITHMS.f08:1031-1095), Temp prescribed
+    INTEGER(C_INT) FUNCTION trt_tgen_step(ctx) BIND(C, name="trt_tgen_step")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
   END INTERFACE
 
 CONTAINS
