@@ -48,6 +48,7 @@ void validate(const trt_problem *p)
     if (p->precision == 2 && p->sweep_mode == 1)
         throw Error("precision 2 runs on the tile engine (sweep_mode 0, 2, 3 or 4)");
     if (p->sweep_mode < 0 || p->sweep_mode > 4) throw Error("sweep_mode must be 0, 1, 2, 3 or 4");
+    if (p->qd_solver < 0 || p->qd_solver > 1) throw Error("qd_solver must be 0 (direct) or 1 (BiCGSTAB)");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
     for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
         if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");
@@ -75,6 +76,7 @@ trt_ctx *ctx_create(const trt_problem *p, bool pooled, Arena *arena)
         c.precision = p->precision;
         c.lin_rtol = p->lin_rtol > 0 ? p->lin_rtol : 1e-13;
         c.lin_maxit = p->lin_maxit > 0 ? p->lin_maxit : 2000;
+        c.qd_solver = p->qd_solver;
         if (const char *v = getenv("TRT_VERBOSE")) c.verbose = atoi(v);
         if (pooled) c.stream = cache_stream_get();
         else TRT_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -97,6 +99,7 @@ void ctx_destroy(trt_ctx *ctx)
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     trt::pk_destroy(c);
     trt::direct_destroy(c);
+    trt::nd_destroy(c);
     for (size_t k = 0; k < c.owned.size(); ++k) {
         if (c.arena) arena_release(c.arena, c.owned[k], c.owned_bytes[k]);
         else if (c.pooled) cache_release(c.owned[k], c.owned_bytes[k]);

@@ -411,12 +411,52 @@ static KrylovArgs kargs(Ctx &c, QDSystems &s)
 // too far from diagonal dominance for Jacobi-BiCGSTAB (a hot cell next to a
 // cold one), so a system that a stage does not bring to 4 rtol on its true
 // residual restarts the next stage from its original guess.
+// Direct solve of the listed systems (nested dissection; block LU if a front is
+// singular), then their true residual checked against 1e4 lin_rtol |b| in
+// unscaled rows.  A failing system throws.
+static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vector<int> &which)
+{
+    const dim3 grid(a.RB, s.nsys), blk(RT);
+    const int fin_blocks = (s.nsys + 63) / 64;
+    try {
+        qd_nd(c, s, which);
+    } catch (const Error &e) {
+        if (e.msg.find("singular") == std::string::npos) throw;
+        if (c.verbose) fprintf(stderr, "[qd_solve] %s; block LU instead\n", e.msg.c_str());
+        qd_direct(c, s, which);
+    }
+    hipLaunchKernelGGL(k_colscale, grid, blk, 0, c.stream, a, s.ref, s.cs, -1.0);   // unit scales
+    hipLaunchKernelGGL(k_init, grid, blk, 0, c.stream, a);                          // |b| in the unscaled rows
+    hipLaunchKernelGGL(k_init_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+    hipLaunchKernelGGL(k_true_res, grid, blk, 0, c.stream, a);
+    hipLaunchKernelGGL(k_true_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a, c.scratch, 1e4 * c.lin_rtol);
+    std::vector<double> h(3 * (size_t)s.nsys);
+    TRT_CHECK(hipMemcpyAsync(h.data(), c.scratch, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
+    for (int k : which)
+        if (!(h[3 * k] <= 1e4 * c.lin_rtol * h[3 * k + 1] || h[3 * k] == 0.0)) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "QD direct solve inaccurate (system %d, |r|/|b| = %.3e)", k, h[3 * k] / h[3 * k + 1]);
+            throw Error(msg);
+        }
+}
+
 void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
 {
     KrylovArgs a = kargs(c, s);
     const dim3 grid(a.RB, s.nsys), blk(RT);
     const int fin_blocks = (s.nsys + 63) / 64;
     const size_t n = (size_t)s.nsys * s.nrow;
+    if (c.qd_solver == 0) {
+        // every system by the direct solver (reported as -1 iterations, the reference's
+        // flag for a direct solve, QS:499)
+        std::vector<int> all(s.nsys);
+        for (int k = 0; k < s.nsys; ++k) all[k] = k;
+        hipLaunchKernelGGL(k_clear_acc, dim3(fin_blocks), dim3(64), 0, c.stream, a);
+        direct_and_check(c, s, a, all);
+        if (its_out) for (int k = 0; k < s.nsys; ++k) its_out[k] = -1;
+        return;
+    }
     if (!ref_given) dcopy(c, s.ref, s.x, n);
     dcopy(c, s.x0, s.x, n);
     hipLaunchKernelGGL(k_refmax, grid, blk, 0, c.stream, a, s.ref);
@@ -506,27 +546,13 @@ void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
         hipLaunchKernelGGL(k_colscale, grid, blk, 0, c.stream, a, s.ref, s.cs, -1.0);   // unit scales
         hipLaunchKernelGGL(k_unscale, grid, blk, 0, c.stream, a);
         const auto t0 = std::chrono::steady_clock::now();
-        qd_direct(c, s, which);
+        direct_and_check(c, s, a, which);
         if (c.verbose) {
-            TRT_CHECK(hipStreamSynchronize(c.stream));
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             fprintf(stderr, "[qd_solve nsys=%d] direct solve of %zu systems (%zu skipped BiCGSTAB): %.1f ms\n", s.nsys,
                     which.size(), skipped.size(), ms);
         }
-        hipLaunchKernelGGL(k_init, grid, blk, 0, c.stream, a);          // |b| in the unscaled rows
-        hipLaunchKernelGGL(k_init_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a);
-        hipLaunchKernelGGL(k_true_res, grid, blk, 0, c.stream, a);
-        hipLaunchKernelGGL(k_true_fin, dim3(fin_blocks), dim3(64), 0, c.stream, a, c.scratch, 1e4 * c.lin_rtol);
-        TRT_CHECK(hipMemcpyAsync(h.data(), c.scratch, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
-        TRT_CHECK(hipStreamSynchronize(c.stream));
         for (int k : which) total_its[k] = -1;     // as the reference flags a direct solve (QS:499)
-        for (int k = 0; k < s.nsys; ++k)
-            if (!(h[3 * k] <= 1e4 * c.lin_rtol * h[3 * k + 1] || h[3 * k] == 0.0)) {
-                char msg[160];
-                snprintf(msg, sizeof msg, "QD Krylov solve did not converge (system %d, |r|/|b| = %.3e)", k,
-                         h[3 * k] / h[3 * k + 1]);
-                throw Error(msg);
-            }
     }
     if (its_out) for (int k = 0; k < s.nsys; ++k) its_out[k] = total_its[k];
 }

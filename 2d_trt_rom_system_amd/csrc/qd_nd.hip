@@ -1,0 +1,460 @@
+// Direct solve of the reduced QD systems by nested dissection (multifrontal LU).
+//
+// QD_FV's matrix (MainSrc/QD_SOLVERS.f08:40-538) couples a cell's E_avg to its
+// four faces, and a face's momentum equation to the two cells beside it and
+// the faces of those cells across the other direction.  On this staggered grid
+// a line of face unknowns separates the grid: the E_edgV of one face column
+// splits a rectangle of cells into left and right parts with no equation
+// coupling them except through that line, and likewise a row of E_edgH.  The
+// grid is bisected recursively (the longer side first) down to leaves of at
+// most 4x4 cells; every tree node owns its separator (a leaf owns all of its
+// unknowns), and its front holds those unknowns (I) and the separator unknowns
+// on the perimeter of its rectangle (B).
+//
+// Numeric phase, bottom-up by node height, batched over the nodes of a height
+// and the systems solved: assemble the node's matrix entries and right-hand
+// side into a dense front, add its children's Schur complements (each child
+// in turn, so the sums are deterministic), factor the I block (LU with partial
+// pivoting inside I), form A_II^-1 A_IB and the Schur complement
+// A_BB - A_BI A_II^-1 A_IB for the parent, and eliminate the right-hand side
+// the same way.  Back substitution runs top-down: x_I = A_II^-1 b_I - (A_II^-1
+// A_IB) x_B.  rocSOLVER getrf/getrs and rocBLAS gemm do the dense work on fronts
+// padded to the largest of their height (padding: identity in I, zeros in B).
+// At 256x256 cells this is about 1.5e9 flop per system and fronts of at most
+// 3 x 128 unknowns.
+#include "qd_view.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <functional>
+
+namespace trt {
+
+#define ND_RB(expr)                                                                        \
+    do {                                                                                   \
+        rocblas_status _s = (expr);                                                        \
+        if (_s != rocblas_status_success)                                                  \
+            throw Error(std::string(#expr) + ": rocBLAS status " + std::to_string((int)_s)); \
+    } while (0)
+
+namespace {
+
+constexpr int ND_LEAF = 4;      // leaves: at most ND_LEAF x ND_LEAF cells
+constexpr int ND_MAXT = 8;      // terms per equation (at most 7)
+
+struct NDLevDev {
+    int nn, nIp, nBp, nF;
+    double *F, *f;              // fronts [node][sys][nF][nF] (column-major), right-hand sides [node][sys][nF]
+    const int *up;              // [node][nBp] position of each B unknown in the parent's front
+};
+
+struct NDLevel {
+    int nn = 0, nIp = 0, nBp = 0, nF = 0;
+    int *rows = nullptr;        // [node][nF] global unknown of each front row (-1: padding)
+    int *terms = nullptr;       // [node][nF][ND_MAXT] front column of each term of the row's equation (-1: not here)
+    int *up = nullptr;          // [node][nBp]
+    int *kids = nullptr;        // [node][2] child as level << 20 | index (-1: none)
+    size_t foff = 0, voff = 0, poff = 0;   // offsets of the level in the F, f and pivot buffers (per system chunk)
+};
+
+__global__ void nd_zero(double *p, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0.0;
+}
+
+// one block per (node, system): the node's own matrix entries and right-hand side
+__global__ void nd_assemble(QDView V, const int *sysmap, int ns, int nIp, int nF, const int *rows, const int *terms,
+                            double *F, double *f)
+{
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    double *Fb = F + ((size_t)n * ns + s) * nF * nF;
+    double *fb = f + ((size_t)n * ns + s) * nF;
+    const int sys = sysmap[s];
+    for (int p = threadIdx.x; p < nF; p += blockDim.x) {
+        const int r = rows[(size_t)n * nF + p];
+        if (r < 0) {
+            if (p < nIp) Fb[(size_t)p + (size_t)p * nF] = 1.0;
+            continue;
+        }
+        const int *tc = terms + ((size_t)n * nF + p) * ND_MAXT;
+        int t = 0;
+        double dg, rhs;
+        qd_row_visit(V, sys, (size_t)r, [&](int, int, int, double a) {
+            const int col = tc[t++];
+            if (col >= 0) Fb[(size_t)p + (size_t)col * nF] += a;
+        }, dg, rhs);
+        if (p < nIp) fb[p] += rhs;
+    }
+}
+
+// one block per (node, system): add the children's Schur complements and
+// eliminated right-hand sides, one child after the other
+__global__ void nd_pull(const NDLevDev *lev, int L, int ns, const int *kids)
+{
+    const NDLevDev P = lev[L];
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    double *Fp = P.F + ((size_t)n * ns + s) * P.nF * P.nF;
+    double *fp = P.f + ((size_t)n * ns + s) * P.nF;
+    for (int k = 0; k < 2; ++k) {
+        const int kid = kids[n * 2 + k];
+        if (kid < 0) continue;
+        const NDLevDev C = lev[kid >> 20];
+        const int ci = kid & 0xfffff;
+        const double *Fc = C.F + ((size_t)ci * ns + s) * C.nF * C.nF;
+        const double *fc = C.f + ((size_t)ci * ns + s) * C.nF;
+        const int *up = C.up + (size_t)ci * C.nBp;
+        const int nb = C.nBp;
+        for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+            const int i = e % nb, j = e / nb;
+            const int pi = up[i], pj = up[j];
+            if (pi >= 0 && pj >= 0) Fp[(size_t)pi + (size_t)pj * P.nF] += Fc[(size_t)(C.nIp + i) + (size_t)(C.nIp + j) * C.nF];
+        }
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            const int pi = up[i];
+            if (pi >= 0) fp[pi] += fc[C.nIp + i];
+        }
+        __syncthreads();
+    }
+}
+
+// back substitution helpers: x_B into the front's right-hand side slot, x_I out
+__global__ void nd_gather(const double *x, size_t nrow, const int *sysmap, int ns, int nIp, int nBp, int nF,
+                          const int *rows, double *f)
+{
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    double *fb = f + ((size_t)n * ns + s) * nF;
+    const double *xs = x + (size_t)sysmap[s] * nrow;
+    for (int q = threadIdx.x; q < nBp; q += blockDim.x) {
+        const int r = rows[(size_t)n * nF + nIp + q];
+        fb[nIp + q] = r >= 0 ? xs[r] : 0.0;
+    }
+}
+
+__global__ void nd_scatter(double *x, size_t nrow, const int *sysmap, int ns, int nIp, int nF, const int *rows,
+                           const double *f)
+{
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    const double *fb = f + ((size_t)n * ns + s) * nF;
+    double *xs = x + (size_t)sysmap[s] * nrow;
+    for (int p = threadIdx.x; p < nIp; p += blockDim.x) {
+        const int r = rows[(size_t)n * nF + p];
+        if (r >= 0) xs[r] = fb[p];
+    }
+}
+
+__global__ void nd_info_max(const rocblas_int *info, size_t n, int *out)
+{
+    int m = 0;
+    for (size_t k = threadIdx.x; k < n; k += blockDim.x) m = max(m, (int)info[k]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    __shared__ int sh[16];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t = max(t, sh[k]);
+        *out = t;
+    }
+}
+
+int nblocks(size_t n) { return (int)std::min<size_t>(std::max<size_t>((n + 255) / 256, 1), 65536); }
+
+}  // namespace
+
+struct NDPlan {
+    int nx = 0, ny = 0;
+    size_t nrow = 0;
+    std::vector<NDLevel> lev;
+    NDLevDev *levdev = nullptr;     // device copy of the level table for the current chunk
+    int cap = 0;                    // systems per chunk
+    size_t fsz = 0, vsz = 0, psz = 0;   // per-system doubles of F, f and pivots over all levels
+    double *F = nullptr, *f = nullptr;
+    rocblas_int *ipiv = nullptr, *info = nullptr;
+    int *sysmap = nullptr, *imax = nullptr;
+    size_t ninfo = 0;
+    rocblas_handle h = nullptr;
+    double flops = 0.0;             // per system
+};
+
+void nd_destroy(Ctx &c)
+{
+    if (!c.nd) return;
+    if (c.nd->h) rocblas_destroy_handle(c.nd->h);
+    delete c.nd;     // device buffers belong to the context (dalloc)
+    c.nd = nullptr;
+}
+
+// Symbolic phase: the dissection tree, fronts, assembly and extend-add maps.
+static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
+{
+    const int nx = s.nx, ny = s.ny;
+    const size_t nc = (size_t)nx * ny, nv = (size_t)(nx + 1) * ny;
+    auto C = [&](int i, int j) { return (int)(i + (size_t)nx * j); };
+    auto Vf = [&](int f, int j) { return (int)(nc + f + (size_t)(nx + 1) * j); };
+    auto Hf = [&](int i, int f) { return (int)(nc + nv + i + (size_t)nx * f); };
+    struct Node {
+        int kid[2] = {-1, -1}, height = 0, lo = 0, level = 0, index = 0, parent = -1;
+        std::vector<int> I, B;
+    };
+    std::vector<Node> nodes;      // post-order
+    std::function<int(int, int, int, int)> rec = [&](int i0, int i1, int j0, int j1) -> int {
+        Node nd;
+        if (i1 - i0 <= ND_LEAF && j1 - j0 <= ND_LEAF) {
+            for (int j = j0; j < j1; ++j)
+                for (int i = i0; i < i1; ++i) nd.I.push_back(C(i, j));
+            for (int j = j0; j < j1; ++j)
+                for (int f = i0 + 1; f < i1; ++f) nd.I.push_back(Vf(f, j));
+            if (i0 == 0) for (int j = j0; j < j1; ++j) nd.I.push_back(Vf(0, j));
+            if (i1 == nx) for (int j = j0; j < j1; ++j) nd.I.push_back(Vf(nx, j));
+            for (int f = j0 + 1; f < j1; ++f)
+                for (int i = i0; i < i1; ++i) nd.I.push_back(Hf(i, f));
+            if (j0 == 0) for (int i = i0; i < i1; ++i) nd.I.push_back(Hf(i, 0));
+            if (j1 == ny) for (int i = i0; i < i1; ++i) nd.I.push_back(Hf(i, ny));
+        } else if (i1 - i0 >= j1 - j0) {
+            const int f = (i0 + i1) / 2;
+            nd.kid[0] = rec(i0, f, j0, j1);
+            nd.kid[1] = rec(f, i1, j0, j1);
+            for (int j = j0; j < j1; ++j) nd.I.push_back(Vf(f, j));
+        } else {
+            const int f = (j0 + j1) / 2;
+            nd.kid[0] = rec(i0, i1, j0, f);
+            nd.kid[1] = rec(i0, i1, f, j1);
+            for (int i = i0; i < i1; ++i) nd.I.push_back(Hf(i, f));
+        }
+        if (i0 > 0) for (int j = j0; j < j1; ++j) nd.B.push_back(Vf(i0, j));
+        if (i1 < nx) for (int j = j0; j < j1; ++j) nd.B.push_back(Vf(i1, j));
+        if (j0 > 0) for (int i = i0; i < i1; ++i) nd.B.push_back(Hf(i, j0));
+        if (j1 < ny) for (int i = i0; i < i1; ++i) nd.B.push_back(Hf(i, j1));
+        for (int k = 0; k < 2; ++k)
+            if (nd.kid[k] >= 0) nd.height = std::max(nd.height, nodes[nd.kid[k]].height + 1);
+        nodes.push_back(std::move(nd));
+        return (int)nodes.size() - 1;
+    };
+    const int root = rec(0, nx, 0, ny);
+    // subtree ranges [lo, self] in post-order, parents
+    std::function<int(int)> span = [&](int k) -> int {
+        int lo = k;
+        for (int q = 0; q < 2; ++q)
+            if (nodes[k].kid[q] >= 0) {
+                nodes[nodes[k].kid[q]].parent = k;
+                lo = std::min(lo, span(nodes[k].kid[q]));
+            }
+        nodes[k].lo = lo;
+        return lo;
+    };
+    span(root);
+    auto anc_or_self = [&](int a, int k) { return nodes[a].lo <= k && k <= a; };
+    std::vector<int> owner(s.nrow, -1);
+    for (int k = 0; k < (int)nodes.size(); ++k)
+        for (int u : nodes[k].I) owner[u] = k;
+    for (size_t u = 0; u < s.nrow; ++u)
+        if (owner[u] < 0) throw Error("nested dissection: an unknown has no owner");
+
+    NDPlan *P = new NDPlan();
+    P->nx = nx; P->ny = ny; P->nrow = s.nrow;
+    const int H = nodes[root].height;
+    P->lev.resize(H + 1);
+    for (int k = 0; k < (int)nodes.size(); ++k) {
+        NDLevel &L = P->lev[nodes[k].height];
+        nodes[k].level = nodes[k].height;
+        nodes[k].index = L.nn++;
+        L.nIp = std::max(L.nIp, (int)nodes[k].I.size());
+        L.nBp = std::max(L.nBp, (int)nodes[k].B.size());
+    }
+    for (NDLevel &L : P->lev) L.nF = L.nIp + L.nBp;
+    // front position of a global unknown in node k's front
+    auto front_pos = [&](int k, int u) -> int {
+        const Node &nd = nodes[k];
+        const NDLevel &L = P->lev[nd.level];
+        for (size_t p = 0; p < nd.I.size(); ++p) if (nd.I[p] == u) return (int)p;
+        for (size_t q = 0; q < nd.B.size(); ++q) if (nd.B[q] == u) return L.nIp + (int)q;
+        return -1;
+    };
+    // host stencil walk: coefficient values are irrelevant, only the terms' unknowns
+    QDView V{};
+    V.nx = nx; V.ny = ny; V.nb = std::max(nx, ny);
+    V.nc = nc; V.nv = nv; V.nh = (size_t)nx * (ny + 1); V.nrow = s.nrow;
+    std::vector<double> zc((size_t)NCOEF * nc, 0.0), zb((size_t)NBND * V.nb, 0.0);
+    V.coef = zc.data(); V.bnd = zb.data(); V.cs = nullptr;
+    std::vector<std::vector<int>> hrows(H + 1), hterms(H + 1), hup(H + 1), hkids(H + 1);
+    for (int l = 0; l <= H; ++l) {
+        const NDLevel &L = P->lev[l];
+        hrows[l].assign((size_t)L.nn * L.nF, -1);
+        hterms[l].assign((size_t)L.nn * L.nF * ND_MAXT, -1);
+        hup[l].assign((size_t)L.nn * std::max(L.nBp, 1), -1);
+        hkids[l].assign((size_t)L.nn * 2, -1);
+    }
+    double flops = 0.0;
+    for (int k = 0; k < (int)nodes.size(); ++k) {
+        const Node &nd = nodes[k];
+        const int l = nd.level, n = nd.index;
+        const NDLevel &L = P->lev[l];
+        // a local lookup for the node's front (small fronts: linear search is fine)
+        std::vector<std::pair<int, int>> loc;
+        for (size_t p = 0; p < nd.I.size(); ++p) loc.push_back({nd.I[p], (int)p});
+        for (size_t q = 0; q < nd.B.size(); ++q) loc.push_back({nd.B[q], L.nIp + (int)q});
+        std::sort(loc.begin(), loc.end());
+        auto pos = [&](int u) {
+            auto it = std::lower_bound(loc.begin(), loc.end(), std::make_pair(u, -1));
+            return (it != loc.end() && it->first == u) ? it->second : -1;
+        };
+        auto walk = [&](int p, int r, bool own_row) {
+            int t = 0;
+            double dg, rhs;
+            qd_row_visit(V, 0, (size_t)r, [&](int kind, int i, int j, double) {
+                const int col = (int)qd_index(V, kind, i, j);
+                const int o = owner[col];
+                int fc = -1;
+                if (own_row) {
+                    if (anc_or_self(o, k)) {
+                        fc = pos(col);
+                        if (fc < 0) throw Error("nested dissection: a coupling leaves its front");
+                    } else if (!anc_or_self(k, o)) {
+                        throw Error("nested dissection: a coupling crosses a separator");
+                    }
+                } else if (o == k) {
+                    fc = pos(col);
+                }
+                if (t >= ND_MAXT) throw Error("nested dissection: too many terms in an equation");
+                hterms[l][((size_t)n * L.nF + p) * ND_MAXT + t] = fc;
+                ++t;
+            }, dg, rhs);
+        };
+        for (size_t p = 0; p < nd.I.size(); ++p) {
+            hrows[l][(size_t)n * L.nF + p] = nd.I[p];
+            walk((int)p, nd.I[p], true);
+        }
+        for (size_t q = 0; q < nd.B.size(); ++q) {
+            const int p = L.nIp + (int)q;
+            hrows[l][(size_t)n * L.nF + p] = nd.B[q];
+            walk(p, nd.B[q], false);
+            if (nd.parent >= 0) {
+                const int pp = front_pos(nd.parent, nd.B[q]);
+                if (pp < 0) throw Error("nested dissection: a boundary unknown is missing from the parent front");
+                hup[l][(size_t)n * L.nBp + q] = pp;
+            }
+        }
+        for (int q = 0; q < 2; ++q)
+            if (nd.kid[q] >= 0) hkids[l][(size_t)n * 2 + q] = (nodes[nd.kid[q]].level << 20) | nodes[nd.kid[q]].index;
+        const double nI = (double)nd.I.size(), nB = (double)nd.B.size();
+        flops += 2.0 / 3.0 * nI * nI * nI + 2.0 * nI * nI * nB + 2.0 * nB * nB * nI;
+    }
+    P->flops = flops;
+    size_t fo = 0, vo = 0, po = 0;
+    for (int l = 0; l <= H; ++l) {
+        NDLevel &L = P->lev[l];
+        L.rows = static_cast<int *>(c.dalloc(hrows[l].size() * sizeof(int)));
+        L.terms = static_cast<int *>(c.dalloc(hterms[l].size() * sizeof(int)));
+        L.up = static_cast<int *>(c.dalloc(hup[l].size() * sizeof(int)));
+        L.kids = static_cast<int *>(c.dalloc(hkids[l].size() * sizeof(int)));
+        TRT_CHECK(hipMemcpy(L.rows, hrows[l].data(), hrows[l].size() * sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpy(L.terms, hterms[l].data(), hterms[l].size() * sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpy(L.up, hup[l].data(), hup[l].size() * sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpy(L.kids, hkids[l].data(), hkids[l].size() * sizeof(int), hipMemcpyHostToDevice));
+        L.foff = fo; L.voff = vo; L.poff = po;
+        fo += (size_t)L.nn * L.nF * L.nF;
+        vo += (size_t)L.nn * L.nF;
+        po += (size_t)L.nn * L.nIp;
+    }
+    P->fsz = fo; P->vsz = vo; P->psz = po;
+    // systems per chunk: fronts of all levels within 8 GiB
+    const double budget = 8.0 * (1u << 30);
+    P->cap = (int)std::max(1.0, std::min<double>(std::max(s.nsys, c.ng), std::floor(budget / (8.0 * (double)fo))));
+    P->F = static_cast<double *>(c.dalloc((size_t)P->cap * fo * sizeof(double)));
+    P->f = static_cast<double *>(c.dalloc((size_t)P->cap * vo * sizeof(double)));
+    P->ipiv = static_cast<rocblas_int *>(c.dalloc((size_t)P->cap * po * sizeof(rocblas_int)));
+    size_t ni = 0;
+    for (const NDLevel &L : P->lev) ni += (size_t)L.nn;
+    P->ninfo = ni * P->cap;
+    P->info = static_cast<rocblas_int *>(c.dalloc(P->ninfo * sizeof(rocblas_int)));
+    P->sysmap = static_cast<int *>(c.dalloc((size_t)std::max(P->cap, 1) * sizeof(int)));
+    P->imax = static_cast<int *>(c.dalloc(sizeof(int)));
+    P->levdev = static_cast<NDLevDev *>(c.dalloc((H + 1) * sizeof(NDLevDev)));
+    ND_RB(rocblas_create_handle(&P->h));
+    return P;
+}
+
+// Solve the listed systems of s; their solutions replace s.x.
+void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
+{
+    if (which.empty()) return;
+    if (c.nd && (c.nd->nx != s.nx || c.nd->ny != s.ny || c.nd->nrow != s.nrow)) nd_destroy(c);
+    if (!c.nd) c.nd = nd_plan(c, s);
+    NDPlan &P = *c.nd;
+    ND_RB(rocblas_set_stream(P.h, c.stream));
+    QDView V{};
+    V.nx = s.nx; V.ny = s.ny; V.nb = std::max(s.nx, s.ny);
+    V.nc = (size_t)s.nx * s.ny; V.nv = (size_t)(s.nx + 1) * s.ny; V.nh = (size_t)s.nx * (s.ny + 1);
+    V.nrow = s.nrow; V.coef = s.coef; V.bnd = s.bnd; V.cs = nullptr;
+    const int H = (int)P.lev.size() - 1;
+    const double one = 1.0, mone = -1.0;
+    for (size_t c0 = 0; c0 < which.size(); c0 += P.cap) {
+        const int ns = (int)std::min<size_t>(P.cap, which.size() - c0);
+        TRT_CHECK(hipMemcpyAsync(P.sysmap, which.data() + c0, ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+        // level table for this chunk size
+        std::vector<NDLevDev> tab(H + 1);
+        for (int l = 0; l <= H; ++l) {
+            const NDLevel &L = P.lev[l];
+            tab[l] = NDLevDev{L.nn, L.nIp, L.nBp, L.nF, P.F + L.foff * ns, P.f + L.voff * ns, L.up};
+        }
+        TRT_CHECK(hipMemcpyAsync(P.levdev, tab.data(), tab.size() * sizeof(NDLevDev), hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.fsz * ns)), dim3(256), 0, c.stream, P.F, P.fsz * ns);
+        hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
+        size_t ioff = 0;
+        for (int l = 0; l <= H; ++l) {
+            const NDLevel &L = P.lev[l];
+            double *F = tab[l].F, *f = tab[l].f;
+            const int nb = L.nn * ns;
+            hipLaunchKernelGGL(nd_assemble, dim3(nb), dim3(128), 0, c.stream, V, P.sysmap, ns, L.nIp, L.nF, L.rows,
+                               L.terms, F, f);
+            if (l > 0) hipLaunchKernelGGL(nd_pull, dim3(nb), dim3(256), 0, c.stream, P.levdev, l, ns, L.kids);
+            TRT_CHECK(hipGetLastError());
+            const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
+            rocblas_int *piv = P.ipiv + L.poff * ns;
+            ND_RB(rocsolver_dgetrf_strided_batched(P.h, L.nIp, L.nIp, F, L.nF, sF, piv, sP, P.info + ioff, nb));
+            ioff += (size_t)nb;
+            ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, 1, F, L.nF, sF, piv, sP, f,
+                                                   L.nF, sf, nb));
+            if (L.nBp > 0) {
+                double *FIB = F + (size_t)L.nIp * L.nF, *FBI = F + L.nIp, *FBB = F + L.nIp + (size_t)L.nIp * L.nF;
+                ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, L.nBp, F, L.nF, sF, piv,
+                                                       sP, FIB, L.nF, sF, nb));
+                ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nBp, L.nBp,
+                                                    L.nIp, &mone, FBI, L.nF, sF, FIB, L.nF, sF, &one, FBB, L.nF, sF,
+                                                    nb));
+                ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nBp, 1,
+                                                    L.nIp, &mone, FBI, L.nF, sF, f, L.nF, sf, &one, f + L.nIp, L.nF,
+                                                    sf, nb));
+            }
+        }
+        hipLaunchKernelGGL(nd_info_max, dim3(1), dim3(1024), 0, c.stream, P.info, ioff, P.imax);
+        // back substitution, top-down
+        for (int l = H; l >= 0; --l) {
+            const NDLevel &L = P.lev[l];
+            double *F = tab[l].F, *f = tab[l].f;
+            const int nb = L.nn * ns;
+            if (L.nBp > 0) {
+                hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, s.x, s.nrow, P.sysmap, ns, L.nIp,
+                                   L.nBp, L.nF, L.rows, f);
+                const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF;
+                ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nIp, 1,
+                                                    L.nBp, &mone, F + (size_t)L.nIp * L.nF, L.nF, sF, f + L.nIp,
+                                                    L.nF, sf, &one, f, L.nF, sf, nb));
+            }
+            hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, s.x, s.nrow, P.sysmap, ns, L.nIp, L.nF,
+                               L.rows, f);
+        }
+        TRT_CHECK(hipGetLastError());
+        int imax = 0;
+        TRT_CHECK(hipMemcpyAsync(&imax, P.imax, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+        if (imax > 0) throw Error("nested dissection: a front's pivot block is singular");
+    }
+    if (c.verbose > 1)
+        fprintf(stderr, "[qd_nd] %zu systems, %zu levels, %.2e flop per system\n", which.size(), P.lev.size(), P.flops);
+}
+
+}  // namespace trt

@@ -23,8 +23,10 @@ struct QDView {
 // The equation owned by unknown `row` of system `sys`: term(kind, i, j, a) is called
 // for every coefficient a multiplying unknown (kind, i, j) (kind 0 = E_avg(i,j),
 // 1 = E_edgV(i,j), 2 = E_edgH(i,j)); diag and rhs receive A_row,row and b_row.
+// (host-callable too: the direct solver's plan walks the stencil on the host)
 template <typename F>
-__device__ __forceinline__ void qd_row_visit(const QDView &V, int sys, size_t row, F term, double &diag, double &rhs)
+__host__ __device__ __forceinline__ void qd_row_visit(const QDView &V, int sys, size_t row, F term, double &diag,
+                                                      double &rhs)
 {
     const int nx = V.nx, ny = V.ny;
     const double *cf = V.coef + (size_t)sys * NCOEF * V.nc;
@@ -84,7 +86,7 @@ __device__ __forceinline__ void qd_row_visit(const QDView &V, int sys, size_t ro
 #undef CF
 }
 
-__device__ __forceinline__ size_t qd_index(const QDView &V, int kind, int i, int j)
+__host__ __device__ __forceinline__ size_t qd_index(const QDView &V, int kind, int i, int j)
 {
     return kind == 0 ? (size_t)i + (size_t)V.nx * j
          : kind == 1 ? V.nc + (size_t)i + (size_t)(V.nx + 1) * j

@@ -70,6 +70,7 @@ struct QDSystems {
 struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
 struct Arena;     // one device allocation carved into contexts' blocks (state.hip)
 struct DirectWork;  // block-LU workspace of the direct QD fallback (qd_direct.hip)
+struct NDPlan;      // nested-dissection plan and fronts of the direct QD solver (qd_nd.hip)
 
 struct Ctx {
     trt_problem p{};
@@ -119,6 +120,8 @@ struct Ctx {
     double lin_rtol = 1e-13;
     std::vector<int> last_kits;
     DirectWork *direct = nullptr;
+    NDPlan *nd = nullptr;
+    int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback
 
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
@@ -223,6 +226,9 @@ void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given);
 // direct block-LU solve of the listed systems of s (their solutions replace s.x)
 void qd_direct(Ctx &c, QDSystems &s, const std::vector<int> &which);
 void direct_destroy(Ctx &c);
+// nested-dissection direct solve of the listed systems of s (solutions replace s.x)
+void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which);
+void nd_destroy(Ctx &c);
 // device copy helper
 void dcopy(Ctx &c, double *dst, const double *src, size_t n);
 void dfill(Ctx &c, double *dst, double v, size_t n);

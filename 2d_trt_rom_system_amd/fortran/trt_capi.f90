@@ -22,6 +22,7 @@ MODULE TRT_CAPI
     REAL(C_DOUBLE) :: lin_rtol
     INTEGER(C_INT) :: lin_maxit
     INTEGER(C_INT) :: sweep_mode
+    INTEGER(C_INT) :: qd_solver
   END TYPE trt_problem
 
   ! the context of the running problem, created by TRT_GPU_INIT
@@ -241,6 +242,7 @@ SUBROUTINE TRT_GPU_INIT(Omega_x, Omega_y, quad_weight, Nu_g, Delx, Dely, Mat, Ka
   p%kapE_dT_flag = MERGE(1, 0, kapE_dT_flag)
   p%E_bound_low = E_Bound_Low; p%T_bound_low = T_Bound_Low
   p%threads = 1; p%precision = precision; p%tile_x = 0; p%tile_y = 0; p%lin_rtol = 0d0; p%lin_maxit = 0; p%sweep_mode = 0
+  p%qd_solver = 0
   CALL TRT_CHECK(trt_ctx_create(TRT_CTX, p), 'trt_ctx_create')
   DEALLOCATE(mat0, ox, oy, w, nu, dx, dy, km)
 

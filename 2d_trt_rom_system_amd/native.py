@@ -33,7 +33,7 @@ class TrtProblem(C.Structure):
         ("E_bound_low", C.c_double), ("T_bound_low", C.c_double),
         ("threads", C.c_int),
         ("precision", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
-        ("lin_rtol", C.c_double), ("lin_maxit", C.c_int), ("sweep_mode", C.c_int),
+        ("lin_rtol", C.c_double), ("lin_maxit", C.c_int), ("sweep_mode", C.c_int), ("qd_solver", C.c_int),
     ]
 
 
@@ -137,7 +137,7 @@ def load(path: str = LIBPATH):
 
 
 def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0,
-                        sweep_mode: int = 0):
+                        sweep_mode: int = 0, qd_solver: int = 0):
     """Fill a TrtProblem from a problem.Problem; returns (struct, keep-alive list)."""
     keep = []
 
@@ -171,6 +171,7 @@ def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float =
     s.tile_x, s.tile_y = tile
     s.lin_rtol, s.lin_maxit = lin_rtol, lin_maxit
     s.sweep_mode = sweep_mode
+    s.qd_solver = qd_solver
     return s, keep
 
 
@@ -192,11 +193,11 @@ class Context:
     """One device-resident problem (a trt_ctx)."""
 
     def __init__(self, prob, precision: int = 0, tile=(0, 0), lin_rtol: float = 0.0, lin_maxit: int = 0,
-                 sweep_mode: int = 0):
+                 sweep_mode: int = 0, qd_solver: int = 0):
         self.L = load()
         self.prob = prob
         self.precision = precision
-        s, keep = make_problem_struct(prob, precision, tile, lin_rtol, lin_maxit, sweep_mode)
+        s, keep = make_problem_struct(prob, precision, tile, lin_rtol, lin_maxit, sweep_mode, qd_solver)
         h = C.c_void_p()
         rc = self.L.trt_ctx_create(C.byref(h), C.byref(s))
         if rc != 0:

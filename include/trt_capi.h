@@ -51,12 +51,15 @@ typedef struct {
                                        arithmetic in the corner solve (tile engine only) */
     int tile_x, tile_y;             /* sweep tile shape; 0 = default                 */
     double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
-    int lin_maxit;                  /* iteration cap of the QD Krylov solves; 0 = default 2000.  A system
-                                       that BiCGSTAB does not solve goes to a block-LU direct solve */
+    int lin_maxit;                  /* iteration cap of the QD Krylov solves (qd_solver 1); 0 = default
+                                       2000.  A system BiCGSTAB does not solve goes to the direct solve */
     int sweep_mode;                 /* 0 = default (2); 1 = one launch per tile diagonal with a
                                        per-cell moment reduction; 2 = one persistent launch per
                                        sweep; 3 = one launch per tile diagonal with per-row
-                                       moment reduction                                   */
+                                       moment reduction; 4 = one persistent launch over column
+                                       segments of tiles                                  */
+    int qd_solver;                  /* reduced QD systems: 0 = nested-dissection direct solve;
+                                       1 = batched BiCGSTAB with the direct solve as fallback */
 } trt_problem;
 
 /* Context life cycle.  trt_ctx_create allocates the device-resident state and

@@ -18,14 +18,14 @@ def sync_to_gpu(o, ctx, names):
         ctx.push(n, o[n])
 
 
-def _pair(trt, oracle_mod, steps=1, lin_maxit=0, **kw):
+def _pair(trt, oracle_mod, steps=1, lin_maxit=0, qd_solver=0, **kw):
     prob = trt.problem.small(nx=9, ny=7, order=8, ng=3)
     for k, v in kw.items():
         setattr(prob, k, v)
     o = oracle_mod.Oracle(prob, threads=3)
     for _ in range(steps):
         o.step()
-    ctx = trt.native.Context(prob, lin_maxit=lin_maxit)
+    ctx = trt.native.Context(prob, lin_maxit=lin_maxit, qd_solver=qd_solver)
     return prob, o, ctx
 
 
@@ -53,11 +53,12 @@ def test_old_coefs_match(trt, oracle_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("lin_maxit", [0, 1])
-def test_mloqd_fv_matches(trt, oracle_mod, lin_maxit):
-    """lin_maxit = 1 starves BiCGSTAB (one iteration per restart), so groups go
-    through the block-LU direct fallback (reported as -1 iterations)."""
-    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit)
+@pytest.mark.parametrize("qd_solver,lin_maxit", [(0, 0), (1, 0), (1, 1)])
+def test_mloqd_fv_matches(trt, oracle_mod, qd_solver, lin_maxit):
+    """qd_solver 0 solves every group directly (nested dissection, reported as -1
+    iterations); with BiCGSTAB (qd_solver 1) lin_maxit = 1 starves it (one iteration
+    per restart), so groups go through the direct fallback."""
+    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit, qd_solver=qd_solver)
     sync_to_gpu(o, ctx, ML_INPUTS)
     if lin_maxit == 1:
         # an initial guess that is not already the solution
@@ -66,7 +67,10 @@ def test_mloqd_fv_matches(trt, oracle_mod, lin_maxit):
     o.call("mloqd_fv")
     kits = ctx.mloqd_fv()
     assert len(kits) == prob.ng, kits
-    assert (-1 in kits) if lin_maxit == 1 else all(k >= 0 for k in kits), kits
+    if qd_solver == 0:
+        assert all(k == -1 for k in kits), kits
+    else:
+        assert (-1 in kits) if lin_maxit == 1 else all(k >= 0 for k in kits), kits
     for n in ("Eg_avg", "Eg_edgV", "Eg_edgH", "Fxg_edgV", "Fyg_edgH"):
         assert rel_err(ctx.pull(n), o[n].ravel(order="F")) < 1e-10, n
     ctx.close()
@@ -94,12 +98,12 @@ GREY_STATE = ["E_avg", "E_edgV", "E_edgH", "Fx_edgV", "Fy_edgH", "Temp", "KapE_B
               "F_in_L", "F_in_B", "F_in_R", "F_in_T", "Temp_MGQDold2", "KapE_bar_MGQDold"]
 
 
-@pytest.mark.parametrize("mgqd_it,lin_maxit", [(1, 0), (2, 0), (1, 1)])
-def test_egp_newton_matches(trt, oracle_mod, mgqd_it, lin_maxit):
+@pytest.mark.parametrize("mgqd_it,qd_solver,lin_maxit", [(1, 0, 0), (2, 0, 0), (1, 1, 0), (2, 1, 0), (1, 1, 1)])
+def test_egp_newton_matches(trt, oracle_mod, mgqd_it, qd_solver, lin_maxit):
     """Line search off: both Newton iterations reach the same grey fixed point (the line
     search's accept/reject decisions follow rounding noise once a delta is tiny).
-    lin_maxit = 1 sends the grey linear solves through the direct fallback."""
-    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit, use_line_search=0)
+    With BiCGSTAB, lin_maxit = 1 sends the grey linear solves through the direct fallback."""
+    prob, o, ctx = _pair(trt, oracle_mod, steps=1, lin_maxit=lin_maxit, qd_solver=qd_solver, use_line_search=0)
     o.call("mloqd_fv")
     o.call("collapse_mg_ef")
     o.call("cbar_calc")
@@ -139,17 +143,20 @@ def _l2(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
 
 
-@pytest.mark.parametrize("name,kw,tol", [
-    ("small", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8),
-    ("small", dict(), 1e-5),
-    ("cfg0", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8),
+@pytest.mark.parametrize("name,kw,tol,qd_solver", [
+    ("small", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 0),
+    ("small", dict(), 1e-5, 0),
+    ("cfg0", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 0),
+    ("cfg0", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 1),
+    ("map20", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 0),
 ])
-def test_time_steps_match(trt, oracle_mod, name, kw, tol):
-    prob = trt.problem.small(nx=12, ny=10, order=4, ng=3) if name == "small" else trt.problem.cfg0()
+def test_time_steps_match(trt, oracle_mod, name, kw, tol, qd_solver):
+    prob = {"small": lambda: trt.problem.small(nx=12, ny=10, order=4, ng=3), "cfg0": trt.problem.cfg0,
+            "map20": lambda: trt.problem.mapped(nx=20, order=4, ng=6)}[name]()
     for k, v in kw.items():
         setattr(prob, k, v)
     o = oracle_mod.Oracle(prob, threads=4)
-    ctx = trt.native.Context(prob)
+    ctx = trt.native.Context(prob, qd_solver=qd_solver)
     for s in range(3):
         it_o = o.step()
         it_g = ctx.step()
