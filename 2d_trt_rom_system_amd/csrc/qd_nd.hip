@@ -12,7 +12,9 @@
 // on the perimeter of its rectangle (B).
 //
 // Numeric phase, bottom-up by node height, batched over the nodes of a height
-// and the systems solved: assemble the node's matrix entries and right-hand
+// and the systems solved (every equation scaled by 1 / its largest
+// coefficient, then three steps of iterative refinement with the stored
+// factors at the end): assemble the node's matrix entries and right-hand
 // side into a dense front, add its children's Schur complements (each child
 // in turn, so the sums are deterministic), factor the I block (LU with partial
 // pivoting inside I), form A_II^-1 A_IB and the Schur complement
@@ -65,9 +67,24 @@ __global__ void nd_zero(double *p, size_t n)
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0.0;
 }
 
+// Row equilibration: each equation is scaled by 1 / max |coefficient|, so that
+// the partial pivoting inside a front compares rows of one scale (energy
+// balances of optically thick cells otherwise dwarf the momentum rows).
+__global__ void nd_rowscale(QDView V, const int *sysmap, int ns, double *W)
+{
+    const size_t tot = (size_t)ns * V.nrow;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(k / V.nrow);
+        const size_t row = k - (size_t)s * V.nrow;
+        double m = 0.0, dg, rhs;
+        qd_row_visit(V, sysmap[s], row, [&](int, int, int, double a) { m = fmax(m, fabs(a)); }, dg, rhs);
+        W[k] = m > 0.0 ? 1.0 / m : 1.0;
+    }
+}
+
 // one block per (node, system): the node's own matrix entries and right-hand side
 __global__ void nd_assemble(QDView V, const int *sysmap, int ns, int nIp, int nF, const int *rows, const int *terms,
-                            double *F, double *f)
+                            const double *W, double *F, double *f)
 {
     const int n = blockIdx.x / ns, s = blockIdx.x % ns;
     double *Fb = F + ((size_t)n * ns + s) * nF * nF;
@@ -80,13 +97,14 @@ __global__ void nd_assemble(QDView V, const int *sysmap, int ns, int nIp, int nF
             continue;
         }
         const int *tc = terms + ((size_t)n * nF + p) * ND_MAXT;
+        const double w = W[(size_t)s * V.nrow + r];
         int t = 0;
         double dg, rhs;
         qd_row_visit(V, sys, (size_t)r, [&](int, int, int, double a) {
             const int col = tc[t++];
-            if (col >= 0) Fb[(size_t)p + (size_t)col * nF] += a;
+            if (col >= 0) Fb[(size_t)p + (size_t)col * nF] += w * a;
         }, dg, rhs);
-        if (p < nIp) fb[p] += rhs;
+        if (p < nIp) fb[p] += w * rhs;
     }
 }
 
@@ -145,6 +163,63 @@ __global__ void nd_scatter(double *x, size_t nrow, const int *sysmap, int ns, in
     }
 }
 
+// iterative refinement: residual r = b - A x of the chunk's systems, right-hand
+// sides of the fronts from r, the children's eliminated parts, and x += dx
+__global__ void nd_residual(QDView V, const int *sysmap, int ns, const double *x, double *R)
+{
+    const size_t tot = (size_t)ns * V.nrow;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(k / V.nrow);
+        const size_t row = k - (size_t)s * V.nrow;
+        const int sys = sysmap[s];
+        const double *xs = x + (size_t)sys * V.nrow;
+        double acc = 0.0, dg, rhs;
+        qd_row_visit(V, sys, row, [&](int kind, int i, int j, double a) { acc += a * xs[qd_index(V, kind, i, j)]; },
+                     dg, rhs);
+        R[k] = rhs - acc;
+    }
+}
+
+__global__ void nd_rhs(const double *R, const double *W, size_t nrow, int ns, int nIp, int nF, const int *rows,
+                       double *f)
+{
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    double *fb = f + ((size_t)n * ns + s) * nF;
+    for (int p = threadIdx.x; p < nIp; p += blockDim.x) {
+        const int r = rows[(size_t)n * nF + p];
+        fb[p] = r >= 0 ? W[(size_t)s * nrow + r] * R[(size_t)s * nrow + r] : 0.0;
+    }
+}
+
+__global__ void nd_pull_vec(const NDLevDev *lev, int L, int ns, const int *kids)
+{
+    const NDLevDev P = lev[L];
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
+    double *fp = P.f + ((size_t)n * ns + s) * P.nF;
+    for (int k = 0; k < 2; ++k) {
+        const int kid = kids[n * 2 + k];
+        if (kid < 0) continue;
+        const NDLevDev C = lev[kid >> 20];
+        const int ci = kid & 0xfffff;
+        const double *fc = C.f + ((size_t)ci * ns + s) * C.nF;
+        const int *up = C.up + (size_t)ci * C.nBp;
+        for (int i = threadIdx.x; i < C.nBp; i += blockDim.x) {
+            const int pi = up[i];
+            if (pi >= 0) fp[pi] += fc[C.nIp + i];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void nd_add(double *x, const double *D, const int *sysmap, int ns, size_t nrow)
+{
+    const size_t tot = (size_t)ns * nrow;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(k / nrow);
+        x[(size_t)sysmap[s] * nrow + (k - (size_t)s * nrow)] += D[k];
+    }
+}
+
 __global__ void nd_info_max(const rocblas_int *info, size_t n, int *out)
 {
     int m = 0;
@@ -174,6 +249,9 @@ struct NDPlan {
     double *F = nullptr, *f = nullptr;
     rocblas_int *ipiv = nullptr, *info = nullptr;
     int *sysmap = nullptr, *imax = nullptr;
+    int *ident = nullptr;           // 0, 1, ..., cap-1
+    double *R = nullptr, *D = nullptr;   // refinement residual and correction [cap][nrow]
+    double *W = nullptr;                 // row scales [cap][nrow]
     size_t ninfo = 0;
     rocblas_handle h = nullptr;
     double flops = 0.0;             // per system
@@ -371,6 +449,15 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
     P->ninfo = ni * P->cap;
     P->info = static_cast<rocblas_int *>(c.dalloc(P->ninfo * sizeof(rocblas_int)));
     P->sysmap = static_cast<int *>(c.dalloc((size_t)std::max(P->cap, 1) * sizeof(int)));
+    P->ident = static_cast<int *>(c.dalloc((size_t)std::max(P->cap, 1) * sizeof(int)));
+    {
+        std::vector<int> id(std::max(P->cap, 1));
+        for (size_t k = 0; k < id.size(); ++k) id[k] = (int)k;
+        TRT_CHECK(hipMemcpy(P->ident, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    P->R = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
+    P->D = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
+    P->W = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
     P->imax = static_cast<int *>(c.dalloc(sizeof(int)));
     P->levdev = static_cast<NDLevDev *>(c.dalloc((H + 1) * sizeof(NDLevDev)));
     ND_RB(rocblas_create_handle(&P->h));
@@ -403,13 +490,15 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
         TRT_CHECK(hipMemcpyAsync(P.levdev, tab.data(), tab.size() * sizeof(NDLevDev), hipMemcpyHostToDevice, c.stream));
         hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.fsz * ns)), dim3(256), 0, c.stream, P.F, P.fsz * ns);
         hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
+        hipLaunchKernelGGL(nd_rowscale, dim3(nblocks((size_t)ns * s.nrow)), dim3(256), 0, c.stream, V, P.sysmap, ns,
+                           P.W);
         size_t ioff = 0;
         for (int l = 0; l <= H; ++l) {
             const NDLevel &L = P.lev[l];
             double *F = tab[l].F, *f = tab[l].f;
             const int nb = L.nn * ns;
             hipLaunchKernelGGL(nd_assemble, dim3(nb), dim3(128), 0, c.stream, V, P.sysmap, ns, L.nIp, L.nF, L.rows,
-                               L.terms, F, f);
+                               L.terms, P.W, F, f);
             if (l > 0) hipLaunchKernelGGL(nd_pull, dim3(nb), dim3(256), 0, c.stream, P.levdev, l, ns, L.kids);
             TRT_CHECK(hipGetLastError());
             const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
@@ -452,6 +541,46 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
         TRT_CHECK(hipMemcpyAsync(&imax, P.imax, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         TRT_CHECK(hipStreamSynchronize(c.stream));
         if (imax > 0) throw Error("nested dissection: a front's pivot block is singular");
+        // three steps of iterative refinement with the stored factors: pivoting
+        // inside each front only can leave residuals well above rounding
+        for (int it = 0; it < 3; ++it) {
+            const size_t nv = (size_t)ns * s.nrow;
+            hipLaunchKernelGGL(nd_residual, dim3(nblocks(nv)), dim3(256), 0, c.stream, V, P.sysmap, ns, s.x, P.R);
+            hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
+            for (int l = 0; l <= H; ++l) {
+                const NDLevel &L = P.lev[l];
+                double *F = tab[l].F, *f = tab[l].f;
+                const int nb = L.nn * ns;
+                hipLaunchKernelGGL(nd_rhs, dim3(nb), dim3(128), 0, c.stream, P.R, P.W, s.nrow, ns, L.nIp, L.nF, L.rows,
+                                   f);
+                if (l > 0) hipLaunchKernelGGL(nd_pull_vec, dim3(nb), dim3(128), 0, c.stream, P.levdev, l, ns, L.kids);
+                const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
+                rocblas_int *piv = P.ipiv + L.poff * ns;
+                ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, 1, F, L.nF, sF, piv, sP, f,
+                                                       L.nF, sf, nb));
+                if (L.nBp > 0)
+                    ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nBp, 1,
+                                                        L.nIp, &mone, F + L.nIp, L.nF, sF, f, L.nF, sf, &one,
+                                                        f + L.nIp, L.nF, sf, nb));
+            }
+            for (int l = H; l >= 0; --l) {
+                const NDLevel &L = P.lev[l];
+                double *F = tab[l].F, *f = tab[l].f;
+                const int nb = L.nn * ns;
+                if (L.nBp > 0) {
+                    hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, P.D, s.nrow, P.ident, ns, L.nIp,
+                                       L.nBp, L.nF, L.rows, f);
+                    const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF;
+                    ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nIp, 1,
+                                                        L.nBp, &mone, F + (size_t)L.nIp * L.nF, L.nF, sF, f + L.nIp,
+                                                        L.nF, sf, &one, f, L.nF, sf, nb));
+                }
+                hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, P.D, s.nrow, P.ident, ns, L.nIp,
+                                   L.nF, L.rows, f);
+            }
+            hipLaunchKernelGGL(nd_add, dim3(nblocks(nv)), dim3(256), 0, c.stream, s.x, P.D, P.sysmap, ns, s.nrow);
+            TRT_CHECK(hipGetLastError());
+        }
     }
     if (c.verbose > 1)
         fprintf(stderr, "[qd_nd] %zu systems, %zu levels, %.2e flop per system\n", which.size(), P.lev.size(), P.flops);

@@ -36,6 +36,11 @@ def rel_l2(a, b) -> float:
 def build_problem(name: str, tight: bool = False):
     if name == "cfg0":
         p = problem.cfg0()
+    elif name == "cfg1":
+        p = problem.cfg1()
+    elif name in ("cfg1map1", "cfg1map0"):
+        # the 256x256, S16, 32-group problem with a lognormal(0, 1) map, or uniform material
+        p = problem.mapped(nx=256, order=16, ng=32, sigma=1.0 if name == "cfg1map1" else 0.0, nsteps=50)
     elif name.startswith("map"):
         # S8 and 16 groups keep the oracle's band solves to seconds per step
         p = problem.mapped(nx=int(name[3:]), order=8, ng=16, sigma=1.0)
@@ -95,14 +100,51 @@ def run(name: str, steps: int, threads: int = 8, tight: bool = False) -> dict:
             "worst": worst}
 
 
+def run_vs_fp64(name: str, steps: int, every: int) -> dict:
+    """Problems too large for the oracle: the FP32 forms against the FP64 GPU run
+    (which is itself pinned to the oracle on smaller problems), every `every` steps."""
+    prob = build_problem(name)
+    forms = {"fp64": native.Context(prob, precision=0), "fp32": native.Context(prob, precision=1),
+             "fp32s": native.Context(prob, precision=2)}
+    rows = []
+    try:
+        for s in range(1, steps + 1):
+            row = {"step": s}
+            for k, g in forms.items():
+                t0 = time.perf_counter()
+                g.step()
+                row[f"{k}_gpu_s"] = time.perf_counter() - t0
+            print(f"[precision] step {s} " + " ".join(f"{k} {row[k + '_gpu_s']:.2f}s" for k in forms),
+                  file=sys.stderr, flush=True)
+            if s % every == 0 or s == steps:
+                ref = {q: forms["fp64"].pull(arr) for q, arr in QUANTITIES.items()}
+                for k in ("fp32", "fp32s"):
+                    for q, arr in QUANTITIES.items():
+                        a = forms[k].pull(arr)
+                        row[f"{k}_{q}"] = rel_l2(a, ref[q])
+                        row[f"{k}_{q}_max"] = float(np.max(np.abs(a - ref[q])) / max(np.max(np.abs(ref[q])), 1e-300))
+                print(json.dumps(row), file=sys.stderr, flush=True)
+            rows.append(row)
+    finally:
+        for g in forms.values():
+            g.close()
+    return {"problem": name, "nx": prob.nx, "ny": prob.ny, "order": prob.order, "ng": prob.ng, "steps": steps,
+            "every": every, "reference": "fp64 GPU run", "rows": rows}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--problem", default="map48")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--tight", action="store_true", help="converge the iterations to 1e-11..1e-13")
+    ap.add_argument("--vs-fp64", type=int, default=0, metavar="EVERY",
+                    help="compare with the FP64 GPU run every EVERY steps instead of the oracle (large problems)")
     a = ap.parse_args(argv)
-    print(json.dumps(run(a.problem, a.steps, a.threads, a.tight)))
+    if a.vs_fp64:
+        print(json.dumps(run_vs_fp64(a.problem, a.steps, a.vs_fp64)))
+    else:
+        print(json.dumps(run(a.problem, a.steps, a.threads, a.tight)))
     return 0
 
 
