@@ -41,6 +41,11 @@ def build_problem(name: str, tight: bool = False):
     elif name in ("cfg1map1", "cfg1map0"):
         # the 256x256, S16, 32-group problem with a lognormal(0, 1) map, or uniform material
         p = problem.mapped(nx=256, order=16, ng=32, sigma=1.0 if name == "cfg1map1" else 0.0, nsteps=50)
+    elif name.startswith("s16g32n"):
+        # the 256x256 problem's physics (S16, 32 groups, lognormal(0, sigma) map) on an
+        # n x n grid: "s16g32n128" (sigma 1) or "s16g32n128u" (uniform material)
+        n = int(name[7:].rstrip("u"))
+        p = problem.mapped(nx=n, order=16, ng=32, sigma=0.0 if name.endswith("u") else 1.0, nsteps=50)
     elif name.startswith("map"):
         # S8 and 16 groups keep the oracle's band solves to seconds per step
         p = problem.mapped(nx=int(name[3:]), order=8, ng=16, sigma=1.0)
@@ -107,15 +112,27 @@ def run_vs_fp64(name: str, steps: int, every: int) -> dict:
     forms = {"fp64": native.Context(prob, precision=0), "fp32": native.Context(prob, precision=1),
              "fp32s": native.Context(prob, precision=2)}
     rows = []
+    failed = None
     try:
         for s in range(1, steps + 1):
             row = {"step": s}
             for k, g in forms.items():
                 t0 = time.perf_counter()
-                g.step()
+                try:
+                    g.step()
+                except native.TrtError as e:
+                    failed = {"step": s, "form": k, "error": str(e)}
+                    break
                 row[f"{k}_gpu_s"] = time.perf_counter() - t0
+            if failed:
+                print(f"[precision] step {s}: {failed['form']} failed: {failed['error']}", file=sys.stderr, flush=True)
+                break
             print(f"[precision] step {s} " + " ".join(f"{k} {row[k + '_gpu_s']:.2f}s" for k in forms),
                   file=sys.stderr, flush=True)
+            for k, g in forms.items():
+                # cells and groups whose radiation energy density is not positive (negative
+                # intensities of the SCB sweep) make the Eddington factors H/E unbounded
+                row[f"{k}_Eg_nonpos"] = int(np.count_nonzero(g.pull("HO_Eg_avg") <= 0.0))
             if s % every == 0 or s == steps:
                 ref = {q: forms["fp64"].pull(arr) for q, arr in QUANTITIES.items()}
                 for k in ("fp32", "fp32s"):
@@ -129,7 +146,7 @@ def run_vs_fp64(name: str, steps: int, every: int) -> dict:
         for g in forms.values():
             g.close()
     return {"problem": name, "nx": prob.nx, "ny": prob.ny, "order": prob.order, "ng": prob.ng, "steps": steps,
-            "every": every, "reference": "fp64 GPU run", "rows": rows}
+            "every": every, "reference": "fp64 GPU run", "failed": failed, "rows": rows}
 
 
 def main(argv=None):
