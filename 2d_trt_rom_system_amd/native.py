@@ -65,7 +65,8 @@ class RequestStats(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("ticks", C.c_longlong), ("updates", C.c_longlong), ("sweep_ms", C.c_double),
                 ("peak_bytes", C.c_size_t), ("peak_active", C.c_int), ("admit_s", C.c_double),
-                ("rebuild_s", C.c_double), ("step_s", C.c_double), ("retire_s", C.c_double)]
+                ("rebuild_s", C.c_double), ("step_s", C.c_double), ("retire_s", C.c_double),
+                ("setup_s", C.c_double)]
 
 
 def load(path: str = LIBPATH):

@@ -134,6 +134,7 @@ def run_load(rate: float, n: int, steps: int = 20, budget_gib: float = 200.0, pr
         "peak_active": summ["peak_active"], "peak_device_gib": summ["peak_bytes"] / 2**30,
         "host_s": {k: summ[k] for k in ("admit_s", "rebuild_s", "step_s", "retire_s")},
         "budget_gib": budget_gib,
+        "setup_s": summ["setup_s"],
     }
 
 

@@ -229,6 +229,9 @@ def main(argv=None):
             bt[key] = measure_batch(trt, probs, prec, max(2, args.steps // 2))
         bt["fp32_over_fp64"] = bt["fp32"]["updates_per_s"] / bt["fp64"]["updates_per_s"]
         out["batched"] = bt
+        if "fp64" in res:
+            # the batched FP32 sweep set against the plain FP64 form (one 256x256 problem per launch)
+            out["fp32_batched_over_plain_fp64"] = bt["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
     if not args.no_serve and not args.only:
         print(f"[bench] request loop, {args.serve_requests} requests at {args.serve_rate}/s", file=sys.stderr,
               flush=True)

@@ -136,7 +136,7 @@ int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *k
 int trt_calib_copy(size_t nbytes, int reps, double *ms);
 /* Accumulated sweep statistics since creation: sweeps run and total sweep ms. */
 int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
-/* The sweep configuration in use: sweep_mode (1, 2 or 3), tile shape, and the
+/* The sweep configuration in use: sweep_mode (1, 2, 3 or 4), tile shape, and the
  * persistent launch's grid (workgroups; 0 before the first persistent sweep). */
 int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, int *grid);
 
@@ -203,6 +203,9 @@ typedef struct {
     int peak_active;                    /* largest active set */
     double admit_s, rebuild_s, step_s, retire_s;   /* host seconds in admission (context creation),
                                                       batch rebuilds, steps and retirement */
+    double setup_s;                     /* seconds to reserve the arena of hbm_budget bytes (done
+                                           once, before the first run's clock starts; held until
+                                           trt_server_destroy) */
 } trt_server_stats;
 int trt_server_create(trt_server **out, size_t hbm_budget, int max_active);
 int trt_server_destroy(trt_server *s);
