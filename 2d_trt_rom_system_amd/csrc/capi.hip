@@ -354,12 +354,15 @@ int trt_tgen_step(trt_ctx *ctx)
     });
 }
 
-int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n)
+int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n) { return trt_batch_create_seg(out, ctxs, n, 0); }
+
+int trt_batch_create_seg(trt_batch **out, trt_ctx *const *ctxs, int n, int seg)
 {
     if (!out) return -1;
     *out = nullptr;
     try {
         if (!ctxs || n < 1) throw Error("a batch needs at least one context");
+        if (seg < 0) throw Error("segment length must be 0 (automatic) or positive");
         std::vector<Ctx *> v;
         for (int i = 0; i < n; ++i) {
             if (!ctxs[i]) throw Error("null context in batch");
@@ -367,7 +370,7 @@ int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n)
         }
         trt_batch *b = new trt_batch();
         try {
-            b->b = trt::pk_batch_create(v);
+            b->b = trt::pk_batch_create(v, false, seg);
         } catch (...) {
             delete b;
             throw;
@@ -455,5 +458,7 @@ int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, 
     trt::pk_batch_info(*b->b, updates_per_sweep, grid, nitems);
     return 0;
 }
+
+int trt_batch_segment_length(const trt_batch *b) { return (b && b->b) ? trt::pk_batch_seg(*b->b) : -1; }
 
 }  // extern "C"

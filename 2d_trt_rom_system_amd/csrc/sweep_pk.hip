@@ -916,6 +916,7 @@ struct PkBatch {
     std::vector<size_t> owned_bytes;
     bool pooled = false;            // device blocks, staging and stream from the process caches
     bool segs = false;              // column-segment items
+    int seg = 1;                    // column-segment length chosen for the batch
     std::vector<int> seg_of;        // tiles per segment of each problem
     const void *kern = nullptr;
     bool use_graph = false;
@@ -959,7 +960,7 @@ void pk_batch_destroy(PkBatch *b)
     delete b;
 }
 
-PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_req)
 {
     if (ctxs.empty()) throw Error("a batch needs at least one context");
     Ctx &c0 = *ctxs[0];
@@ -983,9 +984,10 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
         const int n = (int)ctxs.size();
         b->host.resize(n);
         b->up_iold.assign(n, -1);
-        // The batch picks its own item form (the results are bitwise the same): column
-        // segments of up to 4 tiles when the segment wavefronts of all members still
-        // offer twice the resident workgroups, else single tiles.
+        // The batch picks its own item form (the results are bitwise the same): the
+        // requested segment length, or by default column segments of up to 4 tiles when
+        // the segment wavefronts of all members still offer twice the resident
+        // workgroups, else single tiles.
         {
             const size_t ti0 = b->precision ? 4 : 8;
             int maxk0 = 0;
@@ -993,7 +995,8 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
             const size_t lds0 = (size_t)maxk0 * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti0;
             const long long g2 = 2LL * resident_grid(b->k->seg, lds0, 1 << 30);
             int seg = 1;
-            if (const char *v = getenv("TRT_SEG_LEN")) seg = std::max(1, atoi(v));
+            if (seg_req > 0) seg = seg_req;
+            else if (const char *v = getenv("TRT_SEG_LEN")) seg = std::max(1, atoi(v));
             else
                 for (int s = 4; s >= 2 && seg == 1; s /= 2) {
                     long long par = 0;
@@ -1002,6 +1005,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled)
                     if (par >= g2) seg = s;
                 }
             b->segs = seg > 1;
+            b->seg = seg;
             b->seg_of.resize(n);
             for (int i = 0; i < n; ++i) b->seg_of[i] = std::min(seg, ctxs[i]->sg.NTY);
         }
@@ -1150,6 +1154,8 @@ double pk_batch_elapsed_ms(PkBatch &b)
     TRT_CHECK(hipEventElapsedTime(&ms, b.t0, b.t1));
     return ms;
 }
+
+int pk_batch_seg(const PkBatch &b) { return b.seg; }
 
 void pk_batch_info(const PkBatch &b, long long *updates, int *grid, int *nitems)
 {

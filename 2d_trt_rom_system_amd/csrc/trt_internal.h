@@ -177,7 +177,9 @@ void pk_destroy(Ctx &c);
 int pk_grid(const Ctx &c);
 // batched persistent sweep over many contexts (one precision and tile shape)
 struct PkBatch;
-PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled = false);
+// seg: column-segment length of the work items (0 = chosen from the parallelism)
+PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled = false, int seg = 0);
+int pk_batch_seg(const PkBatch &b);
 void pk_batch_destroy(PkBatch *b);
 void pk_batch_sweep(PkBatch &b, int mark);
 void pk_batch_use_graph(PkBatch &b, bool on);

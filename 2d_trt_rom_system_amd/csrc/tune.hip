@@ -1,6 +1,7 @@
-// Tile-shape autotuner (no counterpart in the reference): times the batched
-// persistent sweep of a set of problems with every tile shape the tile engine
-// offers for the precision and returns the fastest.
+// Autotuner (no counterpart in the reference): times the batched persistent
+// sweep of a set of problems with every tile shape the tile engine offers for
+// the precision -- and, for trt_autotune_batch, every column-segment length --
+// and returns the fastest.
 #include "trt_internal.h"
 
 #include <limits>
@@ -16,12 +17,17 @@ struct Shape {
 const Shape kFp32[] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}, {8, 2}, {2, 8}};
 const Shape kFp64[] = {{2, 2}, {2, 4}, {4, 2}};
 
-// ms per batched sweep of the problems with one tile shape
-double time_shape(const trt_problem *probs, int n, int precision, Shape sh, int sweeps)
+// the segment lengths trt_autotune_batch tries
+const int kSegs[] = {1, 2, 4, 8};
+
+// ms per batched sweep of the problems with one tile shape, for each of the
+// nseg segment lengths segs[] (0 = the batch's own choice); contexts are
+// created once per shape
+void time_shape(const trt_problem *probs, int n, int precision, Shape sh, int sweeps, const int *segs, int nseg,
+                double *ms)
 {
     std::vector<trt_ctx *> ctxs;
     trt::PkBatch *b = nullptr;
-    double ms = 0.0;
     try {
         std::vector<trt::Ctx *> cs;
         for (int i = 0; i < n; ++i) {
@@ -35,20 +41,28 @@ double time_shape(const trt_problem *probs, int n, int precision, Shape sh, int 
             ctxs.push_back(c);
             cs.push_back(&c->c);
         }
-        b = trt::pk_batch_create(cs);
-        trt::pk_batch_sweep(*b, 3);                 // warm-up
-        trt::pk_batch_check(*b);
-        for (int k = 0; k < sweeps; ++k) trt::pk_batch_sweep(*b, (k == 0 ? 1 : 0) | (k == sweeps - 1 ? 2 : 0));
-        ms = trt::pk_batch_elapsed_ms(*b) / sweeps;
-        trt::pk_batch_check(*b);
+        for (int s = 0; s < nseg; ++s) {
+            b = trt::pk_batch_create(cs, false, segs[s]);
+            trt::pk_batch_sweep(*b, 3);                 // warm-up
+            trt::pk_batch_check(*b);
+            for (int k = 0; k < sweeps; ++k) trt::pk_batch_sweep(*b, (k == 0 ? 1 : 0) | (k == sweeps - 1 ? 2 : 0));
+            ms[s] = trt::pk_batch_elapsed_ms(*b) / sweeps;
+            trt::pk_batch_check(*b);
+            trt::pk_batch_destroy(b);
+            b = nullptr;
+        }
     } catch (...) {
         trt::pk_batch_destroy(b);
         for (trt_ctx *c : ctxs) trt_ctx_destroy(c);
         throw;
     }
-    trt::pk_batch_destroy(b);
     for (trt_ctx *c : ctxs) trt_ctx_destroy(c);
-    return ms;
+}
+
+int shapes_of(int precision, const Shape **shapes)
+{
+    *shapes = precision == 1 ? kFp32 : kFp64;
+    return precision == 1 ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
 }
 
 thread_local std::string g_tune_error;
@@ -62,11 +76,13 @@ int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps
 {
     if (!probs || n < 1 || !tile_x || !tile_y || precision < 0 || precision > 2) return -1;
     try {
-        const Shape *shapes = precision == 1 ? kFp32 : kFp64;
-        const int ns = precision == 1 ? (int)(sizeof(kFp32) / sizeof(Shape)) : (int)(sizeof(kFp64) / sizeof(Shape));
+        const Shape *shapes = nullptr;
+        const int ns = shapes_of(precision, &shapes);
         double best = std::numeric_limits<double>::infinity();
+        const int automatic = 0;
         for (int s = 0; s < ns; ++s) {
-            const double ms = time_shape(probs, n, precision, shapes[s], sweeps > 0 ? sweeps : 3);
+            double ms = 0.0;
+            time_shape(probs, n, precision, shapes[s], sweeps > 0 ? sweeps : 3, &automatic, 1, &ms);
             if (all_ms) all_ms[s] = ms;
             if (ms < best) {
                 best = ms;
@@ -82,6 +98,46 @@ int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps
         g_tune_error = e.what();
     }
     return -2;
+}
+
+int trt_autotune_batch(const trt_problem *probs, int n, int precision, int sweeps, int *tile_x, int *tile_y,
+                       int *seg, double *best_ms, double *all_ms)
+{
+    if (!probs || n < 1 || !tile_x || !tile_y || !seg || precision < 0 || precision > 2) return -1;
+    try {
+        const Shape *shapes = nullptr;
+        const int ns = shapes_of(precision, &shapes);
+        const int nseg = (int)(sizeof(kSegs) / sizeof(int));
+        double best = std::numeric_limits<double>::infinity();
+        for (int s = 0; s < ns; ++s) {
+            double ms[sizeof(kSegs) / sizeof(int)];
+            time_shape(probs, n, precision, shapes[s], sweeps > 0 ? sweeps : 3, kSegs, nseg, ms);
+            for (int q = 0; q < nseg; ++q) {
+                if (all_ms) all_ms[s * nseg + q] = ms[q];
+                if (ms[q] < best) {
+                    best = ms[q];
+                    *tile_x = shapes[s].tx;
+                    *tile_y = shapes[s].ty;
+                    *seg = kSegs[q];
+                }
+            }
+        }
+        if (best_ms) *best_ms = best;
+        return 0;
+    } catch (const trt::Error &e) {
+        g_tune_error = e.msg;
+    } catch (const std::exception &e) {
+        g_tune_error = e.what();
+    }
+    return -2;
+}
+
+int trt_autotune_segments(int *seg, int cap)
+{
+    const int nseg = (int)(sizeof(kSegs) / sizeof(int));
+    for (int q = 0; q < nseg && q < cap; ++q)
+        if (seg) seg[q] = kSegs[q];
+    return nseg;
 }
 
 int trt_autotune_shapes(int precision, int *tile_x, int *tile_y, int cap)

@@ -45,6 +45,7 @@ EXPORTS = [
     "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
     "trt_sweep_config", "trt_ctx_device_bytes", "trt_tgen_step", "trt_batch_create", "trt_batch_destroy",
     "trt_batch_last_error", "trt_batch_sweep_timed", "trt_batch_tgen_step", "trt_batch_use_graph", "trt_batch_info",
+    "trt_batch_create_seg", "trt_batch_segment_length", "trt_autotune_batch", "trt_autotune_segments",
     "trt_server_create", "trt_server_destroy", "trt_server_last_error", "trt_server_submit", "trt_server_run",
     "trt_server_request", "trt_server_request_error", "trt_server_summary", "trt_server_output_size",
     "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
@@ -110,6 +111,8 @@ def load(path: str = LIBPATH):
     L.trt_ctx_device_bytes.restype = C.c_size_t
     L.trt_tgen_step.argtypes = [vp]
     L.trt_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
+    L.trt_batch_create_seg.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int]
+    L.trt_batch_segment_length.argtypes = [vp]
     L.trt_batch_destroy.argtypes = [vp]
     L.trt_batch_last_error.argtypes = [vp]
     L.trt_batch_last_error.restype = C.c_char_p
@@ -132,6 +135,8 @@ def load(path: str = LIBPATH):
     L.trt_server_output.argtypes = [vp, C.c_int, C.c_char_p, _dp, C.c_size_t]
     L.trt_autotune_tile.argtypes = [C.POINTER(TrtProblem), C.c_int, C.c_int, C.c_int, ip, ip, _dp, _dp]
     L.trt_autotune_shapes.argtypes = [C.c_int, ip, ip, C.c_int]
+    L.trt_autotune_batch.argtypes = [C.POINTER(TrtProblem), C.c_int, C.c_int, C.c_int, ip, ip, ip, _dp, _dp]
+    L.trt_autotune_segments.argtypes = [ip, C.c_int]
     L.trt_autotune_last_error.restype = C.c_char_p
     _lib = L
     return L
@@ -305,12 +310,13 @@ class Batch:
     """Contexts swept together in one persistent launch (a trt_batch).  The
     contexts must stay open while the batch exists."""
 
-    def __init__(self, ctxs, graph: bool = False):
+    def __init__(self, ctxs, graph: bool = False, seg: int = 0):
+        """seg: column-segment length of the work items (0 = chosen from the parallelism)."""
         self.L = load()
         self.ctxs = list(ctxs)
         arr = (C.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
         h = C.c_void_p()
-        rc = self.L.trt_batch_create(C.byref(h), arr, len(self.ctxs))
+        rc = self.L.trt_batch_create_seg(C.byref(h), arr, len(self.ctxs), int(seg))
         if rc != 0:
             raise TrtError(f"trt_batch_create failed ({rc}): {self.L.trt_batch_last_error(None).decode()}")
         self.h = h
@@ -330,6 +336,10 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+    @property
+    def segment_length(self) -> int:
+        return int(self.L.trt_batch_segment_length(self.h))
 
     def _check(self, rc, what):
         if rc != 0:
@@ -378,3 +388,33 @@ def autotune_tile(probs, precision: int = 1, sweeps: int = 3):
     if rc != 0:
         raise TrtError(f"trt_autotune_tile failed ({rc}): {L.trt_autotune_last_error().decode()}")
     return (tx.value, ty.value), best.value, dict(zip(shapes, list(all_ms)))
+
+
+def autotune_segments():
+    """Column-segment lengths autotune_batch tries."""
+    L = load()
+    n = L.trt_autotune_segments(None, 0)
+    seg = (C.c_int * n)()
+    L.trt_autotune_segments(seg, n)
+    return list(seg)
+
+
+def autotune_batch(probs, precision: int = 1, sweeps: int = 3):
+    """Fastest (tile shape, segment length) for a batched sweep of probs:
+    (tile, seg, best ms, {(tile, seg): ms})."""
+    L = load()
+    structs, keep = [], []
+    for p in probs:
+        s, k = make_problem_struct(p, precision)
+        structs.append(s)
+        keep.append(k)
+    arr = (TrtProblem * len(structs))(*structs)
+    shapes, segs = autotune_shapes(precision), autotune_segments()
+    tx, ty, seg, best = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    all_ms = (C.c_double * (len(shapes) * len(segs)))()
+    rc = L.trt_autotune_batch(arr, len(structs), int(precision), int(sweeps), C.byref(tx), C.byref(ty), C.byref(seg),
+                              C.byref(best), all_ms)
+    if rc != 0:
+        raise TrtError(f"trt_autotune_batch failed ({rc}): {L.trt_autotune_last_error().decode()}")
+    table = {(sh, sg): all_ms[i * len(segs) + q] for i, sh in enumerate(shapes) for q, sg in enumerate(segs)}
+    return (tx.value, ty.value), seg.value, best.value, table

@@ -97,8 +97,9 @@ def batch_mix(trt, budget_gib: float, base_seed: int = 20240611):
 
 
 def measure_batch(trt, probs, precision: int, steps: int):
-    """Batched persistent sweep of probs vs the same problems swept one at a time."""
-    tile = (4, 2) if precision else (2, 4)
+    """Batched persistent sweep of probs (tile shape and column-segment length chosen by
+    the autotuner on these problems) vs the same problems swept one at a time."""
+    tile, seg, _, table = trt.native.autotune_batch(probs, precision, sweeps=3)
     ctxs = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]
     try:
         upd = sum(p.updates_per_sweep() for p in probs)
@@ -106,16 +107,18 @@ def measure_batch(trt, probs, precision: int, steps: int):
         for c in ctxs:
             c.sweep_timed(1)
             alone_ms += c.sweep_timed(2) / 2
-        with trt.native.Batch(ctxs, graph=True) as b:
+        with trt.native.Batch(ctxs, graph=True, seg=seg) as b:
             b.sweep_timed(1)
             ms = b.sweep_timed(steps) / steps
             _, grid, nitems = b.info()
+            seg = b.segment_length
     finally:
         for c in ctxs:
             c.close()
     return {"updates_per_sweep": upd, "ms_per_sweep": ms, "updates_per_s": upd / (ms * 1e-3),
-            "one_at_a_time_updates_per_s": upd / (alone_ms * 1e-3), "tile": list(tile), "grid": grid,
-            "work_items": nitems}
+            "one_at_a_time_updates_per_s": upd / (alone_ms * 1e-3), "tile": list(tile), "segment_length": seg,
+            "grid": grid, "work_items": nitems,
+            "autotune_ms_per_sweep": {f"{t[0]}x{t[1]}/seg{sg}": round(v, 3) for (t, sg), v in table.items()}}
 
 
 def cpu_baseline(trt, budget_s: float = 12.0):
@@ -224,7 +227,8 @@ def main(argv=None):
         bt = {"problems": len(probs),
               "mix": [f"{p.nx}x{p.ny} S{p.order} G{p.ng}" for p in probs],
               "note": "problems of the batched distribution (grid 64..512, 16..64 groups, S8..S16) in draw order "
-                      f"that fit {args.batch_gib:g} GiB in FP64; one persistent launch per sweep, HIP-graph replay"}
+                      f"that fit {args.batch_gib:g} GiB in FP64; one persistent launch per sweep, HIP-graph replay; "
+                      "tile shape and column-segment length from the autotuner (trt_autotune_batch)"}
         for prec, key in ((1, "fp32"), (0, "fp64")):
             bt[key] = measure_batch(trt, probs, prec, max(2, args.steps // 2))
         bt["fp32_over_fp64"] = bt["fp32"]["updates_per_s"] / bt["fp64"]["updates_per_s"]

@@ -154,6 +154,13 @@ int trt_tgen_step(trt_ctx *ctx);
  * stream, and later work on a member's stream waits for it. */
 typedef struct trt_batch trt_batch;
 int trt_batch_create(trt_batch **out, trt_ctx *const *ctxs, int n);
+/* As trt_batch_create with the column-segment length of the work items fixed:
+ * seg tiles of one tile column per item, their H faces carried in registers
+ * (1 = single tiles; 0 = chosen from the available parallelism, as
+ * trt_batch_create does).  Every length gives bitwise the same results. */
+int trt_batch_create_seg(trt_batch **out, trt_ctx *const *ctxs, int n, int seg);
+/* the column-segment length the batch uses (-1 for a null batch) */
+int trt_batch_segment_length(const trt_batch *b);
 int trt_batch_destroy(trt_batch *b);
 const char *trt_batch_last_error(const trt_batch *b);
 /* nsweeps batched TRANSPORT_SCB sweeps (each as trt_transport_scb on every
@@ -175,6 +182,13 @@ int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, 
 int trt_autotune_tile(const trt_problem *probs, int n, int precision, int sweeps, int *tile_x, int *tile_y,
                       double *best_ms, double *all_ms);
 int trt_autotune_shapes(int precision, int *tile_x, int *tile_y, int cap);
+/* As trt_autotune_tile, jointly over tile shape and the column-segment length
+ * of trt_batch_create_seg (trt_autotune_segments lists the lengths tried; their
+ * count is the return value).  all_ms (may be NULL) receives the ms per sweep
+ * of shape s with length q at [s * nsegments + q]. */
+int trt_autotune_batch(const trt_problem *probs, int n, int precision, int sweeps, int *tile_x, int *tile_y,
+                       int *seg, double *best_ms, double *all_ms);
+int trt_autotune_segments(int *seg, int cap);
 const char *trt_autotune_last_error(void);
 
 /* ---- Request loop -----------------------------------------------------------

@@ -35,11 +35,14 @@ def _randomize(ctx, prob, seed):
     ctx.call("material_update")
 
 
-@pytest.mark.parametrize("precision,tile,seg", [(1, (4, 2), None), (0, (2, 4), None), (1, (2, 2), None),
-                                               (1, (4, 2), "2"), (0, (2, 4), "3"), (2, (2, 2), "4")])
-def test_batch_sweep_equals_single(trt, monkeypatch, precision, tile, seg):
-    """seg forces the batch's column-segment length (TRT_SEG_LEN); by default the
-    batch picks tiles or segments from the available parallelism."""
+@pytest.mark.parametrize("precision,tile,seg,arg", [(1, (4, 2), None, 0), (0, (2, 4), None, 0),
+                                                   (1, (2, 2), None, 0), (1, (4, 2), "2", 0), (0, (2, 4), "3", 0),
+                                                   (2, (2, 2), "4", 0), (1, (2, 8), None, 8), (0, (4, 2), None, 1),
+                                                   (1, (4, 4), "1", 3)])
+def test_batch_sweep_equals_single(trt, monkeypatch, precision, tile, seg, arg):
+    """seg forces the batch's column-segment length through TRT_SEG_LEN, arg through
+    trt_batch_create_seg (which wins over the environment); by default the batch
+    picks tiles or segments from the available parallelism."""
     if seg:
         monkeypatch.setenv("TRT_SEG_LEN", seg)
     probs = _probs(trt)
@@ -50,10 +53,14 @@ def test_batch_sweep_equals_single(trt, monkeypatch, precision, tile, seg):
         _randomize(mixed[k], p, k)
     for c in alone:
         c.sweep_timed(2)
-    with trt.native.Batch(mixed) as b:
+    with trt.native.Batch(mixed, seg=arg) as b:
         upd, grid, nitems = b.info()
         assert upd == sum(p.updates_per_sweep() for p in probs)
         assert grid >= 1 and nitems > 0
+        if arg:
+            assert b.segment_length == arg
+        elif seg:
+            assert b.segment_length == int(seg)
         b.sweep_timed(2)
     for a, m in zip(alone, mixed):
         assert np.array_equal(a.pull("I_crn"), m.pull("I_crn"))

@@ -98,3 +98,13 @@ def test_autotuner_returns_a_listed_shape(trt):
         tile, best, table = trt.native.autotune_tile(probs, precision=prec, sweeps=2)
         assert tile in shapes and set(table) == set(shapes)
         assert best == min(table.values()) and best > 0
+
+
+def test_batch_autotuner_covers_shapes_and_segments(trt):
+    probs = _mix(trt)[:2]
+    shapes, segs = trt.native.autotune_shapes(1), trt.native.autotune_segments()
+    assert 1 in segs and len(segs) >= 3
+    tile, seg, best, table = trt.native.autotune_batch(probs, precision=1, sweeps=2)
+    assert tile in shapes and seg in segs
+    assert set(table) == {(sh, sg) for sh in shapes for sg in segs}
+    assert best == min(table.values()) == table[(tile, seg)] and best > 0
