@@ -104,6 +104,7 @@ def measure_batch(trt, probs, precision: int, steps: int):
     try:
         upd = sum(p.updates_per_sweep() for p in probs)
         alone_ms = 0.0
+        balg = sum(alg_bytes(p, precision) for p in probs)
         for c in ctxs:
             c.sweep_timed(1)
             alone_ms += c.sweep_timed(2) / 2
@@ -117,7 +118,7 @@ def measure_batch(trt, probs, precision: int, steps: int):
             c.close()
     return {"updates_per_sweep": upd, "ms_per_sweep": ms, "updates_per_s": upd / (ms * 1e-3),
             "one_at_a_time_updates_per_s": upd / (alone_ms * 1e-3), "tile": list(tile), "segment_length": seg,
-            "grid": grid, "work_items": nitems,
+            "grid": grid, "work_items": nitems, "achieved_alg_gbs": balg / (ms * 1e-3) / 1e9,
             "autotune_ms_per_sweep": {f"{t[0]}x{t[1]}/seg{sg}": round(v, 3) for (t, sg), v in table.items()}}
 
 
@@ -215,6 +216,11 @@ def main(argv=None):
             "avg_launch_ms": head["launch_avg_ms"],
         },
     }
+    tr = out["roofline"]["traffic"]
+    if tr:
+        # HBM bytes actually moved per launch (calibrated counters) over the live launch time
+        out["roofline"]["traffic_gbs"] = tr / (head["launch_avg_ms"] * 1e-3) / 1e9
+        out["roofline"]["traffic_frac"] = out["roofline"]["traffic_gbs"] / HBM_PEAK_GBS
     for k, v in res.items():
         out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
     if "fp32" in res and "fp64" in res:
