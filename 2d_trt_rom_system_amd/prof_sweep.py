@@ -33,6 +33,22 @@ def run(precision: int, sweeps: int, copy_mib: int, tile=(0, 0), mode: int = 0) 
     ctx.close()
 
 
+def run_steps(precision: int, steps: int) -> None:
+    """Full MLQD time steps of the 256x256, S16, 32-group problem with uniform
+    material (cfg1's multiplier map drives the grey Newton iteration to NaN):
+    one warm-up step, then `steps` timed ones."""
+    import time
+    trt = importlib.import_module("2d_trt_rom_system_amd")
+    prob = trt.problem.mapped(nx=256, order=16, ng=32, sigma=0.0, nsteps=steps + 1)
+    ctx = trt.native.Context(prob, precision=precision)
+    ctx.step()
+    for k in range(steps):
+        t0 = time.perf_counter()
+        its = ctx.step()
+        print(f"step {k + 2}: {time.perf_counter() - t0:.3f} s, iterations {its}", flush=True)
+    ctx.close()
+
+
 def _per_kernel(path: str, counter: str):
     vals = {}
     for r in csv.DictReader(open(path)):
@@ -66,10 +82,13 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--sweeps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=0, help="full time steps instead of sweeps (uniform material)")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--summarize", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL"))
     a = ap.parse_args()
     if a.summarize:
         print(json.dumps(summarize(a.summarize[0], a.summarize[1], a.copy_mib, a.summarize[2]), indent=1))
+    elif a.steps:
+        run_steps(a.precision, a.steps)
     else:
         run(a.precision, a.sweeps, a.copy_mib)
