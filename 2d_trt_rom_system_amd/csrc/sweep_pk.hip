@@ -367,6 +367,19 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     }
 }
 
+// v of the lane whose index differs in bit 0 (XOR 1) or bit 1 (XOR 2): a DPP quad
+// permute of both halves, done in the VALU without a round trip through the LDS
+// unit (as ds_bpermute, what __shfl_xor compiles to); same partner, same value.
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_XOR1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;   // quad_perm [2,3,0,1]
+
 // Sum the staged corner averages of a tile over the directions of each group and
 // write the quadrant partial moments of its cells and downwind faces.
 template <typename TS, int TX, int TY, bool px, bool py>
@@ -403,9 +416,18 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const
                 m[12] += cc[0] * ih;
             }
         }
-        for (int off = 1; off < S; off <<= 1) {
+        // butterfly over the S lanes of a pair (every lane of the block is active here)
+        if (S > 1) {
 #pragma unroll
-            for (int v = 0; v < NMOM; ++v) m[v] += __shfl_xor(m[v], off);
+            for (int v = 0; v < NMOM; ++v) m[v] += quad_perm<DPP_XOR1>(m[v]);
+        }
+        if (S > 2) {
+#pragma unroll
+            for (int v = 0; v < NMOM; ++v) m[v] += quad_perm<DPP_XOR2>(m[v]);
+        }
+        if (S > 4) {
+#pragma unroll
+            for (int v = 0; v < NMOM; ++v) m[v] += __shfl_xor(m[v], 4);
         }
         if (ok) {
             const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
