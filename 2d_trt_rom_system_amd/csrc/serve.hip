@@ -178,12 +178,13 @@ void run(trt::Server &s)
     }
     const auto t0 = clock::now();
     auto now = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
-    std::vector<size_t> order(s.reqs.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    // the waiting requests in arrival order (finished or failed ones of earlier runs stay out)
+    std::vector<size_t> order;
+    for (size_t i = 0; i < s.reqs.size(); ++i)
+        if (s.reqs[i].status == 1) order.push_back(i);
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return s.reqs[a].arrival < s.reqs[b].arrival; });
     size_t next = 0;                       // first request in arrival order not yet admitted
-    while (next < order.size() && s.reqs[order[next]].status != 1) ++next;
     s.ticks = 0; s.updates = 0; s.peak_bytes = 0; s.peak_active = 0; s.sweep_ms = 0.0;
     s.t_admit = s.t_rebuild = s.t_step = s.t_retire = 0.0;
     std::vector<size_t> active, draining;

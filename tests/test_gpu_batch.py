@@ -178,3 +178,14 @@ def test_fused_collapse_equals_separate_routines(trt):
         assert np.array_equal(a.pull(n), b.pull(n)), n
     a.close()
     b.close()
+
+
+def test_batch_rejects_negative_segment_length(trt):
+    probs = _probs(trt)[:2]
+    ctxs = [trt.native.Context(p, precision=1, tile=(4, 2)) for p in probs]
+    try:
+        with pytest.raises(trt.native.TrtError, match="segment length"):
+            trt.native.Batch(ctxs, seg=-1)
+    finally:
+        for c in ctxs:
+            c.close()

@@ -47,6 +47,29 @@ def test_request_same_alone_and_in_mix(trt):
         assert srv.summary()["peak_active"] >= 2
 
 
+def test_server_runs_twice_on_one_arena(trt):
+    """A second run serves only what was submitted after the first (a finished
+    request that sorts after a new one by arrival time is not served again), on
+    the arena the first run reserved; the same problem gives the same results."""
+    probs = _mix(trt)
+    with trt.serve.Server(8 << 30) as srv:
+        a = srv.submit(probs[0], steps=3, arrival_s=0.05)
+        srv.run()
+        s1 = srv.summary()
+        ra = srv.request(a)
+        assert ra["status"] == 0 and s1["ticks"] == 3
+        b = srv.submit(probs[0], steps=3, arrival_s=0.0)
+        c = srv.submit(probs[1], steps=2, arrival_s=0.0)
+        srv.run()
+        s2 = srv.summary()
+        assert srv.request(b)["status"] == 0 and srv.request(c)["status"] == 0
+        assert srv.request(a) == ra                 # untouched by the second run
+        assert s2["ticks"] == 3 and s2["updates"] == 3 * probs[0].updates_per_sweep() + 2 * probs[1].updates_per_sweep()
+        assert s1["setup_s"] >= 0.0 and s2["setup_s"] == s1["setup_s"]
+        for n in OUTS:
+            assert np.array_equal(srv.output(a, n), srv.output(b, n)), n
+
+
 def test_admission_honours_budget_and_cap(trt):
     p = trt.problem.small(nx=96, ny=96, order=8, ng=16, seed=41)
     need = trt.native.device_bytes(p, precision=1, tile=(4, 2))
