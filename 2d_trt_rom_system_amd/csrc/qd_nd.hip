@@ -220,6 +220,159 @@ __global__ void nd_add(double *x, const double *D, const int *sysmap, int ns, si
     }
 }
 
+// Fronts of at most ND_SF unknowns with at most ND_SI pivot rows, one wavefront
+// each (lane t holds row t of the front in registers): partial pivoting among
+// the pivot rows, elimination of the pivot columns from every row (pivot and
+// border rows, and the right-hand side f), then back substitution of the border
+// columns and f.  The front is left as getrf + getrs + gemm leave it: LU and the
+// getrf pivots in the pivot block (rows in pivot order), A_II^-1 F_IB and
+// A_II^-1 f_I, F_BI unchanged, the Schur complement F_BB - F_BI A_II^-1 F_IB and
+// f_B - F_BI A_II^-1 f_I; info = first zero pivot (1-based) as getrf reports.
+constexpr int ND_SF = 64, ND_SI = 48;
+
+struct NDSmall {
+    double r[ND_SF];       // this lane's row of the front
+    double rf;             // its right-hand-side entry
+    double *prow;          // LDS: the current pivot row, f entry last
+    int *s_orig;           // LDS: front row chosen at each pivot step
+    int t, nF, nIp, pos, zero_at;
+    bool live, pivot_row, chosen;
+};
+
+// pivot step K (the template argument keeps every register index constant)
+template <int K>
+__device__ __forceinline__ void nd_small_elim(NDSmall &w)
+{
+    if constexpr (K < ND_SI) {
+        if (K < w.nIp) {
+            // the largest |entry| of column K among the pivot rows not chosen yet
+            double key = (w.pivot_row && !w.chosen) ? fabs(w.r[K]) : -1.0;
+            int who = w.t;
+            for (int o = 1; o < 64; o <<= 1) {
+                const double ok = __shfl_xor(key, o);
+                const int ow = __shfl_xor(who, o);
+                if (ok > key || (ok == key && ow < who)) {
+                    key = ok;
+                    who = ow;
+                }
+            }
+            const int p = __builtin_amdgcn_readfirstlane(who);
+            if (w.t == p) {
+                w.chosen = true;
+                w.pos = K;
+                w.s_orig[K] = w.t;
+#pragma unroll
+                for (int j = K; j < ND_SF; ++j)
+                    if (j < w.nF) w.prow[j] = w.r[j];
+                w.prow[ND_SF] = w.rf;
+            }
+            __syncthreads();
+            const double piv = w.prow[K];
+            if (piv == 0.0 && w.zero_at == 0) w.zero_at = K + 1;
+            if (w.live && !w.chosen) {
+                const double l = piv != 0.0 ? w.r[K] * (1.0 / piv) : 0.0;
+                w.r[K] = l;
+#pragma unroll
+                for (int j = K + 1; j < ND_SF; ++j)
+                    if (j < w.nF) w.r[j] = fma(-l, w.prow[j], w.r[j]);
+                w.rf = fma(-l, w.prow[ND_SF], w.rf);
+            }
+            __syncthreads();
+            nd_small_elim<K + 1>(w);
+        }
+    }
+}
+
+// back substitution step K (last pivot first): x_K = z_K / U_KK, then the
+// pivot rows chosen before K remove U_{row,K} x_K
+template <int K>
+__device__ __forceinline__ void nd_small_back(NDSmall &w)
+{
+    if constexpr (K >= 0) {
+        if (K < w.nIp) {
+            if (w.pos == K) {
+                const double inv = w.r[K] != 0.0 ? 1.0 / w.r[K] : 0.0;
+#pragma unroll
+                for (int j = 0; j < ND_SF; ++j)
+                    if (j >= w.nIp && j < w.nF) {
+                        w.r[j] *= inv;
+                        w.prow[j] = w.r[j];
+                    }
+                w.rf *= inv;
+                w.prow[ND_SF] = w.rf;
+            }
+            __syncthreads();
+            if (w.pivot_row && w.pos < K) {
+                const double u = w.r[K];
+#pragma unroll
+                for (int j = 0; j < ND_SF; ++j)
+                    if (j >= w.nIp && j < w.nF) w.r[j] = fma(-u, w.prow[j], w.r[j]);
+                w.rf = fma(-u, w.prow[ND_SF], w.rf);
+            }
+            __syncthreads();
+        }
+        nd_small_back<K - 1>(w);
+    }
+}
+
+__global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int nF, int nIp, size_t sF, size_t sf,
+                                                     rocblas_int *ipiv, rocblas_int *info, int nb)
+{
+    __shared__ double prow[ND_SF + 1];
+    __shared__ int s_orig[ND_SI];
+    __shared__ int s_arr[ND_SI], s_where[ND_SI];
+    const size_t fr = blockIdx.x;
+    if (fr >= (size_t)nb) return;
+    double *A = F + fr * sF;
+    double *v = f + fr * sf;
+    NDSmall w;
+    w.t = threadIdx.x;
+    w.nF = nF;
+    w.nIp = nIp;
+    w.prow = prow;
+    w.s_orig = s_orig;
+    w.live = w.t < nF;
+    w.pivot_row = w.t < nIp;
+    w.chosen = false;
+    w.pos = -1;
+    w.zero_at = 0;
+#pragma unroll
+    for (int j = 0; j < ND_SF; ++j) w.r[j] = (w.live && j < nF) ? A[(size_t)j * nF + w.t] : 0.0;
+    w.rf = w.live ? v[w.t] : 0.0;
+    nd_small_elim<0>(w);
+    nd_small_back<ND_SI - 1>(w);
+    if (w.pivot_row) {
+        // LU row and A_II^-1 F_IB row at the row's pivot position
+#pragma unroll
+        for (int j = 0; j < ND_SF; ++j)
+            if (j < nF) A[(size_t)j * nF + w.pos] = w.r[j];
+        v[w.pos] = w.rf;
+    } else if (w.live) {
+        // border row: F_BI stays, Schur complement and reduced f_B
+#pragma unroll
+        for (int j = 0; j < ND_SF; ++j)
+            if (j >= nIp && j < nF) A[(size_t)j * nF + w.t] = w.r[j];
+        v[w.t] = w.rf;
+    }
+    // getrf's pivot sequence: at step k rows k and ipiv[k]-1 of the current order swap
+    if (w.t == 0) {
+        rocblas_int *pv = ipiv + fr * nIp;
+        for (int i = 0; i < nIp; ++i) {
+            s_arr[i] = i;
+            s_where[i] = i;
+        }
+        for (int k = 0; k < nIp; ++k) {
+            const int o = s_orig[k], pp = s_where[o], ak = s_arr[k];
+            pv[k] = pp + 1;
+            s_arr[pp] = ak;
+            s_where[ak] = pp;
+            s_arr[k] = o;
+            s_where[o] = k;
+        }
+        info[fr] = w.zero_at;
+    }
+}
+
 __global__ void nd_info_max(const rocblas_int *info, size_t n, int *out)
 {
     int m = 0;
@@ -503,6 +656,14 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
             TRT_CHECK(hipGetLastError());
             const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
             rocblas_int *piv = P.ipiv + L.poff * ns;
+            if (L.nF <= ND_SF && L.nIp <= ND_SI) {
+                // small fronts (the leaves and the levels above them): one kernel, one wavefront each
+                hipLaunchKernelGGL(nd_front_small, dim3(nb), dim3(64), 0, c.stream, F, f, L.nF, L.nIp, (size_t)sF,
+                                   (size_t)sf, piv, P.info + ioff, nb);
+                TRT_CHECK(hipGetLastError());
+                ioff += (size_t)nb;
+                continue;
+            }
             ND_RB(rocsolver_dgetrf_strided_batched(P.h, L.nIp, L.nIp, F, L.nF, sF, piv, sP, P.info + ioff, nb));
             ioff += (size_t)nb;
             ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, 1, F, L.nF, sF, piv, sP, f,
