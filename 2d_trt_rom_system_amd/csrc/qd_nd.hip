@@ -30,6 +30,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <functional>
 
@@ -165,19 +166,65 @@ __global__ void nd_scatter(double *x, size_t nrow, const int *sysmap, int ns, in
 
 // iterative refinement: residual r = b - A x of the chunk's systems, right-hand
 // sides of the fronts from r, the children's eliminated parts, and x += dx
-__global__ void nd_residual(QDView V, const int *sysmap, int ns, const double *x, double *R)
+// Residual R = b - A x of each system of the chunk (blockIdx.y) together with
+// per-block partial sums, in the equilibrated rows (row scales W), of |r|^2 and
+// (|A||x| + |b|)^2: the pieces of the normwise backward error.
+constexpr int ND_RB_BLOCKS = 64;
+
+__global__ __launch_bounds__(256) void nd_residual_bkwd(QDView V, const int *sysmap, const double *x, const double *W,
+                                                        double *R, double *part)
 {
-    const size_t tot = (size_t)ns * V.nrow;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
-        const int s = (int)(k / V.nrow);
-        const size_t row = k - (size_t)s * V.nrow;
-        const int sys = sysmap[s];
-        const double *xs = x + (size_t)sys * V.nrow;
-        double acc = 0.0, dg, rhs;
-        qd_row_visit(V, sys, row, [&](int kind, int i, int j, double a) { acc += a * xs[qd_index(V, kind, i, j)]; },
-                     dg, rhs);
-        R[k] = rhs - acc;
+    __shared__ double sh[2][4];
+    const int s = blockIdx.y;
+    const int sys = sysmap[s];
+    const double *xs = x + (size_t)sys * V.nrow;
+    const size_t off = (size_t)s * V.nrow;
+    double rr = 0.0, ss = 0.0;
+    for (size_t row = blockIdx.x * (size_t)blockDim.x + threadIdx.x; row < V.nrow;
+         row += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0, aax = 0.0, dg, rhs;
+        qd_row_visit(V, sys, row, [&](int kind, int i, int j, double a) {
+            const double t = a * xs[qd_index(V, kind, i, j)];
+            acc += t;
+            aax += fabs(t);
+        }, dg, rhs);
+        const double r = rhs - acc, w = W[off + row];
+        R[off + row] = r;
+        rr += (w * r) * (w * r);
+        ss += (w * (aax + fabs(rhs))) * (w * (aax + fabs(rhs)));
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        rr += __shfl_xor(rr, o, 64);
+        ss += __shfl_xor(ss, o, 64);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][wv] = rr;
+        sh[1][wv] = ss;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            a += sh[0][k];
+            b += sh[1][k];
+        }
+        part[((size_t)s * gridDim.x + blockIdx.x) * 2] = a;
+        part[((size_t)s * gridDim.x + blockIdx.x) * 2 + 1] = b;
+    }
+}
+
+// backward error of each system of the chunk from the partial sums
+__global__ void nd_bkwd_fin(const double *part, int nblk, int ns, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+        a += part[((size_t)s * nblk + k) * 2];
+        b += part[((size_t)s * nblk + k) * 2 + 1];
+    }
+    out[s] = a == 0.0 ? 0.0 : sqrt(a) / sqrt(b);
 }
 
 __global__ void nd_rhs(const double *R, const double *W, size_t nrow, int ns, int nIp, int nF, const int *rows,
@@ -405,6 +452,7 @@ struct NDPlan {
     int *ident = nullptr;           // 0, 1, ..., cap-1
     double *R = nullptr, *D = nullptr;   // refinement residual and correction [cap][nrow]
     double *W = nullptr;                 // row scales [cap][nrow]
+    double *part = nullptr, *bkwd = nullptr;   // backward-error partial sums and results of a chunk
     size_t ninfo = 0;
     rocblas_handle h = nullptr;
     double flops = 0.0;             // per system
@@ -611,6 +659,8 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
     P->R = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
     P->D = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
     P->W = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
+    P->part = static_cast<double *>(c.dalloc((size_t)std::max(P->cap, 1) * ND_RB_BLOCKS * 2 * sizeof(double)));
+    P->bkwd = static_cast<double *>(c.dalloc((size_t)std::max(P->cap, 1) * sizeof(double)));
     P->imax = static_cast<int *>(c.dalloc(sizeof(int)));
     P->levdev = static_cast<NDLevDev *>(c.dalloc((H + 1) * sizeof(NDLevDev)));
     ND_RB(rocblas_create_handle(&P->h));
@@ -702,11 +752,28 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
         TRT_CHECK(hipMemcpyAsync(&imax, P.imax, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         TRT_CHECK(hipStreamSynchronize(c.stream));
         if (imax > 0) throw Error("nested dissection: a front's pivot block is singular");
-        // three steps of iterative refinement with the stored factors: pivoting
-        // inside each front only can leave residuals well above rounding
-        for (int it = 0; it < 3; ++it) {
+        // iterative refinement with the stored factors while the worst backward
+        // error of the chunk is above 1e-14 (at most three steps): pivoting inside
+        // each front only can leave it above rounding
+        std::vector<double> hb(ns);
+        for (int it = 0;; ++it) {
             const size_t nv = (size_t)ns * s.nrow;
-            hipLaunchKernelGGL(nd_residual, dim3(nblocks(nv)), dim3(256), 0, c.stream, V, P.sysmap, ns, s.x, P.R);
+            hipLaunchKernelGGL(nd_residual_bkwd, dim3(ND_RB_BLOCKS, ns), dim3(256), 0, c.stream, V, P.sysmap, s.x,
+                               P.W, P.R, P.part);
+            hipLaunchKernelGGL(nd_bkwd_fin, dim3((ns + 63) / 64), dim3(64), 0, c.stream, P.part, ND_RB_BLOCKS, ns,
+                               P.bkwd);
+            TRT_CHECK(hipMemcpyAsync(hb.data(), P.bkwd, ns * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            TRT_CHECK(hipStreamSynchronize(c.stream));
+            bool above = false;    // a system above 1e-14, or not finite
+            double worst = 0.0;
+            for (double b : hb) {
+                if (!(b <= 1e-14)) above = true;
+                worst = std::fmax(worst, b);
+            }
+            if (c.verbose > 2)
+                fprintf(stderr, "[qd_nd] chunk of %d: backward error %.3e%s after %d refinements\n", ns, worst,
+                        above ? " (or not finite)" : "", it);
+            if (!above || it == 3) break;
             hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
             for (int l = 0; l <= H; ++l) {
                 const NDLevel &L = P.lev[l];
