@@ -13,17 +13,18 @@
 //
 // Numeric phase, bottom-up by node height, batched over the nodes of a height
 // and the systems solved (every equation scaled by 1 / its largest
-// coefficient, then three steps of iterative refinement with the stored
-// factors at the end): assemble the node's matrix entries and right-hand
+// coefficient; iterative refinement with the stored factors at the end while
+// the backward error is above 1e-14): assemble the node's matrix entries and right-hand
 // side into a dense front, add its children's Schur complements (each child
 // in turn, so the sums are deterministic), factor the I block (LU with partial
 // pivoting inside I), form A_II^-1 A_IB and the Schur complement
 // A_BB - A_BI A_II^-1 A_IB for the parent, and eliminate the right-hand side
 // the same way.  Back substitution runs top-down: x_I = A_II^-1 b_I - (A_II^-1
-// A_IB) x_B.  rocSOLVER getrf/getrs and rocBLAS gemm do the dense work on fronts
-// padded to the largest of their height (padding: identity in I, zeros in B).
-// At 256x256 cells this is about 1.5e9 flop per system and fronts of at most
-// 3 x 128 unknowns.
+// A_IB) x_B.  Fronts of up to 64 unknowns (the lowest levels) are done by
+// nd_front_small, one wavefront each; rocSOLVER getrf/getrs and rocBLAS gemm do
+// the dense work on the larger fronts.  Fronts are padded to the largest of
+// their height (padding: identity in I, zeros in B).  At 256x256 cells this is
+// about 1.1e9 flop per system and fronts of at most 384 unknowns.
 #include "qd_view.h"
 
 #include <rocblas/rocblas.h>
@@ -164,8 +165,6 @@ __global__ void nd_scatter(double *x, size_t nrow, const int *sysmap, int ns, in
     }
 }
 
-// iterative refinement: residual r = b - A x of the chunk's systems, right-hand
-// sides of the fronts from r, the children's eliminated parts, and x += dx
 // Residual R = b - A x of each system of the chunk (blockIdx.y) together with
 // per-block partial sums, in the equilibrated rows (row scales W), of |r|^2 and
 // (|A||x| + |b|)^2: the pieces of the normwise backward error.
@@ -227,6 +226,8 @@ __global__ void nd_bkwd_fin(const double *part, int nblk, int ns, double *out)
     out[s] = a == 0.0 ? 0.0 : sqrt(a) / sqrt(b);
 }
 
+// iterative refinement: right-hand sides of the fronts from the residual, the
+// children's eliminated parts, and x += dx
 __global__ void nd_rhs(const double *R, const double *W, size_t nrow, int ns, int nIp, int nF, const int *rows,
                        double *f)
 {
