@@ -151,11 +151,15 @@ def run_vs_fp64(name: str, steps: int, every: int) -> dict:
                 row[f"{k}_Eg_nonpos"] = int(np.count_nonzero(g.pull("HO_Eg_avg") <= 0.0))
             if s % every == 0 or s == steps:
                 ref = {q: forms["fp64"].pull(arr) for q, arr in QUANTITIES.items()}
+                eg = np.abs(forms["fp64"].pull("HO_Eg_avg"))
                 for k in ("fp32", "fp32s"):
                     for q, arr in QUANTITIES.items():
                         a = forms[k].pull(arr)
                         row[f"{k}_{q}"] = rel_l2(a, ref[q])
                         row[f"{k}_{q}_max"] = float(np.max(np.abs(a - ref[q])) / max(np.max(np.abs(ref[q])), 1e-300))
+                        if q.startswith("f_"):
+                            # weighted by the group energy density, as against the oracle
+                            row[f"{k}_{q}_Ew"] = weighted_rel(a, ref[q], eg)
                 print(json.dumps(row), file=sys.stderr, flush=True)
             rows.append(row)
     finally:
