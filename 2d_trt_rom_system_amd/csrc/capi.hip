@@ -48,7 +48,8 @@ void validate(const trt_problem *p)
     if (p->precision == 2 && p->sweep_mode == 1)
         throw Error("precision 2 runs on the tile engine (sweep_mode 0, 2, 3 or 4)");
     if (p->sweep_mode < 0 || p->sweep_mode > 4) throw Error("sweep_mode must be 0, 1, 2, 3 or 4");
-    if (p->qd_solver < 0 || p->qd_solver > 1) throw Error("qd_solver must be 0 (direct) or 1 (BiCGSTAB)");
+    if (p->qd_solver < 0 || p->qd_solver > 2)
+        throw Error("qd_solver must be 0 (nested dissection), 1 (BiCGSTAB) or 2 (block LU)");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
     for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
         if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");

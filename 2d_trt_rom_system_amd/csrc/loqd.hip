@@ -466,24 +466,27 @@ static DirectRes direct_residuals(Ctx &c, QDSystems &s, KrylovArgs &a, const std
 // underflows.  A system whose backward error is not finite or above 1e-11 -- a
 // front whose restricted pivoting failed -- is solved again by block LU over
 // grid rows, whose dense row blocks pivot over whole grid rows; one still above
-// 1e-9 throws.  (The reference's direct fallback, GAUSS_COMPLETE, checks nothing.)
+// 1e-9 throws.  qd_solver 2 uses block LU for every system, under the same check.
+// (The reference's direct fallback, GAUSS_COMPLETE, checks nothing.)
 static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vector<int> &which)
 {
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<int> bad;
-    try {
-        qd_nd(c, s, which);
-        const DirectRes d = direct_residuals(c, s, a, which);
-        for (size_t q = 0; q < which.size(); ++q)
-            if (!(d.bkwd[q] <= 1e-11)) bad.push_back(which[q]);
-    } catch (const Error &e) {
-        if (e.msg.find("singular") == std::string::npos) throw;
-        bad = which;
+    if (c.qd_solver == 2) {
+        bad = which;       // block LU chosen for every system
+    } else {
+        try {
+            qd_nd(c, s, which);
+            const DirectRes d = direct_residuals(c, s, a, which);
+            for (size_t q = 0; q < which.size(); ++q)
+                if (!(d.bkwd[q] <= 1e-11)) bad.push_back(which[q]);
+        } catch (const Error &e) {
+            if (e.msg.find("singular") == std::string::npos) throw;
+            bad = which;
+        }
+        if (!bad.empty() && c.verbose) fprintf(stderr, "[qd_solve] %zu systems again by block LU\n", bad.size());
     }
-    if (!bad.empty()) {
-        if (c.verbose) fprintf(stderr, "[qd_solve] %zu systems again by block LU\n", bad.size());
-        qd_direct(c, s, bad);
-    }
+    if (!bad.empty()) qd_direct(c, s, bad);
     const DirectRes d = direct_residuals(c, s, a, which);
     if (c.verbose > 1) {
         size_t w = 0;
@@ -522,7 +525,7 @@ void qd_solve(Ctx &c, QDSystems &s, int *its_out, bool ref_given)
     const dim3 grid(a.RB, s.nsys), blk(RT);
     const int fin_blocks = (s.nsys + 63) / 64;
     const size_t n = (size_t)s.nsys * s.nrow;
-    if (c.qd_solver == 0) {
+    if (c.qd_solver != 1) {
         // every system by the direct solver (reported as -1 iterations, the reference's
         // flag for a direct solve, QS:499)
         std::vector<int> all(s.nsys);

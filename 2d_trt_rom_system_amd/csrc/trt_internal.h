@@ -121,7 +121,7 @@ struct Ctx {
     std::vector<int> last_kits;
     DirectWork *direct = nullptr;
     NDPlan *nd = nullptr;
-    int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback
+    int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback; 2: block LU
 
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);

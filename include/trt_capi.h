@@ -59,7 +59,9 @@ typedef struct {
                                        moment reduction; 4 = one persistent launch over column
                                        segments of tiles                                  */
     int qd_solver;                  /* reduced QD systems: 0 = nested-dissection direct solve;
-                                       1 = batched BiCGSTAB with the direct solve as fallback */
+                                       1 = batched BiCGSTAB with the direct solve as fallback;
+                                       2 = block LU over grid rows (the slow direct solve that
+                                       backs up the other two) */
 } trt_problem;
 
 /* Context life cycle.  trt_ctx_create allocates the device-resident state and
