@@ -22,7 +22,7 @@ MODULE TRT_CAPI
     REAL(C_DOUBLE) :: lin_rtol
     INTEGER(C_INT) :: lin_maxit
     INTEGER(C_INT) :: sweep_mode
-    INTEGER(C_INT) :: qd_solver
+    INTEGER(C_INT) :: qd_solver      ! 0 nested dissection, 1 BiCGSTAB, 2 block LU
   END TYPE trt_problem
 
   ! the context of the running problem, created by TRT_GPU_INIT
