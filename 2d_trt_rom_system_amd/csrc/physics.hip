@@ -37,7 +37,7 @@ __global__ void material_kernel(MatArgs a)
         a.MGQD_Src[idx] = 4.0 * PI * kap * bg;
         a.RT_Src[idx] = kap * bg;
         a.sc[c * a.ng + g] = make_double2(kap + a.inv_cdt, kap * bg);
-        if (a.scf) a.scf[c * a.ng + g] = make_float2((float)(kap + a.inv_cdt), (float)(kap * bg));
+        if (a.scf) a.scf[c * a.ng + g] = make_float2((float)(kap + a.inv_cdt), (float)(kap * bg * kIScale32));
     }
 }
 

@@ -413,7 +413,7 @@ __global__ void icrn_init_kernel(TI *I, const double *bg, size_t ncell, int ng, 
     const size_t tot = ncell * 4 * NL * 4;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
         const size_t L = (i / 4) % NL;
-        I[i] = (TI)bg[L / K];
+        I[i] = (TI)(bg[L / K] * iscale<TI>());
     }
 }
 

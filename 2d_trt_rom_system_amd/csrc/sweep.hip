@@ -124,7 +124,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
     TI *Inew = static_cast<TI *>(a.Inew);
     TI *XV = static_cast<TI *>(a.XV);
     TI *XH = static_cast<TI *>(a.XH);
-    const double eps = a.eps;
+    const double eps = a.eps, isc = 1.0 / iscale<TI>();
     const TI inv_cdt = (TI)a.inv_cdt;
 
     const int i0 = txl * TX, iw = min(TX, nx - i0);
@@ -133,7 +133,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
     // incoming H corner-edge values of each column (local left, local right)
     TI hl[TX], hr[TX];
     {
-        const TI bcy = act ? (TI)a.bcin[(py ? 1 : 3) * ng + g] : (TI)0;
+        const TI bcy = act ? (TI)(a.bcin[(py ? 1 : 3) * ng + g] * iscale<TI>()) : (TI)0;
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             hl[ii] = bcy; hr[ii] = bcy;
@@ -144,7 +144,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
             }
         }
     }
-    const TI bcx = act ? (TI)a.bcin[(px ? 0 : 2) * ng + g] : (TI)0;
+    const TI bcx = act ? (TI)(a.bcin[(px ? 0 : 2) * ng + g] * iscale<TI>()) : (TI)0;
 
     for (int r = 0; r < jh; ++r) {
         const int jl = j0 + r;
@@ -182,7 +182,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
             const TI A = (TI)(dxc * dyc);
             const TI A4 = A * (TI)0.25;
             const TI aa = (TI)(axn * dyc * 0.25), bb = (TI)(ayn * dxc * 0.25);
-            const TI kt = (TI)scv[ii].x, src = (TI)scv[ii].y;
+            const TI kt = (TI)scv[ii].x, src = (TI)(scv[ii].y * iscale<TI>());
             const TI gam = ((TI)(axn * dyc) + (TI)(ayn * dxc) + kt * A) * (TI)0.25;
             // local sources S = RT_Src + I_old / (c dt)  (TRANSPORT_SOLVES.f08:48)
             const TI s0 = src + io[ii].v[perm[0]] * inv_cdt;
@@ -203,8 +203,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
             const TI x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
             const TI x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
             const TI x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
-            const double IVin = ((double)vb + (double)vt) * 0.5;
-            const double IHin = ((double)hl[ii] + (double)hr[ii]) * 0.5;
+            const double IVin = ((double)vb + (double)vt) * 0.5 * isc;
+            const double IHin = ((double)hl[ii] + (double)hr[ii]) * 0.5 * isc;
             if (act) {
                 Vec4<TI> o;
                 o.v[perm[0]] = x0; o.v[perm[1]] = x1; o.v[perm[2]] = x2; o.v[perm[3]] = x3;
@@ -213,9 +213,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
             // upwinding to the outgoing corner edges (TRANSPORT_SOLVES.f08:86-89)
             vb = x1; vt = x2; hl[ii] = x3; hr[ii] = x2;
             // angular moments (COLLAPSE_INTENSITIES, :447-470), eps added in FP64
-            const double Ia = ((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25 + eps;
-            const double Iv = ((double)x1 + (double)x2) * 0.5 + eps;
-            const double Ih = ((double)x3 + (double)x2) * 0.5 + eps;
+            const double Ia = ((double)x0 + (double)x1 + (double)x2 + (double)x3) * 0.25 * isc + eps;
+            const double Iv = ((double)x1 + (double)x2) * 0.5 * isc + eps;
+            const double Ih = ((double)x3 + (double)x2) * 0.5 * isc + eps;
             const double vals[NMOM] = {cxx * Ia, cyy * Ia, cxy * Ia, cw * Ia,
                                        cxx * Iv, cxy * Iv, cx * Iv, cw * Iv,
                                        cyy * Ih, cxy * Ih, cx * Ih, cy * Ih, cw * Ih};
@@ -403,7 +403,7 @@ __global__ void icrn_to_fortran(const TI *src, double *dst, int nx, int ny, int 
         const int corner = top ? (right ? 2 : 3) : (right ? 1 : 0);
         const int q = m / K, k = m % K;
         const size_t c = (size_t)i + (size_t)nx * j;
-        dst[idx] = (double)src[((c * 4 + q) * ((size_t)ng * K) + (size_t)g * K + k) * 4 + corner];
+        dst[idx] = (double)src[((c * 4 + q) * ((size_t)ng * K) + (size_t)g * K + k) * 4 + corner] * (1.0 / iscale<TI>());
     }
 }
 
@@ -421,7 +421,7 @@ __global__ void icrn_from_fortran(const double *src, TI *dst, int nx, int ny, in
         const int corner = top ? (right ? 2 : 3) : (right ? 1 : 0);
         const int q = m / K, k = m % K;
         const size_t c = (size_t)i + (size_t)nx * j;
-        dst[((c * 4 + q) * ((size_t)ng * K) + (size_t)g * K + k) * 4 + corner] = (TI)src[idx];
+        dst[((c * 4 + q) * ((size_t)ng * K) + (size_t)g * K + k) * 4 + corner] = (TI)(src[idx] * iscale<TI>());
     }
 }
 
@@ -488,7 +488,7 @@ __global__ void derived_kernel(const TI *I, const double *bcin, double *out, int
         double v = 0.0;
         if (kind == 0) {
             const TI *p = rec(a, b);
-            v = ((double)p[0] + (double)p[1] + (double)p[2] + (double)p[3]) * 0.25;
+            v = ((double)p[0] + (double)p[1] + (double)p[2] + (double)p[3]) * 0.25 * (1.0 / iscale<TI>());
         } else if (kind == 1 || kind == 3) {          // vertical faces
             const int f = a;
             const int j = kind == 1 ? b : b / 2, top = kind == 3 ? (b & 1) : -1;
@@ -496,7 +496,8 @@ __global__ void derived_kernel(const TI *I, const double *bcin, double *out, int
             else if (!px && f == nx) v = bcin[2 * ng + g];
             else {
                 const TI *p = px ? rec(f - 1, j) : rec(f, j);
-                const double lo = px ? (double)p[1] : (double)p[0], hi = px ? (double)p[2] : (double)p[3];
+                const double sc = 1.0 / iscale<TI>();
+                const double lo = (px ? (double)p[1] : (double)p[0]) * sc, hi = (px ? (double)p[2] : (double)p[3]) * sc;
                 v = top < 0 ? (lo + hi) * 0.5 : (top ? hi : lo);
             }
         } else {                                      // horizontal faces
@@ -506,7 +507,8 @@ __global__ void derived_kernel(const TI *I, const double *bcin, double *out, int
             else if (!py && f == ny) v = bcin[3 * ng + g];
             else {
                 const TI *p = py ? rec(i, f - 1) : rec(i, f);
-                const double lft = py ? (double)p[3] : (double)p[0], rgt = py ? (double)p[2] : (double)p[1];
+                const double sc = 1.0 / iscale<TI>();
+                const double lft = (py ? (double)p[3] : (double)p[0]) * sc, rgt = (py ? (double)p[2] : (double)p[1]) * sc;
                 v = right < 0 ? (lft + rgt) * 0.5 : (right ? rgt : lft);
             }
         }
@@ -553,7 +555,7 @@ __global__ void sc_kernel(const double *KE, const double *RS, double2 *sc, float
         const size_t cc = idx % nc;
         const int g = (int)(idx / nc);
         sc[cc * ng + g] = make_double2(KE[idx] + inv_cdt, RS[idx]);
-        if (scf) scf[cc * ng + g] = make_float2((float)(KE[idx] + inv_cdt), (float)RS[idx]);
+        if (scf) scf[cc * ng + g] = make_float2((float)(KE[idx] + inv_cdt), (float)(RS[idx] * kIScale32));
     }
 }
 

@@ -166,13 +166,15 @@ __device__ __forceinline__ void wait_flag(const int *f, int epoch, int limit, in
     }
 }
 
-__device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q)
+// (weights times isc: the factor of the stored intensities leaves with the weights)
+__device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q, double isc)
 {
     for (int t = threadIdx.x; t < P.K; t += PK_THREADS) {
         const int mm = q * P.K + t;
         const double wm = gptr(P.w)[mm], ox = gptr(P.ox)[mm], oy = gptr(P.oy)[mm];
         double *cc = ctab + t * 6;
-        cc[0] = wm; cc[1] = wm * ox * ox; cc[2] = wm * oy * oy; cc[3] = wm * ox * oy; cc[4] = wm * ox; cc[5] = wm * oy;
+        cc[0] = wm * isc; cc[1] = wm * ox * ox * isc; cc[2] = wm * oy * oy * isc; cc[3] = wm * ox * oy * isc;
+        cc[4] = wm * ox * isc; cc[5] = wm * oy * isc;
     }
 }
 
@@ -248,8 +250,8 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             }
         }
     }
-    const TI bcx = (TI)gptr(P.bcin)[(px ? 0 : 2) * ng + gg];
-    const TI bcy = (TI)gptr(P.bcin)[(py ? 1 : 3) * ng + gg];
+    const TI bcx = (TI)(gptr(P.bcin)[(px ? 0 : 2) * ng + gg] * iscale<TI>());
+    const TI bcy = (TI)(gptr(P.bcin)[(py ? 1 : 3) * ng + gg] * iscale<TI>());
 
     sync();
 
@@ -392,7 +394,7 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const
     const int i0 = txl * TX, iw = min(TX, nx - i0);
     const int j0 = tyl * TY, jh = min(TY, ny - j0);
     const int g0 = gb * gblk;
-    const double eps = P.eps;
+    const double eps = P.eps * iscale<TS>();   // (the weights carry 1 / iscale)
     for (int pb = 0; pb < P.npair; pb += pstride) {
         const int p = pb + pofs;
         const int cidx = p / gblk, gl = p - cidx * gblk;
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
                 if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
                 if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
             }
-            load_ctab(P, ctab, q);
+            load_ctab(P, ctab, q, 1.0 / iscale<TI>());
             __syncthreads();
         }, hc, false, true);
         wait_vm_stores();                 // this wave's face stores have completed
@@ -553,7 +555,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, 
     TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
     TI hc[TX][2];
     tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
-        load_ctab(P, ctab, q);
+        load_ctab(P, ctab, q, 1.0 / iscale<TI>());
         __syncthreads();
     }, hc, false, true);
     __syncthreads();
@@ -616,7 +618,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
                     if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
                     if (tyl == t0 && tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
                 }
-                if (tyl == t0) load_ctab(P, ctab, q);
+                if (tyl == t0) load_ctab(P, ctab, q, 1.0 / iscale<TI>());
                 __syncthreads();
             }, hc, tyl > t0, tyl == t1 - 1);
             wait_vm_stores();             // this wave's face stores have completed
@@ -658,7 +660,7 @@ __global__ void pk_boundary_moments(const PkProb *probs)
         const bool px = (q == 0 || q == 3), py = (q == 0 || q == 1);
         if (f < ny) {
             const int j = f;
-            const double b = (double)(TI)P.bcin[(px ? 0 : 2) * ng + g] + P.eps;
+            const double b = (double)(TI)(P.bcin[(px ? 0 : 2) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
             double s[4] = {0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;
@@ -670,7 +672,7 @@ __global__ void pk_boundary_moments(const PkProb *probs)
             o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3];
         } else {
             const int i = f - ny;
-            const double b = (double)(TI)P.bcin[(py ? 1 : 3) * ng + g] + P.eps;
+            const double b = (double)(TI)(P.bcin[(py ? 1 : 3) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
             double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;

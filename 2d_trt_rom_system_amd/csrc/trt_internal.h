@@ -20,6 +20,16 @@
 
 namespace trt {
 
+// Intensities stored in FP32 carry a factor 2^40: the cold high groups (near
+// 1e-40) then stay clear of FP32's denormal range in the FP32 corner solve,
+// whose largest intermediates (coefficient^3 x intensity) stay below 1e36.  The
+// factor enters with the initial values, the FP32 sources and the boundary
+// intensities, and leaves in the FP64 moments and at push/pull; a power of two,
+// it changes no rounding.
+constexpr double kIScale32 = 1099511627776.0;   // 2^40
+template <typename TI>
+__host__ __device__ constexpr double iscale() { return sizeof(TI) == 4 ? kIScale32 : 1.0; }
+
 struct Error {
     std::string msg;
     explicit Error(std::string m) : msg(std::move(m)) {}
