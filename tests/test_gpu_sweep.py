@@ -182,3 +182,19 @@ def test_persistent_equals_per_diagonal_bitwise(trt, precision, tile):
         for a, b in zip(outs[0], other):
             for x, y in zip(a, b):
                 assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_fp32_storage_keeps_tiny_intensities(trt, precision):
+    """FP32-stored intensities carry a power-of-two factor: values down to 1e-40
+    (cold high groups) survive push and pull with FP32's relative rounding, where
+    plain FP32 would hold them as denormals with a few bits."""
+    prob = trt.problem.small(nx=6, ny=5, order=4, ng=3, seed=4)
+    ctx = trt.native.Context(prob, precision=precision)
+    shape = (2 * prob.nx, 2 * prob.ny, prob.nm, prob.ng)
+    rng = np.random.default_rng(4)
+    vals = np.asfortranarray(10.0 ** rng.uniform(-40.0, 3.0, size=shape))
+    ctx.push("I_crn_old", vals)
+    back = ctx.pull("I_crn_old", shape)
+    assert np.max(np.abs(back - vals) / vals) < 1e-7
+    ctx.close()
