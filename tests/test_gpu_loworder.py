@@ -76,6 +76,23 @@ def test_mloqd_fv_matches(trt, oracle_mod, qd_solver, lin_maxit):
     ctx.close()
 
 
+@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 17), (17, 3), (5, 5), (8, 8), (33, 6)])
+def test_mloqd_fv_grid_shapes(trt, oracle_mod, nx, ny):
+    """Nested dissection on grids whose trees differ: one leaf holding everything
+    (2x2: no border), thin strips, ragged leaves, a power-of-two square."""
+    prob = trt.problem.small(nx=nx, ny=ny, order=4, ng=2, seed=nx * 31 + ny)
+    o = oracle_mod.Oracle(prob, threads=2)
+    o.step()
+    ctx = trt.native.Context(prob, qd_solver=0)
+    sync_to_gpu(o, ctx, ML_INPUTS)
+    o.call("mloqd_fv")
+    kits = ctx.mloqd_fv()
+    assert all(k == -1 for k in kits), kits
+    for n in ("Eg_avg", "Eg_edgV", "Eg_edgH", "Fxg_edgV", "Fyg_edgH"):
+        assert rel_err(ctx.pull(n), o[n].ravel(order="F")) < 1e-10, n
+    ctx.close()
+
+
 def test_grey_chain_matches(trt, oracle_mod):
     prob, o, ctx = _pair(trt, oracle_mod, steps=1)
     names = ML_INPUTS + ["Fxg_edgV", "Fyg_edgH", "KapE_Bar"]
