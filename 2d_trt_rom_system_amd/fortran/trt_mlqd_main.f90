@@ -35,7 +35,7 @@ PROGRAM trt_mlqd
   INTEGER:: t, u, v, i, j
   INTEGER(C_INT):: its4(4)
 
-  ! the driver's arrays, named as in TRT_MLQD_ALGORITHM (ALGORITHMS.f08:59-127)
+  ! the time step's arrays, named as in TRT_MLQD_ALGORITHM (ALGORITHMS.f08:59-127)
   REAL*8,ALLOCATABLE:: I_avg(:,:,:,:), I_edgV(:,:,:,:), I_edgH(:,:,:,:), I_crn(:,:,:,:), I_crn_old(:,:,:,:)
   REAL*8,ALLOCATABLE:: Ic_edgV(:,:,:,:), Ic_edgH(:,:,:,:), RT_Src(:,:,:,:), RT_Residual(:,:,:)
   INTEGER,ALLOCATABLE:: RT_ResLoc_x(:,:,:), RT_ResLoc_y(:,:,:), MGQD_Kits(:), GQD_Kits(:)

@@ -169,7 +169,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
-              "see the sharding note in DESIGN.md)", file=sys.stderr)
+              "it would shard by energy group, one all-reduce of the grey sums per MLOQD iteration)", file=sys.stderr)
         return 2
     trt = importlib.import_module("2d_trt_rom_system_amd")
     prob = trt.problem.cfg1()
