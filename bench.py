@@ -122,6 +122,25 @@ def measure_batch(trt, probs, precision: int, steps: int):
             "autotune_ms_per_sweep": {f"{t[0]}x{t[1]}/seg{sg}": round(v, 3) for (t, sg), v in table.items()}}
 
 
+def measure_steps(trt, precision: int, steps: int = 10):
+    """Full MLQD time steps (the drop-in's whole iteration: sweeps, QD factors, MLOQD and
+    GLOQD solves, temperature update) of the 256x256, S16, 32-group problem with uniform
+    material: cfg1's lognormal(0, 1.5) map stops the reference algorithm in its first step."""
+    prob = trt.problem.mapped(nx=256, order=16, ng=32, sigma=0.0, nsteps=steps + 1)
+    ctx = trt.native.Context(prob, precision=precision)
+    try:
+        ctx.step()                                  # warm-up (plans, handles, first factorisations)
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            its.append(list(ctx.step()))
+        dt = (time.perf_counter() - t0) / steps
+    finally:
+        ctx.close()
+    return {"s_per_step": dt, "steps": steps, "iterations_per_step": its[-1],
+            "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps"}
+
+
 def cpu_baseline(trt, budget_s: float = 12.0):
     """The FP64 CPU restatement (oracle) of TRANSPORT_SCB + COLLAPSE_INTENSITIES on the same
     256x256 S16 grid, OpenMP over groups as the reference, one group per thread."""
@@ -163,6 +182,7 @@ def main(argv=None):
                     help="measure one form only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop load")
+    ap.add_argument("--no-steps", action="store_true", help="skip the full MLQD time steps")
     ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -249,6 +269,11 @@ def main(argv=None):
         sv["note"] = ("requests drawn from the batched distribution, 20 Tgen_QDf steps each (one sweep per step), "
                       "Poisson arrivals, HBM budget 200 GiB, FP32 sweep; latency = completion - arrival")
         out["serving"] = sv
+    if not args.no_steps and not args.only:
+        print("[bench] full MLQD time steps (256x256, uniform material)", file=sys.stderr, flush=True)
+        out["full_step"] = {"problem": "256x256 cells, S16, 32 groups 1e-3..1e2 keV, uniform material, "
+                                       "dt 2e-3 ns, QD systems by nested dissection",
+                            "fp64": measure_steps(trt, 0), "fp32": measure_steps(trt, 1)}
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(trt)
