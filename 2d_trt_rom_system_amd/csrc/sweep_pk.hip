@@ -808,11 +808,22 @@ static void pk_setup(Ctx &c)
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
     P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
-    const bool segs = c.sweep_mode == 4;
     const size_t ti = c.precision ? 4 : 8;
     pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
+    // The default form (sweep_mode 0) takes column segments of 4 or 2 tiles when their
+    // wavefront (items of one diagonal) still offers twice the resident workgroups, as
+    // batches do: on large grids segments save face traffic, on small ones the segment's
+    // serial tiles would lengthen the critical path.  Results are bitwise the same.
+    int auto_seg = 0;
+    if (c.p.sweep_mode == 0 && c.sweep_mode == 2 && !getenv("TRT_SEG_LEN")) {
+        const long long g2 = 2LL * resident_grid(pk->k->seg, pk->lds, 1 << 30);
+        for (int s = 4; s >= 2 && auto_seg == 0; s /= 2)
+            if ((long long)std::min(sg.NTX, (sg.NTY + s - 1) / s) * 4 * sg.nblk_g >= g2) auto_seg = s;
+        if (auto_seg) c.sweep_mode = 4;
+    }
+    const bool segs = c.sweep_mode == 4;
     const int full_grid = resident_grid(segs ? pk->k->seg : pk->k->pk, pk->lds, 1 << 30);
-    P.seg = segs ? seg_length(sg, full_grid) : 1;
+    P.seg = segs ? (auto_seg ? std::min(auto_seg, sg.NTY) : seg_length(sg, full_grid)) : 1;
     P.nseg = (sg.NTY + P.seg - 1) / P.seg;
     PkProb tab[2] = {P, P};
     tab[0].Iold = c.Icrn[0]; tab[0].Inew = c.Icrn[1];
