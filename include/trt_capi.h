@@ -53,7 +53,8 @@ typedef struct {
     double lin_rtol;                /* relative residual target of the QD Krylov solves; 0 = default 1e-13 */
     int lin_maxit;                  /* iteration cap of the QD Krylov solves (qd_solver 1); 0 = default
                                        2000.  A system BiCGSTAB does not solve goes to the direct solve */
-    int sweep_mode;                 /* 0 = default (2); 1 = one launch per tile diagonal with a
+    int sweep_mode;                 /* 0 = default (2, or 4 on grids large enough that column
+                                       segments keep the device busy); 1 = one launch per tile diagonal with a
                                        per-cell moment reduction; 2 = one persistent launch per
                                        sweep; 3 = one launch per tile diagonal with per-row
                                        moment reduction; 4 = one persistent launch over column
