@@ -787,6 +787,15 @@ static int seg_length(const SweepGeom &sg, int grid)
     return seg;
 }
 
+// Flag checks a dependency wait makes before it gives up and sets the error word
+// (TRT_SPIN_LIMIT overrides it, e.g. 0 to exercise the error path: every wait then
+// ends at its first check, every item still runs, and the sweep reports the error).
+static int spin_limit()
+{
+    if (const char *v = getenv("TRT_SPIN_LIMIT")) return std::max(0, atoi(v));
+    return 1 << 20;
+}
+
 static void pk_setup(Ctx &c)
 {
     const SweepGeom &sg = c.sg;
@@ -879,7 +888,7 @@ void pk_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     a.nitems = pk.nitems;
     a.queue = pk.queue;
     a.flags = pk.flags;
-    a.spin_limit = 1 << 20;
+    a.spin_limit = spin_limit();
     void *args[] = {&a};
     if (ev) TRT_CHECK(hipEventRecord((*ev)[0], c.stream));
     TRT_CHECK(hipLaunchKernel(pk.kern, dim3(pk.grid), dim3(PK_THREADS), args, pk.lds, c.stream));
@@ -1110,7 +1119,7 @@ static void batch_enqueue(PkBatch &b)
     a.nitems = b.nitems;
     a.queue = b.queue;
     a.flags = b.flags;
-    a.spin_limit = 1 << 20;
+    a.spin_limit = spin_limit();
     void *args[] = {&a};
     TRT_CHECK(hipLaunchKernel(b.kern, dim3(b.grid), dim3(PK_THREADS), args, b.lds, b.stream));
 }

@@ -198,3 +198,19 @@ def test_fp32_storage_keeps_tiny_intensities(trt, precision):
     back = ctx.pull("I_crn_old", shape)
     assert np.max(np.abs(back - vals) / vals) < 1e-7
     ctx.close()
+
+
+def test_spin_limit_exceeded_is_reported(trt, monkeypatch):
+    """A dependency wait of the persistent sweep that exceeds its spin limit (forced
+    from the host: limit 0, so waits end at their first check) lets the sweep finish
+    and is reported as an error; a context with the normal limit sweeps cleanly."""
+    prob = trt.problem.small(nx=64, ny=64, order=8, ng=3, seed=2)
+    monkeypatch.setenv("TRT_SPIN_LIMIT", "0")
+    ctx = trt.native.Context(prob, precision=1, tile=(4, 2), sweep_mode=2)
+    with pytest.raises(trt.native.TrtError, match="spin limit"):
+        ctx.call("transport_scb")
+    ctx.close()
+    monkeypatch.delenv("TRT_SPIN_LIMIT")
+    ctx = trt.native.Context(prob, precision=1, tile=(4, 2), sweep_mode=2)
+    ctx.call("transport_scb")
+    ctx.close()
