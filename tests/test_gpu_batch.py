@@ -189,3 +189,18 @@ def test_batch_rejects_negative_segment_length(trt):
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_batch_spin_limit_exceeded_is_reported(trt, monkeypatch):
+    """As for one problem: a batched sweep whose dependency waits exceed the spin
+    limit (forced to 0 from the host) finishes and reports the error."""
+    probs = [trt.problem.small(nx=48, ny=40, order=8, ng=3, seed=s) for s in (1, 2)]
+    ctxs = [trt.native.Context(p, precision=1, tile=(4, 2)) for p in probs]
+    try:
+        monkeypatch.setenv("TRT_SPIN_LIMIT", "0")
+        with trt.native.Batch(ctxs) as b:
+            with pytest.raises(trt.native.TrtError, match="spin limit"):
+                b.sweep_timed(1)
+    finally:
+        for c in ctxs:
+            c.close()
