@@ -70,6 +70,30 @@ def test_server_runs_twice_on_one_arena(trt):
             assert np.array_equal(srv.output(a, n), srv.output(b, n)), n
 
 
+def test_error_in_a_tick_fails_held_requests(trt, monkeypatch):
+    """An error inside a tick (a batched sweep whose spin limit, forced to 0 from the
+    host, is exceeded) ends the run with an error and fails the requests it held; the
+    server then serves new requests normally, with the same results as before."""
+    probs = _mix(trt)
+    with trt.serve.Server(8 << 30) as srv:
+        ref = srv.submit(probs[0], steps=2)
+        srv.run()
+        good = {n: srv.output(ref, n) for n in OUTS}
+        monkeypatch.setenv("TRT_SPIN_LIMIT", "0")
+        held = [srv.submit(p, steps=3) for p in probs[:2]]
+        with pytest.raises(trt.native.TrtError, match="spin limit"):
+            srv.run()
+        for i in held:
+            st = srv.request(i)
+            assert st["status"] == -1 and "aborted" in st["error"], st
+        monkeypatch.delenv("TRT_SPIN_LIMIT")
+        again = srv.submit(probs[0], steps=2)
+        srv.run()
+        assert srv.request(again)["status"] == 0
+        for n in OUTS:
+            assert np.array_equal(srv.output(again, n), good[n]), n
+
+
 def test_admission_honours_budget_and_cap(trt):
     p = trt.problem.small(nx=96, ny=96, order=8, ng=16, seed=41)
     need = trt.native.device_bytes(p, precision=1, tile=(4, 2))
