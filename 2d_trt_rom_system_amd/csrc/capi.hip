@@ -270,6 +270,16 @@ int trt_step(trt_ctx *ctx, int its_out[4])
 
 double trt_time(const trt_ctx *ctx) { return ctx ? ctx->c.time : 0.0; }
 
+int trt_restart_out(trt_ctx *ctx, const char *path)
+{
+    return guarded(ctx, [&](Ctx &c) { trt::restart_out(c, path); });
+}
+
+int trt_restart_in(trt_ctx *ctx, const char *path)
+{
+    return guarded(ctx, [&](Ctx &c) { trt::restart_in(c, path); });
+}
+
 int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms)
 {
     return guarded(ctx, [&](Ctx &c) {

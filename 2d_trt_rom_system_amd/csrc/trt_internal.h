@@ -177,6 +177,10 @@ void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, i
 int step(Ctx &c, int its_out[4]);
 void pull_icrn(Ctx &c, int which, double *host);
 void push_icrn(Ctx &c, int which, const double *host);
+// RESTART_OUT / RESTART_IN of the context's state (csrc/restart.hip); restart_in
+// checks the whole file before it changes any device array
+void restart_out(Ctx &c, const char *path);
+void restart_in(Ctx &c, const char *path);
 void sweep_setup(Ctx &c);
 // persistent sweep (sweep_pk.hip)
 bool pk_supported(const Ctx &c);

@@ -144,6 +144,18 @@ MODULE TRT_CAPI
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION
+    ! the device state as the reference's restart file (RESTART_OUT, OUTPUTS.f08:1624-1674) and back
+    ! (RESTART_IN, INPUTS.f08:1734-1783); path is NUL-terminated
+    INTEGER(C_INT) FUNCTION trt_restart_out(ctx, path) BIND(C, name="trt_restart_out")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: path(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_restart_in(ctx, path) BIND(C, name="trt_restart_in")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: path(*)
+    END FUNCTION
   END INTERFACE
 
 CONTAINS
@@ -165,6 +177,21 @@ FUNCTION TRT_ERRMSG() RESULT(msg)
     msg(k:k) = s(k)
   END DO
 END FUNCTION TRT_ERRMSG
+
+!----------------------------------------------------------------------------------------------------------------------------------!
+! checkpoint / resume of the running context in the reference's restart file format; the device-resident
+! counterpart of RESTART_OUT and RESTART_IN, called where the reference calls them (ALGORITHMS.f08:889-894
+! every restart_freq-th step, :234-237 before the time loop)
+!----------------------------------------------------------------------------------------------------------------------------------!
+SUBROUTINE TRT_GPU_RESTART_OUT(restart_outfile)
+  CHARACTER(*), INTENT(IN) :: restart_outfile
+  CALL TRT_CHECK(trt_restart_out(TRT_CTX, TRIM(restart_outfile)//C_NULL_CHAR), 'RESTART_OUT')
+END SUBROUTINE TRT_GPU_RESTART_OUT
+
+SUBROUTINE TRT_GPU_RESTART_IN(restart_infile)
+  CHARACTER(*), INTENT(IN) :: restart_infile
+  CALL TRT_CHECK(trt_restart_in(TRT_CTX, TRIM(restart_infile)//C_NULL_CHAR), 'RESTART_IN')
+END SUBROUTINE TRT_GPU_RESTART_IN
 
 SUBROUTINE TRT_CHECK(rc, what)
   INTEGER(C_INT), INTENT(IN) :: rc

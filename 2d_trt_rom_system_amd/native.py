@@ -49,6 +49,8 @@ EXPORTS = [
     "trt_server_create", "trt_server_destroy", "trt_server_last_error", "trt_server_submit", "trt_server_run",
     "trt_server_request", "trt_server_request_error", "trt_server_summary", "trt_server_output_size",
     "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
+    "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
+    "trt_frecords_last_error",
 ]
 
 _lib = None
@@ -138,6 +140,12 @@ def load(path: str = LIBPATH):
     L.trt_autotune_batch.argtypes = [C.POINTER(TrtProblem), C.c_int, C.c_int, C.c_int, ip, ip, ip, _dp, _dp]
     L.trt_autotune_segments.argtypes = [ip, C.c_int]
     L.trt_autotune_last_error.restype = C.c_char_p
+    L.trt_restart_out.argtypes = [vp, C.c_char_p]
+    L.trt_restart_in.argtypes = [vp, C.c_char_p]
+    L.trt_restart_time_in.argtypes = [C.c_char_p, _dp]
+    L.trt_frecords_write.argtypes = [C.c_char_p, C.c_int, C.POINTER(_dp), C.POINTER(C.c_size_t), C.c_int]
+    L.trt_frecords_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(_dp), C.POINTER(C.c_size_t)]
+    L.trt_frecords_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -274,6 +282,15 @@ class Context:
     def time(self) -> float:
         return float(self.L.trt_time(self.h))
 
+    def restart_out(self, path: str) -> None:
+        """RESTART_OUT (MainSrc/OUTPUTS.f08:1624-1674): the context's state as the
+        reference's 25-record restart file."""
+        self._check(self.L.trt_restart_out(self.h, os.fsencode(path)), "restart_out")
+
+    def restart_in(self, path: str) -> None:
+        """RESTART_IN (MainSrc/INPUTS.f08:1734-1783); the context is unchanged on error."""
+        self._check(self.L.trt_restart_in(self.h, os.fsencode(path)), "restart_in")
+
     def sweep_timed(self, nsweeps: int) -> float:
         ms = C.c_double()
         self._check(self.L.trt_sweep_timed(self.h, int(nsweeps), C.byref(ms)), "sweep_timed")
@@ -360,6 +377,44 @@ class Batch:
         u, g, n = C.c_longlong(), C.c_int(), C.c_int()
         self._check(self.L.trt_batch_info(self.h, C.byref(u), C.byref(g), C.byref(n)), "batch_info")
         return u.value, g.value, n.value
+
+
+def _io_check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise TrtError(f"{what}: {load().trt_frecords_last_error().decode()}")
+
+
+def restart_time_in(path: str) -> float:
+    """RESTART_TIME_IN (MainSrc/INPUTS.f08:1785-1803): the Time record of a restart file."""
+    t = C.c_double()
+    _io_check(load().trt_restart_time_in(os.fsencode(path), C.byref(t)), "restart_time_in")
+    return t.value
+
+
+RESTART_RECORDS = ("Time", "Temp", "I_crn", "fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy",
+                   "fg_edgH_xy", "RT_Src", "MGQD_Src", "GQD_Src", "E_avg", "Eg_avg", "Eg_edgV", "Eg_edgH", "Fxg_edgV",
+                   "Fyg_edgH", "KapE_Bar", "KapE", "KapR", "Cg_L", "Cg_B", "Cg_R", "Cg_T")
+
+
+def write_records(path: str, arrays, append: bool = False) -> None:
+    """Fortran unformatted sequential records of FP64 arrays (Fortran order), by the
+    library's writer (host only)."""
+    arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F")) for a in arrays]
+    n = len(arrs)
+    ptrs = (_dp * n)(*[a.ctypes.data_as(_dp) for a in arrs])
+    cnt = (C.c_size_t * n)(*[a.size for a in arrs])
+    _io_check(load().trt_frecords_write(os.fsencode(path), n, ptrs, cnt, int(bool(append))), "write_records")
+
+
+def read_records(path: str, sizes):
+    """The first len(sizes) records of a Fortran unformatted sequential file, record k
+    holding exactly sizes[k] doubles, by the library's reader (host only)."""
+    arrs = [np.empty(int(k), dtype=np.float64) for k in sizes]
+    n = len(arrs)
+    ptrs = (_dp * n)(*[a.ctypes.data_as(_dp) for a in arrs])
+    cnt = (C.c_size_t * n)(*[a.size for a in arrs])
+    _io_check(load().trt_frecords_read(os.fsencode(path), n, ptrs, cnt), "read_records")
+    return arrs
 
 
 def autotune_shapes(precision: int):

@@ -15,7 +15,9 @@ import argparse
 import importlib
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -141,6 +143,42 @@ def measure_steps(trt, precision: int, steps: int = 10):
             "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps"}
 
 
+def measure_restart(trt):
+    """Checkpoint and resume (RESTART_OUT / RESTART_IN records) of the 128x128, S16,
+    32-group FP64 state after one time step: 4.8 GB in 25 records, I_crn and RT_Src
+    (2.4 GB each) split into subrecords.  The file goes to a temporary directory
+    (TMPDIR) and is deleted afterwards; the resumed context's arrays are compared
+    with the writer's."""
+    import numpy as np
+    prob = trt.problem.mapped(nx=128, order=16, ng=32, sigma=0.0, nsteps=2)
+    d = tempfile.mkdtemp(prefix="trt_restart_")
+    f = os.path.join(d, "state.rst")
+    a = b = None
+    try:
+        a = trt.native.Context(prob)
+        a.step()
+        t0 = time.perf_counter()
+        a.restart_out(f)
+        tw = time.perf_counter() - t0
+        size = os.path.getsize(f)
+        b = trt.native.Context(prob)
+        t0 = time.perf_counter()
+        b.restart_in(f)
+        tr = time.perf_counter() - t0
+        same = bool(all(np.array_equal(a.pull(n), b.pull(n)) for n in ("I_crn", "Temp", "Eg_avg", "KapE", "Cg_T")))
+    finally:
+        for x in (a, b):
+            if x is not None:
+                x.close()
+        shutil.rmtree(d, ignore_errors=True)
+    return {"problem": "128x128 cells, S16, 32 groups, FP64 context after one time step", "file_bytes": size,
+            "write_s": tw, "read_s": tr, "write_gbs": size / tw / 1e9, "read_gbs": size / tr / 1e9,
+            "resumed_equal": same,
+            "note": "write = device-to-host pulls + reference layout + buffered file writes (no fsync); read = "
+                    "whole-file check, then host-to-device pushes; the read follows the write, so it is served "
+                    "from the page cache"}
+
+
 def cpu_baseline(trt, budget_s: float = 12.0):
     """The FP64 CPU restatement (oracle) of TRANSPORT_SCB + COLLAPSE_INTENSITIES on the same
     256x256 S16 grid, OpenMP over groups as the reference, one group per thread."""
@@ -183,6 +221,7 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-problem batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop load")
     ap.add_argument("--no-steps", action="store_true", help="skip the full MLQD time steps")
+    ap.add_argument("--no-restart", action="store_true", help="skip the checkpoint/resume measurement")
     ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -274,6 +313,12 @@ def main(argv=None):
         out["full_step"] = {"problem": "256x256 cells, S16, 32 groups 1e-3..1e2 keV, uniform material, "
                                        "dt 2e-3 ns, QD systems by nested dissection",
                             "fp64": measure_steps(trt, 0), "fp32": measure_steps(trt, 1)}
+    if not args.no_restart and not args.only:
+        print("[bench] checkpoint and resume (128x128, S16, 32 groups)", file=sys.stderr, flush=True)
+        try:
+            out["restart"] = measure_restart(trt)
+        except (trt.native.TrtError, OSError) as e:     # e.g. a temporary directory too small for 4.8 GB
+            out["restart"] = {"error": str(e)}
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(trt)

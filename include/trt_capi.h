@@ -127,6 +127,37 @@ int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *
 int trt_step(trt_ctx *ctx, int its_out[4]);
 double trt_time(const trt_ctx *ctx);
 
+/* Checkpoint and resume.  trt_restart_out ≙ RESTART_OUT, MainSrc/OUTPUTS.f08:1624-1674
+ * (called every restart_freq-th step, ALGORITHMS.f08:889-894); trt_restart_in ≙
+ * RESTART_IN, MainSrc/INPUTS.f08:1734-1783 (before the time loop, ALGORITHMS.f08:234-237).
+ * The file is the reference's: 25 Fortran unformatted sequential records in its order
+ * (Time, Temp, I_crn, fg_avg_xx, fg_avg_yy, fg_edgV_xx, fg_edgV_xy, fg_edgH_yy,
+ * fg_edgH_xy, RT_Src, MGQD_Src, GQD_Src, E_avg, Eg_avg, Eg_edgV, Eg_edgH, Fxg_edgV,
+ * Fyg_edgH, KapE_Bar, KapE, KapR, Cg_L, Cg_B, Cg_R, Cg_T), FP64, Fortran shapes;
+ * 4-byte length markers, records above 2147483639 bytes split into subrecords
+ * (negated markers) as gfortran does; TRT_SUBRECORD_LENGTH lowers the split size.
+ * RT_Src is written in the reference's shape (2N_x,2N_y,N_m,N_g), the same value at
+ * every corner and direction; trt_restart_in rejects a file whose RT_Src is not of
+ * that form.  trt_restart_in reads and checks the whole file (record lengths against
+ * the context's grid, quadrature and group count) before it changes the context: on
+ * an error the context is unchanged.  Intensities of an FP32 context are written as
+ * their exact FP64 values, so an FP32 checkpoint resumes bitwise. */
+int trt_restart_out(trt_ctx *ctx, const char *path);
+int trt_restart_in(trt_ctx *ctx, const char *path);
+/* RESTART_TIME_IN, MainSrc/INPUTS.f08:1785-1803: the first record (Time) of a
+ * restart file, without a context.  Errors: trt_frecords_last_error(). */
+int trt_restart_time_in(const char *path, double *time);
+/* Host-only Fortran unformatted sequential records of FP64 arrays, the framing of
+ * the restart file (no device, no context; no counterpart in the reference, which
+ * uses its Fortran runtime's WRITE/READ).  write: nrec records, record k holding
+ * count[k] doubles from data[k], to a new file (append = 0) or after the end of
+ * an existing one.  read: the first nrec records of the file, each of which must
+ * hold exactly count[k] doubles.  Both return 0 or a negative status with
+ * trt_frecords_last_error() (thread-local). */
+int trt_frecords_write(const char *path, int nrec, const double *const *data, const size_t *count, int append);
+int trt_frecords_read(const char *path, int nrec, double *const *data, const size_t *count);
+const char *trt_frecords_last_error(void);
+
 /* Sweep timing: runs nsweeps sweeps on the context's stream and returns the
  * total elapsed milliseconds measured with HIP events on that stream. */
 int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms);
