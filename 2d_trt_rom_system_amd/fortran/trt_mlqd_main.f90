@@ -7,7 +7,10 @@
 ! state in host arrays as the reference keeps it.
 ! Mode 2 (device-resident): the same step on the GPU through trt_step, state kept on the device between steps.
 !
-! Usage: trt_mlqd <problem.bin> <results.bin>
+! Usage: trt_mlqd <problem.bin> <results.bin> [<restart_infile> [<restart_outfile> <restart_freq>]]
+!   mode 2 only: the state is read from restart_infile before the time loop ('-' for none; RESTART_IN, ALGORITHMS.f08:234-237)
+!   and written to restart_outfile every restart_freq-th step (RESTART_OUT, ALGORITHMS.f08:889-894), in the reference's
+!   restart file format.
 !   problem.bin (stream, native endianness): int32 nx ny nm ng nmat nsteps opacity_model maxit_rte maxit_mloqd maxit_gloqd
 !   conv_type use_line_search kapE_dT_flag bc_type(4) precision mode; float64 c dt theta comp_unit cv kap0 tini bcT(4)
 !   conv_ho conv_lo conv_gr1 conv_gr2 chi line_src; float64 Delx(nx) Dely(ny) Omega_x(nm) Omega_y(nm) quad_weight(nm)
@@ -31,7 +34,8 @@ PROGRAM trt_mlqd
   REAL*8:: c, Delt, Theta, Comp_Unit, cv, Kap0, Tini, bcT(4), conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src
   REAL*8,ALLOCATABLE:: Delx(:), Dely(:), Omega_x(:), Omega_y(:), quad_weight(:), Nu_g(:), Kappa_Mult(:), A(:,:)
   INTEGER,ALLOCATABLE:: Mat(:,:)
-  CHARACTER(LEN=1024):: fin, fout
+  CHARACTER(LEN=1024):: fin, fout, restart_infile, restart_outfile, freq_arg
+  INTEGER:: restart_freq
   INTEGER:: t, u, v, i, j
   INTEGER(C_INT):: its4(4)
 
@@ -72,6 +76,16 @@ PROGRAM trt_mlqd
 
   CALL GET_COMMAND_ARGUMENT(1, fin)
   CALL GET_COMMAND_ARGUMENT(2, fout)
+  restart_infile = ''
+  restart_outfile = ''
+  restart_freq = 0
+  IF (COMMAND_ARGUMENT_COUNT() .GE. 3) CALL GET_COMMAND_ARGUMENT(3, restart_infile)
+  IF (TRIM(restart_infile) .EQ. '-') restart_infile = ''
+  IF (COMMAND_ARGUMENT_COUNT() .GE. 5) THEN
+    CALL GET_COMMAND_ARGUMENT(4, restart_outfile)
+    CALL GET_COMMAND_ARGUMENT(5, freq_arg)
+    READ(freq_arg, *) restart_freq
+  END IF
   OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
   READ(u) N_x, N_y, N_m, N_g, N_mat, N_steps, opacity_model, Maxit_RTE, Maxit_MLOQD, Maxit_GLOQD, Conv_Type, &
     line_flag, dT_flag, BC_Type, precision, mode
@@ -99,6 +113,7 @@ PROGRAM trt_mlqd
     !------------------------------------------------------------------!
     !  device-resident: one trt_step per time step                     !
     !------------------------------------------------------------------!
+    IF (LEN_TRIM(restart_infile) .GT. 0) CALL TRT_GPU_RESTART_IN(restart_infile)
     DO t = 1, N_steps
       CALL TRT_CHECK(trt_step(TRT_CTX, its4), 'trt_step')
       CALL TRT_PULL_ARRAY('Temp', Temp, SIZE(Temp))
@@ -109,11 +124,15 @@ PROGRAM trt_mlqd
       CALL TRT_PULL_ARRAY('fg_avg_yy', fg_avg_yy, SIZE(fg_avg_yy))
       CALL TRT_PULL_ARRAY('fg_avg_xy', fg_avg_xy, SIZE(fg_avg_xy))
       WRITE(v) INT(its4(1)), INT(its4(2)), Temp, E_avg, HO_E_avg, MGQD_E_avg, fg_avg_xx, fg_avg_yy, fg_avg_xy
+      IF (restart_freq .GT. 0) THEN
+        IF (MOD(t, restart_freq) .EQ. 0) CALL TRT_GPU_RESTART_OUT(restart_outfile)
+      END IF
     END DO
     CLOSE(v)
     CALL TRT_GPU_FINALIZE()
     STOP
   END IF
+  IF (LEN_TRIM(restart_infile) .GT. 0 .OR. restart_freq .GT. 0) ERROR STOP 'restart files need mode 2 (device-resident state)'
 
   !--------------------------------------------------------------------!
   !  drop-in: host arrays and the reference call sequence              !

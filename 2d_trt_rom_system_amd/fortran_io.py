@@ -49,13 +49,21 @@ def read_results(path: str, prob):
     return out
 
 
-def run(prob, workdir: str, mode: int = DROPIN, precision: int = 0, timeout: float = 600.0):
-    """Run the Fortran host program on prob; returns (per-step results, stdout)."""
+def run(prob, workdir: str, mode: int = DROPIN, precision: int = 0, timeout: float = 600.0,
+        restart_in: str | None = None, restart_out: str | None = None, restart_freq: int = 0):
+    """Run the Fortran host program on prob; returns (per-step results, stdout).  In mode
+    RESIDENT the state can be read from a restart file before the time loop (restart_in)
+    and written to one every restart_freq-th step (restart_out)."""
     os.makedirs(workdir, exist_ok=True)
     pin = os.path.join(workdir, "problem.bin")
     pout = os.path.join(workdir, "results.bin")
     write_problem(pin, prob, mode, precision)
-    r = subprocess.run([EXE, pin, pout], capture_output=True, text=True, timeout=timeout)
+    args = [EXE, pin, pout]
+    if restart_in or restart_freq > 0:
+        args.append(restart_in or "-")
+    if restart_freq > 0:
+        args += [restart_out, str(int(restart_freq))]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"trt_mlqd failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
     return read_results(pout, prob), r.stdout

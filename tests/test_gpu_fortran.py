@@ -25,3 +25,27 @@ def test_fortran_driver_matches_oracle(trt, oracle_mod, tmp_path, mode):
         for n in ("Temp", "E_avg", "HO_E_avg"):
             assert _l2(res[s][n], o[n]) < 1e-8, (mode, s, n)
         assert np.max(np.abs(res[s]["fg_avg_xx"] - o["fg_avg_xx"])) < 1e-8
+
+
+def test_fortran_driver_restart_resumes_bitwise(trt, tmp_path):
+    """The Fortran host program checkpoints through TRT_GPU_RESTART_OUT where the reference
+    calls RESTART_OUT and resumes through TRT_GPU_RESTART_IN where it calls RESTART_IN:
+    4 steps == 2 steps + checkpoint, then 2 steps from the checkpoint, bitwise."""
+    import oracle.restart as orst
+    fio = __import__("importlib").import_module("2d_trt_rom_system_amd.fortran_io")
+    prob = trt.problem.cfg0()
+    prob.use_line_search = 0
+    prob.conv_ho, prob.conv_lo, prob.conv_gr1 = 1e-11, 1e-12, 1e-13
+    prob.nsteps = 4
+    full, _ = fio.run(prob, str(tmp_path / "full"), mode=fio.RESIDENT)
+    rst = str(tmp_path / "state.rst")
+    prob.nsteps = 2
+    first, _ = fio.run(prob, str(tmp_path / "a"), mode=fio.RESIDENT, restart_out=rst, restart_freq=2)
+    t, temp = orst.read_restart(rst, [(1,), (prob.nx, prob.ny)])
+    assert t[0] == pytest.approx(2 * prob.dt)
+    assert trt.native.restart_time_in(rst) == t[0]
+    second, _ = fio.run(prob, str(tmp_path / "b"), mode=fio.RESIDENT, restart_in=rst)
+    assert np.array_equal(temp, first[1]["Temp"])
+    for s, res in enumerate(first + second):
+        for k, v in full[s].items():
+            assert np.array_equal(np.asarray(res[k]), np.asarray(v)), (s, k)
