@@ -1,0 +1,429 @@
+"""Reader of the reference's input deck (input/input.inp) into a Problem.
+
+Follows INPUT (MainSrc/INPUTS.f08:10-166): the blocks [RUN_STATE] (:220-306),
+[SOLVER_OPTS] (:307-612), [PARAMETERS] (:870-1146), [OUTPUT_OPTS] (:1147-1435) and,
+for enrgy_strc custom, [ENERGY_STRUCTURE] (INPUT_ENERGY :1517-1577), each found as
+LOCATE_BLOCK finds it (:168-214: the first line whose first word is the block name;
+lines whose first word starts with '*', '!' or '#' are comments); then the derived
+quantities of INPUT (uniform Delx/Dely, N_t, C_v from bcT_left), the material layouts
+of INPUT_TEST_TYPE (:1676-1732), the group structures of INPUT_ENERGY and the
+quadrature sets of INPUT_QUAD (problem.named_quadrature).  Blocks the hot path does not
+use ([POD_OPTS], [DMD_OPTS], [MISC_OPTS]: ROM and file-driven runs) are skipped.
+
+Values are read as the reference's list-directed READs read them: the first word is the
+key, the following words its arguments; Fortran exponents (1d-12) are accepted.  An
+out-of-range value is reset to the reference's fallback and the message the reference
+prints is kept in Deck.warnings.
+
+Deliberate differences, each listed in Deck.warnings when it applies:
+  * time units: 1 sh = 10 ns = 1e-8 s, so 'ns' values are divided by 10 and 'sec' values
+    multiplied by 1e8; the reference multiplies ns by 10 and divides sec by 1e8
+    (:987-990, :1007-1010), which makes '2e-3 ns' 2e-2 sh, 100 times too long.
+  * E_low_bnd / T_low_bnd: the reference stores T_low_bnd's value in E_Bound_Low
+    (:571-579); it is reproduced, since the GPU path rejects the safety search either way.
+"""
+from __future__ import annotations
+
+import math
+import re
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import problem as P
+
+H_PLANCK = 6.62613e-19          # erg*sh (MainSrc/main.f08:18)
+ERG = 6.24150934e11             # eV per erg (main.f08:21)
+
+RUN_TYPES = ("mlqd", "tr_no_qd", "mg_pod", "mg_dmd", "gr_pod", "p1", "p13", "diff", "fld", "Tgen_qdf", "qdf_in")
+
+
+class DeckError(ValueError):
+    pass
+
+
+def _words(line: str) -> List[str]:
+    # list-directed input: blanks and commas separate values, '/' ends the record
+    line = line.split("/")[0] if "/" in line and not line.strip().startswith("[") else line
+    return [w for w in re.split(r"[\s,]+", line.strip()) if w]
+
+
+def _real(s: str) -> float:
+    return float(s.replace("d", "e").replace("D", "e"))
+
+
+def _int(s: str) -> int:
+    return int(s)
+
+
+def _is_comment(word: str) -> bool:
+    return word[:1] in ("*", "!", "#")
+
+
+def locate_block(lines: List[str], block: str) -> Optional[int]:
+    """Index of the line after the block's header, or None (LOCATE_BLOCK)."""
+    for k, line in enumerate(lines):
+        w = _words(line)
+        if not w or _is_comment(w[0]):
+            continue
+        if w[0] == block:
+            return k + 1
+    return None
+
+
+def block_entries(lines: List[str], block: str):
+    """(key, args) pairs of a block, up to the next '[...]' line; None if absent."""
+    k = locate_block(lines, block)
+    if k is None:
+        return None
+    out = []
+    for line in lines[k:]:
+        w = _words(line)
+        if not w:
+            continue
+        if w[0].startswith("["):
+            break
+        out.append((w[0], w[1:]))
+    return out
+
+
+@dataclass
+class Deck:
+    run_type: str = "mlqd"
+    restart_infile: str = ""
+    use_grey: int = 1
+    res_calc: bool = False
+    test: str = "FC"
+    maxit_rte: int = 25
+    maxit_mloqd: int = 500
+    maxit_gloqd: int = 100
+    kapE_dT_flag: bool = True
+    chi: float = 0.7
+    theta: float = 1.0
+    conv_ho: float = 1e-8
+    conv_lo: float = 1e-9
+    conv_gr1: float = 1e-10
+    conv_gr2: float = 1e-15
+    conv_type: int = 1
+    quadrature: str = "abu36"
+    threads: int = 1
+    enrgy_strc: str = "JCP"
+    comp_unit: float = 1e13
+    use_line_search: bool = True
+    line_src: float = 2.0
+    use_safety_search: bool = False
+    E_bound_low: float = -1.7976931348623157e308
+    T_bound_low: float = -1.7976931348623157e308
+    xlen: float = 6.0
+    ylen: float = 6.0
+    nx: int = 10
+    ny: int = 10
+    tlen: float = 0.0
+    delt: float = 0.0
+    bc_type: tuple = (0, 0, 0, 0)
+    bcT_left: float = 0.0
+    bcT_right: float = 0.0
+    bcT_upper: float = 0.0
+    bcT_lower: float = 0.0
+    tini: float = 0.0
+    outfile: str = "output.h5"
+    out_freq: int = 1
+    restart_outfile: str = "restart.out"
+    restart_freq: int = 1
+    init_out: int = 0
+    custom_edges: Optional[List[float]] = None
+    start_time: float = 0.0
+    warnings: List[str] = field(default_factory=list)
+
+    # ---- derived quantities of INPUT (INPUTS.f08:89-133) ----
+    @property
+    def n_t(self) -> int:
+        return int(round((self.tlen - self.start_time) / self.delt)) + (1 if self.init_out == 1 else 0)
+
+    def cv(self) -> float:
+        """C_v from bcT_left (INPUTS.f08:121), in erg/comp_unit per cm^3 per eV."""
+        sig_r = 2.0 * math.pi ** 5 / (15.0 * P.C_LIGHT ** 2 * H_PLANCK ** 3 * ERG ** 4 * self.comp_unit)
+        ar = 4.0 * sig_r / P.C_LIGHT
+        return 0.5917 * ar * self.bcT_left ** 3 * ((1.38e-16) ** 4 * (11600.0 ** 4) * ERG ** 4)
+
+    def group_edges(self) -> np.ndarray:
+        if self.enrgy_strc == "custom":
+            if self.custom_edges is None:
+                raise DeckError("[ENERGY_STRUCTURE] block was not located in input file")
+            return np.array(self.custom_edges)
+        key = {"jcp": "JCP", "ne795": "NE795", "one_group": "ONE_GROUP"}.get(self.enrgy_strc.lower())
+        if key is None or self.enrgy_strc not in (key, key.lower()):
+            raise DeckError(f'Invalid "enrgy_strc" ({self.enrgy_strc})')
+        return np.array(P.GROUP_STRUCTURES[key])
+
+    def materials(self):
+        """(Mat, Kappa_Mult) of INPUT_TEST_TYPE (INPUTS.f08:1676-1732), Mat 0-based."""
+        nx, ny = self.nx, self.ny
+        dx, dy = self.xlen / nx, self.ylen / ny
+        if self.test in ("FC", "F-C", "fc", "f-c"):
+            return np.zeros((nx, ny), dtype=np.int32, order="F"), np.array([1.0])
+        if self.test in ("BLOCK", "BLK", "block", "blk"):
+            mat = np.zeros((nx, ny), dtype=np.int32, order="F")
+            for j in range(ny):
+                yc = (j + 0.5) * dy
+                for i in range(nx):
+                    xc = (i + 0.5) * dx
+                    inside_x = not (xc < self.xlen / 2 - 1.2 or xc > self.xlen / 2 + 1.2)
+                    inside_y = not (yc < self.ylen / 2 - 1.2 or yc > self.ylen / 2 + 1.2)
+                    mat[i, j] = 1 if (inside_x and inside_y) else 0
+            return mat, np.array([1.0, 10.0])
+        raise DeckError(f"Unrecognized Test type ({self.test})")
+
+    def problem(self, nsteps: Optional[int] = None) -> P.Problem:
+        """The Problem of the deck (reference code units: cm, sh, eV, erg/comp_unit)."""
+        nu = self.group_edges()
+        mat, kmult = self.materials()
+        p = P.Problem(nx=self.nx, ny=self.ny, order=0, ng=len(nu) - 1, nu=nu, xlen=self.xlen, ylen=self.ylen,
+                      dt=self.delt, nsteps=self.n_t if nsteps is None else nsteps, theta=self.theta, cv=self.cv(),
+                      tini=self.tini, bc_type=tuple(self.bc_type),
+                      bcT=(self.bcT_left, self.bcT_lower, self.bcT_right, self.bcT_upper), mat=mat, kmult=kmult,
+                      opacity_model=0, comp_unit=self.comp_unit, maxit_rte=self.maxit_rte,
+                      maxit_mloqd=self.maxit_mloqd, maxit_gloqd=self.maxit_gloqd, conv_type=self.conv_type,
+                      conv_ho=self.conv_ho, conv_lo=self.conv_lo, conv_gr1=self.conv_gr1, conv_gr2=self.conv_gr2,
+                      chi=self.chi, line_src=self.line_src, use_line_search=int(self.use_line_search),
+                      use_safety_search=int(self.use_safety_search), kapE_dT_flag=int(self.kapE_dT_flag),
+                      E_bound_low=self.E_bound_low, T_bound_low=self.T_bound_low, threads=self.threads,
+                      name=f"deck:{self.test}", quadrature=self.quadrature)
+        return p
+
+
+def _unit_value(d: Deck, key: str, args: List[str], scale: Dict[str, float], default: str) -> float:
+    v = _real(args[0])
+    unit = args[1] if len(args) > 1 else ""
+    if unit in scale:
+        return v * scale[unit]
+    if unit == "" or _is_comment(unit):
+        d.warnings.append(f'NO UNITS GIVEN FOR "{key}", ASSUMING {default}')
+        return v
+    raise DeckError(f'UNKNOWN UNITS ({unit}) GIVEN FOR "{key}"; acceptable units are [{", ".join(scale)}]')
+
+
+LENGTH = {"cm": 1.0, "m": 100.0, "mm": 0.1}
+TIME = {"sh": 1.0, "ns": 0.1, "sec": 1e8}
+TEMP = {"ev": 1.0, "erg": ERG}
+
+
+def read_deck(text: str, restart_time=None) -> Deck:
+    """Parse the text of an input deck.  restart_time(path) -> Time of a restart file
+    (RESTART_TIME_IN); needed only when the deck names one."""
+    lines = text.splitlines()
+    d = Deck()
+
+    ent = block_entries(lines, "[RUN_STATE]")
+    if ent is None:
+        raise DeckError("[RUN_STATE] block was not located in input file")
+    for key, a in ent:
+        if key == "run_type":
+            if a[0] not in RUN_TYPES:
+                raise DeckError(f"unrecognized run_type ({a[0]})")
+            d.run_type = a[0]
+        elif key == "Test":
+            d.test = a[0]
+        elif key == "restart":
+            d.restart_infile = a[0]
+        elif key == "use_grey":
+            d.use_grey = _int(a[0])
+        elif key == "res_calc":
+            d.res_calc = _int(a[0]) == 1
+
+    ent = block_entries(lines, "[SOLVER_OPTS]")
+    if ent is None:
+        raise DeckError("[SOLVER_OPTS] block was not located in input file")
+    flags = {"conv_ho": 0, "conv_lo": 0, "conv_gr": 0}
+    bnd = 0
+    for key, a in ent:
+        if key in ("maxit_RTE", "maxit_MLOQD", "maxit_GLOQD"):
+            v = _int(a[0])
+            if v <= 0:
+                v = {"maxit_RTE": 15, "maxit_MLOQD": 100, "maxit_GLOQD": 100}[key]
+                d.warnings.append(f"{key} must be a positive integer; reset to {v}")
+            if key == "maxit_RTE":
+                d.maxit_rte = v
+            elif key == "maxit_MLOQD":
+                d.maxit_mloqd = v
+            else:
+                d.maxit_gloqd = v
+        elif key == "chi":
+            v = _real(a[0])
+            if v < 0 or v >= 1:
+                d.warnings.append("chi must be in [0, 1); reset to 0.7")
+                v = 0.7
+            d.chi = v
+        elif key == "theta":
+            v = _real(a[0])
+            if v < 0 or v > 1:
+                d.warnings.append("theta must be in (0, 1]; reset to 1")
+                v = 1.0
+            elif v == 0:
+                d.warnings.append("theta = 0 is replaced by 1e-5")
+                v = 1e-5
+            d.theta = v
+        elif key in ("conv_ho", "conv_lo"):
+            v = _real(a[0])
+            flags[key] = 1
+            if v <= 0 or v >= 1:
+                v = 1e-8 if key == "conv_ho" else 1e-9
+                d.warnings.append(f"{key} must be in (0, 1); reset to {v:g}")
+            setattr(d, key, v)
+        elif key == "conv_gr":
+            g1, g2 = _real(a[0]), _real(a[1])
+            flags["conv_gr"] = 1
+            if g1 <= 0 or g2 <= 0 or g1 >= 1 or g2 >= 1:
+                g1, g2 = 1e-10, 1e-15
+                d.warnings.append("conv_gr must be in (0, 1); reset to 1e-10 1e-15")
+            d.conv_gr1, d.conv_gr2 = g1, g2
+        elif key == "conv_type":
+            v = _int(a[0])
+            if v not in (1, 2, 3):
+                d.warnings.append("conv_type must be 1, 2 or 3; reset to 1")
+                v = 1
+            d.conv_type = v
+        elif key == "quad":
+            d.quadrature = a[0]
+        elif key == "threads":
+            v = _int(a[0])
+            d.threads = v if v > 0 else 1
+        elif key == "enrgy_strc":
+            d.enrgy_strc = a[0]
+        elif key == "kapE_dT":
+            if a[0] == "on":
+                d.kapE_dT_flag = True
+            else:
+                if a[0] != "off":
+                    d.warnings.append("kapE_dT must be on or off; set off")
+                d.kapE_dT_flag = False
+        elif key == "comp_unit":
+            v = _real(a[0])
+            d.comp_unit = v if v >= 1 else 1e13
+        elif key == "line_src":
+            if a[0] in ("off", "Off", "OFF"):
+                d.use_line_search, d.line_src = False, 1.0
+            else:
+                d.use_line_search, d.line_src = True, _real(a[0])
+        elif key in ("E_low_bnd", "T_low_bnd"):
+            bnd += 1
+            if a[0] in ("off", "Off", "OFF"):
+                if bnd == 1:
+                    d.use_safety_search = False
+                d.E_bound_low = -1.7976931348623157e308
+            else:
+                d.use_safety_search = True
+                d.E_bound_low = _real(a[0])
+    # the cascade of unset tolerances (INPUTS.f08:588-606)
+    if flags["conv_ho"]:
+        if flags["conv_lo"]:
+            if not flags["conv_gr"]:
+                d.conv_gr1 = d.conv_lo / 10.0
+        else:
+            d.conv_lo = d.conv_ho / 10.0
+            if not flags["conv_gr"]:
+                d.conv_gr1 = d.conv_ho / 100.0
+    else:
+        if flags["conv_lo"]:
+            d.conv_ho = d.conv_lo * 10.0
+            if not flags["conv_gr"]:
+                d.conv_gr1 = d.conv_lo / 10.0
+        elif flags["conv_gr"]:
+            d.conv_ho = d.conv_gr1 * 100.0
+            d.conv_lo = d.conv_gr1 * 10.0
+
+    if d.restart_infile:
+        if restart_time is None:
+            raise DeckError("the deck names a restart file; pass restart_time to read its Time record")
+        d.start_time = float(restart_time(d.restart_infile))
+
+    ent = block_entries(lines, "[PARAMETERS]")
+    if ent is None:
+        raise DeckError("[PARAMETERS] block was not located in input file")
+    for key, a in ent:
+        if key in ("xlen", "ylen"):
+            setattr(d, key, _unit_value(d, key, a, LENGTH, "cm"))
+        elif key in ("x_cells", "y_cells"):
+            v = _int(a[0])
+            if v <= 0:
+                d.warnings.append(f"{key} must be a positive integer; reset to 10")
+                v = 10
+            setattr(d, "nx" if key == "x_cells" else "ny", v)
+        elif key in ("tlen", "delt"):
+            if len(a) > 1 and a[1] in ("ns", "sec"):
+                d.warnings.append(f'"{key}" in {a[1]} converted with 1 sh = 10 ns = 1e-8 s (the reference '
+                                  f"inverts this conversion)")
+            setattr(d, key, _unit_value(d, key, a, TIME, "sh"))
+        elif key == "bc_type":
+            bt = tuple(_int(x) for x in a[:4])
+            if min(bt) < 0 or max(bt) > 2:
+                raise DeckError("bc_type values must be 0, 1 or 2")
+            d.bc_type = bt
+        elif key in ("bcT_left", "bcT_right", "bcT_upper", "bcT_lower", "Tini"):
+            setattr(d, "tini" if key == "Tini" else key, _unit_value(d, key, a, TEMP, "ev"))
+
+    ent = block_entries(lines, "[OUTPUT_OPTS]")
+    for key, a in ent or []:
+        if key == "outfile":
+            d.outfile = a[0]
+        elif key == "out_freq":
+            d.out_freq = _int(a[0])
+        elif key == "restart_outfile":
+            d.restart_outfile = a[0]
+        elif key == "restart_freq":
+            d.restart_freq = _int(a[0])
+        elif key == "Init_out":
+            d.init_out = _int(a[0])
+
+    if d.enrgy_strc == "custom":
+        k = locate_block(lines, "[ENERGY_STRUCTURE]")
+        if k is None:
+            raise DeckError("[ENERGY_STRUCTURE] block was not located in input file")
+        vals = []
+        for line in lines[k:]:
+            w = _words(line)
+            if w:
+                vals.append(w[0])
+        ng = _int(vals[0])
+        d.custom_edges = [0.0] + [_real(x) for x in vals[1:1 + ng]]     # nu_g(1) = 0 (INPUTS.f08:1539)
+    return d
+
+
+def read_deck_file(path: str, restart_time=None) -> Deck:
+    with open(path) as f:
+        return read_deck(f.read(), restart_time)
+
+
+def main(argv=None) -> int:
+    """python -m 2d_trt_rom_system_amd.deck input.inp [--problem-file problem.bin]: print the
+    deck's settings, or write the problem file the Fortran host program (trt_mlqd) reads."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("deck")
+    ap.add_argument("--problem-file", help="write trt_mlqd's problem file here")
+    ap.add_argument("--mode", type=int, default=2, help="1: drop-in call sequence, 2: device-resident step")
+    ap.add_argument("--steps", type=int, default=None, help="time steps (default: the deck's N_t)")
+    args = ap.parse_args(argv)
+
+    def restart_time(path):
+        from . import native
+        return native.restart_time_in(path)
+
+    d = read_deck_file(args.deck, restart_time)
+    for w in d.warnings:
+        print("warning:", w)
+    p = d.problem(nsteps=args.steps)
+    if args.problem_file:
+        from . import fortran_io
+        fortran_io.write_problem(args.problem_file, p, mode=args.mode)
+    print(json.dumps({"run_type": d.run_type, "test": d.test, "cells": [p.nx, p.ny], "directions": p.nm,
+                      "groups": p.ng, "dt_sh": p.dt, "steps": p.nsteps, "cv": p.cv, "restart_infile": d.restart_infile,
+                      "restart_outfile": d.restart_outfile, "restart_freq": d.restart_freq}))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

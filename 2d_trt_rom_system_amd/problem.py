@@ -41,6 +41,68 @@ _LQN = {
 }
 
 
+# ---------------------------------------------------------------------------
+# Named quadrature sets of the reference's input deck (INPUT_QUAD,
+# MainSrc/INPUTS.f08:1579-1674): first-quadrant direction cosines Omega_x and
+# weights of the Abu-Shumays sets abu36 and abu20 (36 and 20 directions per
+# quadrant) and of the 2-direction '1d_test' set.  In each set Omega_y is the
+# Omega_x list reversed and the weights read the same backwards.  The other
+# quadrants mirror the first (:1659-1668) and the weights are renormalised to
+# 4*pi (:1669), as level_symmetric() does.
+# ---------------------------------------------------------------------------
+_NAMED_QUAD = {
+    "abu36": (
+        [9.717784813336e-1, 7.656319455497e-1, 4.445439440056e-1, 9.701698603928e-1, 7.633693960835e-1,
+         1.483114568272e-1, 9.622473153642e-1, 7.524467626583e-1, 9.410672772109e-1, 4.288508824476e-1,
+         7.241384940891e-1, 8.997294996538e-1, 6.711819639118e-1, 1.293388490485e-1, 8.335743322378e-1,
+         3.670788892962e-1, 5.909368760506e-1, 7.417637460141e-1, 6.279014865859e-1, 4.869502395267e-1,
+         2.519357740235e-1, 4.996274255819e-1, 7.447663982495e-2, 3.685670882907e-1, 3.673697853806e-1,
+         2.488983098158e-1, 1.196054590036e-1, 2.426233944222e-1, 1.419535016004e-1, 1.362124657777e-1,
+         1.670387759191e-2, 5.995074957044e-2, 5.695764868253e-2, 2.447911451942e-2, 1.160393058611e-2,
+         1.096881837272e-2],
+        [8.454511187252e-3, 8.352354145856e-3, 1.460888798152e-2, 1.913728513580e-2, 1.873220073879e-2,
+         6.404244616724e-3, 2.863542971348e-2, 2.759429759588e-2, 3.648716160597e-2, 2.995376809966e-2,
+         3.442681426024e-2, 4.244873302980e-2, 3.901232700510e-2, 1.162080754372e-2, 4.642823955812e-2,
+         3.798783310581e-2, 4.130171453748e-2, 4.841339013884e-2, 4.841339013884e-2, 4.130171453748e-2,
+         3.798783310581e-2, 4.642823955812e-2, 1.162080754372e-2, 3.901232700510e-2, 4.244873302980e-2,
+         3.442681426024e-2, 2.995376809966e-2, 3.648716160597e-2, 2.759429759588e-2, 2.863542971348e-2,
+         6.404244616724e-3, 1.873220073879e-2, 1.913728513580e-2, 1.460888798152e-2, 8.352354145856e-3,
+         8.454511187252e-3]),
+    "abu20": (
+        [9.713274064903e-1, 7.645615896150e-1, 4.424202396002e-1, 9.586898685237e-1, 7.375714298063e-1,
+         1.409476441875e-1, 9.028558915298e-1, 3.858240341629e-1, 6.313311043797e-1, 7.770210099715e-1,
+         5.837054752370e-1, 4.332989313333e-1, 2.221674140412e-1, 3.596178122512e-1, 4.908227124734e-2,
+         2.057068622698e-1, 1.593344524838e-1, 4.982847370367e-2, 4.210110375297e-2, 3.157215799340e-2],
+        [2.419260514149e-2, 2.998205782366e-2, 2.932993043666e-2, 5.213067212540e-2, 6.147460425028e-2,
+         1.802505216045e-2, 7.185542471164e-2, 5.322055875020e-2, 7.796304620960e-2, 8.182604839076e-2,
+         8.182604839076e-2, 7.796304620960e-2, 5.322055875020e-2, 7.185542471164e-2, 1.802505216045e-2,
+         6.147460425028e-2, 5.213067212540e-2, 2.932993043666e-2, 2.998205782366e-2, 2.419260514149e-2]),
+    "1d_test": ([0.92388, 0.382683], [0.5, 0.5]),
+}
+
+
+def named_quadrature(name: str):
+    """(Omega_x, Omega_y, quad_weight) of a named set of the input deck (quad key)."""
+    if name not in _NAMED_QUAD:
+        raise ValueError(f"unrecognized quadrature {name!r} (abu36, abu20, 1d_test)")
+    ox, w = (np.array(v) for v in _NAMED_QUAD[name])
+    oy = ox[::-1].copy()
+    Ox = np.concatenate([ox, -ox, -ox, ox])
+    Oy = np.concatenate([oy, oy, -oy, -oy])
+    W = np.concatenate([w, w, w, w])
+    return Ox, Oy, 4.0 * math.pi * W / W.sum()
+
+
+# Group structures of the input deck (INPUT_ENERGY, MainSrc/INPUTS.f08:1517-1577): edges in eV.
+GROUP_STRUCTURES = {
+    "JCP": [0.0, 7.075e2, 1.415e3, 2.123e3, 2.830e3, 3.538e3, 4.245e3, 5.129e3, 6.014e3, 6.898e3, 7.783e3, 8.667e3,
+            9.551e3, 1.044e4, 1.132e4, 1.220e4, 1.309e4, 1e10],
+    "NE795": [0.0, 1.9047, 3.6278, 6.9097, 13.161, 25.067, 47.744, 90.937, 173.21, 329.90, 628.35, 1196.8, 2279.5,
+              4341.7, 8269.5, 15751.0, 3e4, 1e10],
+    "ONE_GROUP": [0.0, 1e10],
+}
+
+
 def level_symmetric(order: int):
     """Return (Omega_x, Omega_y, quad_weight) for a 2D level-symmetric S_N set.
 
@@ -163,6 +225,7 @@ class Problem:
     T_bound_low: float = -1.7976931348623157e308
     threads: int = 1
     name: str = "custom"
+    quadrature: Optional[str] = None  # a named set of the input deck (abu36, abu20, 1d_test) instead of LQn
 
     def __post_init__(self):
         self.nu = np.ascontiguousarray(self.nu, dtype=np.float64)
@@ -172,7 +235,7 @@ class Problem:
             self.kmult = np.ones(1)
         self.mat = np.asfortranarray(self.mat, dtype=np.int32)
         self.kmult = np.ascontiguousarray(self.kmult, dtype=np.float64)
-        ox, oy, w = level_symmetric(self.order)
+        ox, oy, w = named_quadrature(self.quadrature) if self.quadrature else level_symmetric(self.order)
         self.ox, self.oy, self.w = ox, oy, w
         self.dx = np.full(self.nx, self.xlen / self.nx)
         self.dy = np.full(self.ny, self.ylen / self.ny)

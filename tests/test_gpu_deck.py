@@ -1,0 +1,50 @@
+"""Time steps of problems read from input decks, GPU against the oracle: the reference's
+own deck (tests/golden/mainsrc_input.inp: Fleck-Cummings layout, abu36 quadrature, JCP
+groups, 1 keV left boundary, 10x10 cells) run as mlqd, and a BLOCK layout with abu20
+and NE795 groups."""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+
+pytestmark = pytest.mark.gpu
+
+D = importlib.import_module("2d_trt_rom_system_amd.deck")
+GOLDEN = f"{ROOT}/tests/golden/mainsrc_input.inp"
+
+
+def _l2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def _block_deck():
+    text = open(GOLDEN).read()
+    text = text.replace("Test\t\t\tFC", "Test BLOCK").replace("quad\t\tabu36", "quad abu20")
+    text = text.replace("enrgy_strc\tJCP", "enrgy_strc NE795")
+    d = D.read_deck(text)
+    assert (d.test, d.quadrature, d.enrgy_strc) == ("BLOCK", "abu20", "NE795")
+    return d
+
+
+@pytest.mark.parametrize("which,line_search,tol", [("reference", False, 1e-8), ("reference", True, 1e-5),
+                                                   ("block", False, 1e-8)])
+def test_deck_steps_match_oracle(trt, oracle_mod, which, line_search, tol):
+    d = D.read_deck_file(GOLDEN) if which == "reference" else _block_deck()
+    d.run_type = "mlqd"
+    prob = d.problem(nsteps=3)
+    if not line_search:
+        prob.use_line_search = 0
+    o = oracle_mod.Oracle(prob, threads=8)
+    ctx = trt.native.Context(prob)
+    for s in range(3):
+        it_o = o.step()
+        it_g = ctx.step()
+        for n in ("Temp", "E_avg", "HO_E_avg", "MGQD_E_avg"):
+            err = _l2(ctx.pull(n), o[n].ravel(order="F"))
+            assert err < tol, (s, n, err, it_o, it_g)
+        for n in ("fg_avg_xx", "fg_avg_yy", "fg_avg_xy"):
+            assert rel_err(ctx.pull(n), o[n].ravel(order="F")) < 10 * tol, (s, n)
+    assert ctx.time == pytest.approx(o.time) == pytest.approx(3 * d.delt)
+    ctx.close()
