@@ -57,7 +57,7 @@ def test_restart_out_holds_the_context_state(trt, tmp_path, precision):
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_resume_continues_bitwise(trt, tmp_path, precision):
     """N steps, checkpoint, M more == checkpoint read by a fresh context, M steps."""
     prob = _prob(trt)
@@ -162,6 +162,28 @@ def test_split_records_resume_bitwise(trt, tmp_path, monkeypatch):
     a.step()
     b.step()
     for n in WATCH:
+        assert np.array_equal(a.pull(n), b.pull(n)), n
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tgen_resume_bitwise(trt, tmp_path, precision):
+    """The sweep-only Tgen_QDf step (the served unit of work) resumes bitwise too."""
+    prob = _prob(trt)
+    f = str(tmp_path / "tgen.rst")
+    a = trt.native.Context(prob, precision=precision)
+    for _ in range(2):
+        a.tgen_step()
+    a.restart_out(f)
+    for _ in range(2):
+        a.tgen_step()
+    b = trt.native.Context(prob, precision=precision)
+    b.restart_in(f)
+    for _ in range(2):
+        b.tgen_step()
+    assert a.time == b.time
+    for n in ("I_crn", "fg_avg_xx", "fg_avg_xy", "fg_edgV_xy", "fg_edgH_yy", "Cg_L", "Cg_T", "HO_Eg_avg", "KapE"):
         assert np.array_equal(a.pull(n), b.pull(n)), n
     a.close()
     b.close()
