@@ -143,6 +143,42 @@ def measure_steps(trt, precision: int, steps: int = 10):
             "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps"}
 
 
+def measure_dropin_sweep(trt, prob, reps: int = 2):
+    """TRANSPORT_SCB through the host-array boundary, as the Fortran drop-in shim calls it
+    (fortran/trt_dropin.f90): push KapE, RT_Src and I_crn_old, sweep, pull I_crn and the
+    five intensity views.  FP64; the host buffers are allocated once, as a caller's arrays
+    are, and hold the context's own values."""
+    import ctypes as C
+    import numpy as np
+    ins = ("KapE", "RT_Src", "I_crn_old")
+    outs = ("I_crn", "I_avg", "I_edgV", "I_edgH", "Ic_edgV", "Ic_edgH")
+    ctx = trt.native.Context(prob, precision=0)
+    try:
+        L, dp = ctx.L, C.POINTER(C.c_double)
+        bufs = {n: ctx.pull(n) for n in ins}
+        bufs.update({n: np.zeros(ctx.size(n)) for n in outs})
+
+        def call():
+            for n in ins:
+                ctx._check(L.trt_push(ctx.h, n.encode(), bufs[n].ctypes.data_as(dp), bufs[n].size), "push " + n)
+            ctx._check(L.trt_transport_scb(ctx.h), "transport_scb")
+            for n in outs:
+                ctx._check(L.trt_pull(ctx.h, n.encode(), bufs[n].ctypes.data_as(dp), bufs[n].size), "pull " + n)
+
+        call()                                      # first touch of the host buffers
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        dt = (time.perf_counter() - t0) / reps
+        moved = 8 * sum(bufs[n].size for n in ins + outs)
+    finally:
+        ctx.close()
+    return {"s_per_sweep": dt, "updates_per_s": prob.updates_per_sweep() / dt, "host_bytes_per_sweep": moved,
+            "host_gbs": moved / dt / 1e9,
+            "note": "per call: push KapE, RT_Src (per cell and group), I_crn_old; one FP64 sweep; pull I_crn, "
+                    "I_avg, I_edgV, I_edgH, Ic_edgV, Ic_edgH into pageable host arrays"}
+
+
 def measure_restart(trt):
     """Checkpoint and resume (RESTART_OUT / RESTART_IN records) of the 128x128, S16,
     32-group FP64 state after one time step: 4.8 GB in 25 records, I_crn and RT_Src
@@ -222,6 +258,7 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop load")
     ap.add_argument("--no-steps", action="store_true", help="skip the full MLQD time steps")
     ap.add_argument("--no-restart", action="store_true", help="skip the checkpoint/resume measurement")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-array (PCIe-inclusive) sweep")
     ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -313,6 +350,9 @@ def main(argv=None):
         out["full_step"] = {"problem": "256x256 cells, S16, 32 groups 1e-3..1e2 keV, uniform material, "
                                        "dt 2e-3 ns, QD systems by nested dissection",
                             "fp64": measure_steps(trt, 0), "fp32": measure_steps(trt, 1)}
+    if not args.no_dropin and not args.only:
+        print("[bench] drop-in TRANSPORT_SCB with host arrays (PCIe-inclusive)", file=sys.stderr, flush=True)
+        out["dropin_sweep"] = measure_dropin_sweep(trt, prob)
     if not args.no_restart and not args.only:
         print("[bench] checkpoint and resume (128x128, S16, 32 groups)", file=sys.stderr, flush=True)
         try:
