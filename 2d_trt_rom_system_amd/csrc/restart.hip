@@ -65,7 +65,13 @@ public:
     {
         FILE *f = f_;
         f_ = nullptr;
-        if (fclose(f) != 0) throw Error("closing " + path_ + " failed: " + strerror(errno));
+        if (f && fclose(f) != 0) throw Error("closing " + path_ + " failed: " + strerror(errno));
+    }
+    // closes without reporting errors (the file is being discarded)
+    void abandon()
+    {
+        if (f_) fclose(f_);
+        f_ = nullptr;
     }
     const std::string &path() const { return path_; }
 
@@ -74,9 +80,25 @@ private:
     FILE *f_ = nullptr;
 };
 
+// A new file is written under path + ".tmp" and renamed over path only once it is
+// complete, so a failed write (a full disk, a bad array) leaves an existing file
+// at path as it was; appending writes to path itself.
 class RecordWriter {
 public:
-    RecordWriter(const char *path, bool append) : file_(path, append ? "ab" : "wb"), max_(subrecord_limit()) {}
+    RecordWriter(const char *path, bool append)
+        : final_(path ? path : ""), tmp_(append || final_.empty() ? final_ : final_ + ".tmp"),
+          max_(subrecord_limit()), file_(tmp_.c_str(), append ? "ab" : "wb")
+    {
+    }
+    ~RecordWriter()
+    {
+        if (!done_ && tmp_ != final_) {
+            file_.abandon();
+            remove(tmp_.c_str());
+        }
+    }
+    RecordWriter(const RecordWriter &) = delete;
+    RecordWriter &operator=(const RecordWriter &) = delete;
     // Starts a record of `bytes` data bytes; put() must then supply exactly that many.
     void begin(size_t bytes)
     {
@@ -102,6 +124,9 @@ public:
     {
         if (open_) throw Error("file closed inside a record");
         file_.close();
+        if (tmp_ != final_ && rename(tmp_.c_str(), final_.c_str()) != 0)
+            throw Error("renaming " + tmp_ + " to " + final_ + " failed: " + strerror(errno));
+        done_ = true;
     }
 
 private:
@@ -122,10 +147,11 @@ private:
         first_ = false;
         if (left_ > 0) open_sub();
     }
+    std::string final_, tmp_;
+    size_t max_;       // checked before the file is created
     File file_;
-    size_t max_;
     size_t left_ = 0, sub_ = 0, sub_left_ = 0;
-    bool first_ = true, open_ = false;
+    bool first_ = true, open_ = false, done_ = false;
 };
 
 class RecordReader {

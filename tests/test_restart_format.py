@@ -180,6 +180,7 @@ def test_reader_errors(trt, tmp_path, monkeypatch):
     monkeypatch.setenv("TRT_SUBRECORD_LENGTH", "0")
     with pytest.raises(nat.TrtError, match="TRT_SUBRECORD_LENGTH"):
         nat.write_records(str(f), [np.arange(2.0)])
+    assert f.read_bytes() == raw and not (tmp_path / "e.bin.tmp").exists()
 
 
 def test_oracle_restart_arrays_order_and_rt_src(oracle_mod, tmp_path):
@@ -203,3 +204,21 @@ def test_oracle_restart_arrays_order_and_rt_src(oracle_mod, tmp_path):
     back = orst.read_restart(f, [a.shape for a in arrs])
     for a, b in zip(arrs, back):
         assert np.array_equal(a, b)
+
+
+def test_failed_write_keeps_the_existing_file(trt, tmp_path):
+    """A write that fails part-way (here: record 2 has no data) leaves the file that was
+    there untouched and no temporary file behind."""
+    import ctypes as C
+    nat = trt.native
+    f = tmp_path / "keep.bin"
+    nat.write_records(str(f), [np.arange(3.0)])
+    before = f.read_bytes()
+    a = np.arange(5.0)
+    dp = C.POINTER(C.c_double)
+    ptrs = (dp * 2)(a.ctypes.data_as(dp), dp())
+    cnt = (C.c_size_t * 2)(5, 4)
+    assert nat.load().trt_frecords_write(str(f).encode(), 2, ptrs, cnt, 0) != 0
+    assert "record 2 has no data" in nat.load().trt_frecords_last_error().decode()
+    assert f.read_bytes() == before
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["keep.bin"]
