@@ -268,6 +268,11 @@ int trt_step(trt_ctx *ctx, int its_out[4])
     });
 }
 
+int trt_step_given_qdf(trt_ctx *ctx, int its_out[4])
+{
+    return guarded(ctx, [&](Ctx &c) { trt::step(c, its_out, 1); });
+}
+
 double trt_time(const trt_ctx *ctx) { return ctx ? ctx->c.time : 0.0; }
 
 int trt_restart_out(trt_ctx *ctx, const char *path)

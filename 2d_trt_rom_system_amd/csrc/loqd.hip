@@ -1339,7 +1339,7 @@ static void cp(Ctx &c, const char *dst, const char *src)
     dcopy(c, d.p, c.arr(src), d.size());
 }
 
-int step(Ctx &c, int its_out[4])
+int step(Ctx &c, int its_out[4], int ho_form)
 {
     const int nx = c.nx, ny = c.ny;
     const size_t nc = (size_t)nx * ny;
@@ -1361,13 +1361,22 @@ int step(Ctx &c, int its_out[4])
     int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0;
     bool rt_conv = false;
     const int rt_start = 2;                  // AL:302
-    while (!rt_conv && rt_its < c.p.maxit_rte) {
+    // ho_form 1 (the ROM runs, HO_Form 1 and 3, AL:573-631): in each RT iteration the
+    // QD factors and C_g of the new time level come from the caller's reconstruction
+    // (pushed to the *_next arrays) where POD/DMD_RECONSTRUCT_fg would write them; one RT
+    // iteration, as INPUT sets maxit_RTE = 1 for 'fg' ROM runs (IP:94-104)
+    const int maxit_rte = ho_form ? 1 : c.p.maxit_rte;
+    while (!rt_conv && rt_its < maxit_rte) {
         rt_its++;
-        if (rt_its >= rt_start) {
+        if (!ho_form && rt_its >= rt_start) {
             transport_sweep(c);              // AL:541-542
             sweeps++;
             collapse_with_fg(c);             // AL:551-553 and fg_Calc AL:568-569
             cg_calc(c);                      // AL:572
+        } else if (ho_form) {
+            for (auto n : {"fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy", "Cg_L",
+                           "Cg_B", "Cg_R", "Cg_T"})
+                cp(c, n, (std::string(n) + "_next").c_str());   // AL:582-593
         }
         dfill(c, c.arr("MGQD_E_avg_MGQDold2"), 0.0, nc);   // AL:676-679
         cp(c, "MGQD_E_avg_MGQDold", "HO_E_avg");

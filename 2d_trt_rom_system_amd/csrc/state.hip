@@ -246,7 +246,7 @@ size_t state_bytes(const trt_problem &p)
 {
     const size_t na = (size_t)p.nx * p.ny, nv = (size_t)(p.nx + 1) * p.ny, nh = (size_t)p.nx * (p.ny + 1);
     const size_t nbd = (size_t)p.nx + p.ny;
-    size_t n = (25 * na + 12 * nv + 12 * nh + 6 * nbd) * p.ng;       // per-group arrays of init_state
+    size_t n = (27 * na + 14 * nv + 14 * nh + 8 * nbd) * p.ng;       // per-group arrays of init_state
     n += 40 * na + 6 * nv + 6 * nh + 6 * nbd + 3 * (size_t)p.nm + p.ng + 1 + p.nmat + p.nx + p.ny;
     n += std::max<size_t>(4096, 8 * (size_t)std::max(p.nx, p.ny) * p.ng + 4096) + 4 * (size_t)p.ng;   // scratch, bcin
     return n * 8 + na * sizeof(int);
@@ -435,6 +435,12 @@ void init_state(Ctx &c)
     for (auto n : hg) c.reg(n, nx, ny + 1, ng);
     for (auto n : {"Cg_L", "Cg_R", "Eg_in_L", "Eg_in_R", "Fg_in_L", "Fg_in_R"}) c.reg(n, ny, ng);
     for (auto n : {"Cg_B", "Cg_T", "Eg_in_B", "Eg_in_T", "Fg_in_B", "Fg_in_T"}) c.reg(n, nx, ng);
+    // the next time level's QD factors and C_g of the ROM runs' step, pushed by the caller
+    for (auto n : {"fg_avg_xx_next", "fg_avg_yy_next"}) c.reg(n, nx, ny, ng);
+    for (auto n : {"fg_edgV_xx_next", "fg_edgV_xy_next"}) c.reg(n, nx + 1, ny, ng);
+    for (auto n : {"fg_edgH_yy_next", "fg_edgH_xy_next"}) c.reg(n, nx, ny + 1, ng);
+    for (auto n : {"Cg_L_next", "Cg_R_next"}) c.reg(n, ny, ng);
+    for (auto n : {"Cg_B_next", "Cg_T_next"}) c.reg(n, nx, ng);
     const char *grey[] = {"HO_E_avg", "MGQD_E_avg", "Temp", "Temp_Old", "E_avg", "E_avg_old", "KapE_Bar",
                           "KapE_Bar_old", "KapE_Bar_dT", "KapE_bar_MGQDold", "GQD_Src", "GQD_Src_old", "Gold_Hat",
                           "Rhat_old", "DC_xx", "DL_xx", "DR_xx", "DC_yy", "DB_yy", "DT_yy", "DL_xy", "DR_xy", "DB_xy",

@@ -174,7 +174,8 @@ void grey_coefs(Ctx &c);
 int egp_fv_newt(Ctx &c, int mgqd_it);
 void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, int n, int conv_type,
                  double eps, int its, double *norm, double *rho, int *conv);
-int step(Ctx &c, int its_out[4]);
+// ho_form 0: full-order step; 1: the ROM runs' step (QD factors and C_g from the context, no sweep)
+int step(Ctx &c, int its_out[4], int ho_form = 0);
 void pull_icrn(Ctx &c, int which, double *host);
 void push_icrn(Ctx &c, int which, const double *host);
 // RESTART_OUT / RESTART_IN of the context's state (csrc/restart.hip); restart_in

@@ -50,7 +50,7 @@ EXPORTS = [
     "trt_server_request", "trt_server_request_error", "trt_server_summary", "trt_server_output_size",
     "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
-    "trt_frecords_last_error",
+    "trt_frecords_last_error", "trt_step_given_qdf",
 ]
 
 _lib = None
@@ -101,6 +101,7 @@ def load(path: str = LIBPATH):
     L.trt_convergence.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_int, _dp, _dp,
                                   C.POINTER(C.c_int)]
     L.trt_step.argtypes = [vp, C.POINTER(C.c_int * 4)]
+    L.trt_step_given_qdf.argtypes = [vp, C.POINTER(C.c_int * 4)]
     L.trt_time.argtypes = [vp]
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
@@ -276,6 +277,12 @@ class Context:
     def step(self):
         it = (C.c_int * 4)()
         self._check(self.L.trt_step(self.h, C.byref(it)), "step")
+        return tuple(it)
+
+    def step_given_qdf(self):
+        """The ROM runs' time step: QD factors and C_g pushed by the caller, no sweep."""
+        it = (C.c_int * 4)()
+        self._check(self.L.trt_step_given_qdf(self.h, C.byref(it)), "step_given_qdf")
         return tuple(it)
 
     @property

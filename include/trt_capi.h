@@ -125,6 +125,17 @@ int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *
  * MainSrc/ALGORITHMS.f08:389-897.  its_out (may be NULL) receives
  * {RT iterations, MLOQD iterations, Newton iterations, sweeps}. */
 int trt_step(trt_ctx *ctx, int its_out[4]);
+/* The time step of the ROM runs (run_type mg_pod / mg_dmd with fg databases, HO_Form 1
+ * and 3, MainSrc/ALGORITHMS.f08:573-631).  Before the call the caller pushes the new time
+ * level's QD factors and C_g -- what POD_RECONSTRUCT_fg / DMD_RECONSTRUCT_fg and their BCg
+ * counterparts write (MainSrc/POD_ROUTINES.f08:28, MainSrc/DMD_ROUTINES.f08:131) -- to the
+ * arrays fg_avg_xx_next, fg_avg_yy_next, fg_edgV_xx_next, fg_edgV_xy_next, fg_edgH_yy_next,
+ * fg_edgH_xy_next, Cg_L_next, Cg_B_next, Cg_R_next, Cg_T_next (shapes of the arrays without
+ * _next).  The step keeps the current factors as the old level's, as the reference does at
+ * its start, copies the _next arrays in where the reconstruction writes, and runs the
+ * MLOQD/GLOQD iteration with one RT iteration and no sweep (maxit_RTE = 1,
+ * MainSrc/INPUTS.f08:94-104).  its_out as trt_step; its_out[3] (sweeps) is 0. */
+int trt_step_given_qdf(trt_ctx *ctx, int its_out[4]);
 double trt_time(const trt_ctx *ctx);
 
 /* Checkpoint and resume.  trt_restart_out ≙ RESTART_OUT, MainSrc/OUTPUTS.f08:1624-1674

@@ -80,6 +80,8 @@ def lib():
         L.orc_egp_fv_newt.restype = C.c_int
         L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_int * 4)]
         L.orc_step.restype = C.c_int
+        L.orc_step_ex.argtypes = [C.c_void_p, C.POINTER(C.c_int * 4), C.c_int]
+        L.orc_step_ex.restype = C.c_int
         L.orc_time.argtypes = [C.c_void_p]
         L.orc_time.restype = C.c_double
         for f in ("orc_sigint",):
@@ -178,6 +180,12 @@ class Oracle:
     def step(self):
         it = (C.c_int * 4)()
         self.L.orc_step(self.h, C.byref(it))
+        return tuple(it)
+
+    def step_given_qdf(self):
+        """The step of the ROM runs: QD factors and C_g from the state, no sweep."""
+        it = (C.c_int * 4)()
+        self.L.orc_step_ex(self.h, C.byref(it), 1)
         return tuple(it)
 
     @property

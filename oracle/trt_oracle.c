@@ -1412,6 +1412,12 @@ orc_state *orc_create(const orc_problem *pp)
     R3(fg_edgV_xx_old, nx + 1, ny); R3(fg_edgV_xy_old, nx + 1, ny); R3(fg_edgH_yy_old, nx, ny + 1); R3(fg_edgH_xy_old, nx, ny + 1);
     s->Cg_L = reg(s, "Cg_L", ny, ng, 1, 1); s->Cg_R = reg(s, "Cg_R", ny, ng, 1, 1);
     s->Cg_B = reg(s, "Cg_B", nx, ng, 1, 1); s->Cg_T = reg(s, "Cg_T", nx, ng, 1, 1);
+    /* the next time level's QD factors and C_g of the ROM runs' step (orc_step_ex, ho_form 1) */
+    reg(s, "fg_avg_xx_next", nx, ny, ng, 1); reg(s, "fg_avg_yy_next", nx, ny, ng, 1);
+    reg(s, "fg_edgV_xx_next", nx + 1, ny, ng, 1); reg(s, "fg_edgV_xy_next", nx + 1, ny, ng, 1);
+    reg(s, "fg_edgH_yy_next", nx, ny + 1, ng, 1); reg(s, "fg_edgH_xy_next", nx, ny + 1, ng, 1);
+    reg(s, "Cg_L_next", ny, ng, 1, 1); reg(s, "Cg_R_next", ny, ng, 1, 1);
+    reg(s, "Cg_B_next", nx, ng, 1, 1); reg(s, "Cg_T_next", nx, ng, 1, 1);
     s->Eg_in_L = reg(s, "Eg_in_L", ny, ng, 1, 1); s->Eg_in_R = reg(s, "Eg_in_R", ny, ng, 1, 1);
     s->Eg_in_B = reg(s, "Eg_in_B", nx, ng, 1, 1); s->Eg_in_T = reg(s, "Eg_in_T", nx, ng, 1, 1);
     s->Fg_in_L = reg(s, "Fg_in_L", ny, ng, 1, 1); s->Fg_in_R = reg(s, "Fg_in_R", ny, ng, 1, 1);
@@ -1552,7 +1558,13 @@ static void copy_arr(orc_state *s, const char *dst, const char *src)
     memcpy(b, a, sizeof(double) * (size_t)d[0] * d[1] * d[2] * d[3]);
 }
 
-int orc_step(orc_state *s, int it_out[4])
+int orc_step(orc_state *s, int it_out[4]) { return orc_step_ex(s, it_out, 0); }
+
+/* ho_form 0: the full-order step.  ho_form 1: the step of the ROM runs (HO_Form 1 and 3,
+   AL:573-631): each RT iteration takes the new level's QD factors and C_g from the *_next
+   arrays (what POD/DMD_RECONSTRUCT_fg write) instead of sweeping, and maxit_RTE is 1 as INPUT sets it for
+   'fg' ROM runs (IP:94-104). */
+int orc_step_ex(orc_state *s, int it_out[4], int ho_form)
 {
     const int nx = s->nx, ny = s->ny;
     const size_t n = (size_t)nx * ny;
@@ -1576,9 +1588,19 @@ int orc_step(orc_state *s, int it_out[4])
     memset(s->Temp_RTold2, 0, n * 8); memset(s->Temp_RTold, 0, n * 8);
     int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0, rt_conv = 0;
     const int rt_start = 2;  /* AL:302 */
-    while (!rt_conv && rt_its < s->p.maxit_rte) {
+    const int maxit_rte = ho_form ? 1 : s->p.maxit_rte;
+    while (!rt_conv && rt_its < maxit_rte) {
         rt_its++;
-        if (rt_its >= rt_start) {
+        if (ho_form) {
+            /* AL:582-593: the reconstruction writes the new level's factors and C_g */
+            static const char *const qdf[] = {"fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy",
+                                              "fg_edgH_xy", "Cg_L", "Cg_B", "Cg_R", "Cg_T"};
+            for (int k = 0; k < 10; ++k) {
+                char next[32];
+                snprintf(next, sizeof next, "%s_next", qdf[k]);
+                copy_arr(s, qdf[k], next);
+            }
+        } else if (rt_its >= rt_start) {
             orc_transport_scb(s);                /* AL:541-542 */
             sweeps++;
             orc_collapse_intensities(s);         /* AL:551-553 */

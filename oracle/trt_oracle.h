@@ -75,6 +75,7 @@ int  orc_convergence(int conv_type, double eps, const double *x, const double *x
  * number of outer (RT) iterations.  it_out (may be NULL) receives
  * {RT_Its, total MLOQD its, total Newton its, total sweeps}. */
 int orc_step(orc_state *s, int it_out[4]);
+int orc_step_ex(orc_state *s, int it_out[4], int ho_form);
 double orc_time(orc_state *s);
 
 /* the reduced QD system (QD_SOLVERS.f08:40-538): build the CSR matrix and RHS in
