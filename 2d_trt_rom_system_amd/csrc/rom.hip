@@ -1,0 +1,187 @@
+// POD reconstruction of a QD-factor field for the ROM runs' step: POD_RECONSTRUCT
+// (MainSrc/POD_ROUTINES.f08:372-432) and LINEAR_INTERPOLATION between two data
+// times (MainSrc/GRID_FUNCTIONS.f08:819-836), as POD_RECONSTRUCT_fg calls them
+// (POD_ROUTINES.f08:28-130) for data on the simulation grid.
+//
+// A basis (centering vector C, singular values S, left vectors U, right vectors
+// V) lives on the device; a reconstruction writes one context array (normally a
+// *_next array of trt_step_given_qdf):
+//   per group (PODgsum 0):  f[g][i] = sum_r S_g[r] U_g[r][i] V_g[t][r] + C[g][i]
+//   all groups (PODgsum 1): f[i]    = sum_r S[r]   U[r][i]   V[t][r]   + C[i]
+// with the reference's storage: U group blocks of rank(g) columns of glen values,
+// S and V concatenated over the groups, V of a group t-major (rank(g) values per
+// data time).  Between two data times each element is y1 + (x - x1)(y2 - y1)/(x2 - x1).
+// One pass over U: a thread per element, both time points accumulated together.
+#include "trt_internal.h"
+
+#include <numeric>
+
+struct trt_pod {
+    trt_ctx *ctx = nullptr;
+    std::string name;
+    int gsum = 0, ng = 0, nt = 0;
+    size_t len = 0, glen = 0;
+    double *C = nullptr, *S = nullptr, *U = nullptr, *V = nullptr;
+    long long *uoff = nullptr;   // per group: offset of its U block
+    int *soff = nullptr;         // per group: offset of its S values (p3)
+    int *rank = nullptr;
+    size_t ubytes = 0;
+};
+
+namespace {
+
+__global__ void pod_kernel(const double *__restrict__ C, const double *__restrict__ S, const double *__restrict__ U,
+                           const double *__restrict__ V, const long long *__restrict__ uoff,
+                           const int *__restrict__ soff, const int *__restrict__ rank, int nt, size_t len, size_t glen,
+                           int gsum, int t1, int t2, double x, double x1, double x2, double *__restrict__ out)
+{
+    const bool interp = t1 != t2;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        const int g = gsum ? 0 : (int)(i / glen);
+        const size_t ii = gsum ? i : i - (size_t)g * glen;
+        const size_t stride = gsum ? len : glen;
+        const int rk = rank[g], p3 = soff[g];
+        const double *u = U + uoff[g] + ii;
+        const double *s = S + p3;
+        const double *v1 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t1 * rk;
+        const double *v2 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t2 * rk;
+        double a1 = 0.0, a2 = 0.0;
+        for (int r = 0; r < rk; ++r) {
+            const double su = s[r] * u[(size_t)r * stride];
+            a1 += su * v1[r];
+            if (interp) a2 += su * v2[r];
+        }
+        const double y1 = a1 + C[i];
+        out[i] = interp ? y1 + (x - x1) * ((a2 + C[i]) - y1) / (x2 - x1) : y1;
+    }
+}
+
+template <typename T>
+T *upload(const T *h, size_t n, hipStream_t st, size_t *bytes)
+{
+    T *d = nullptr;
+    TRT_CHECK(hipMalloc(reinterpret_cast<void **>(&d), std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) TRT_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    if (bytes) *bytes += n * sizeof(T);
+    return d;
+}
+
+void pod_free(trt_pod *p)
+{
+    for (void *q : {(void *)p->C, (void *)p->S, (void *)p->U, (void *)p->V, (void *)p->uoff, (void *)p->soff,
+                    (void *)p->rank})
+        if (q) (void)hipFree(q);
+    delete p;
+}
+
+template <typename F>
+int pguarded(trt_ctx *ctx, F &&f)
+{
+    if (!ctx) return -1;
+    try {
+        f(ctx->c);
+        return 0;
+    } catch (const trt::Error &e) {
+        ctx->c.err = e.msg;
+    } catch (const std::exception &e) {
+        ctx->c.err = e.what();
+    }
+    return -2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int nt, const int *rank,
+                   const double *C, const double *S, const double *U, const double *V)
+{
+    return pguarded(ctx, [&](trt::Ctx &c) {
+        if (!out || !name || !rank || !C || !S || !U || !V) throw trt::Error("null argument");
+        *out = nullptr;
+        const size_t len = trt_array_size(ctx, name);
+        if (len == 0 || trt::derived_size(c, name)) throw trt::Error(std::string("no array ") + name + " to reconstruct into");
+        if (gsum != 0 && gsum != 1) throw trt::Error("PODgsum must be 0 or 1");
+        if (nt < 1) throw trt::Error("the basis needs at least one data time");
+        const int ng = gsum ? 1 : c.ng;
+        if (len % (size_t)ng) throw trt::Error(std::string(name) + " does not split into N_g equal group blocks");
+        std::vector<int> rk(rank, rank + ng);
+        for (int r : rk)
+            if (r < 0) throw trt::Error("negative rank");
+        const size_t glen = len / ng;
+        std::vector<long long> uoff(ng);
+        std::vector<int> soff(ng);
+        size_t uo = 0, so = 0;
+        for (int g = 0; g < ng; ++g) {
+            uoff[g] = (long long)uo;
+            soff[g] = (int)so;
+            uo += glen * (size_t)rk[g];
+            so += (size_t)rk[g];
+        }
+        trt_pod *p = new trt_pod;
+        try {
+            p->ctx = ctx; p->name = name; p->gsum = gsum; p->ng = ng; p->nt = nt; p->len = len; p->glen = glen;
+            size_t b = 0;
+            p->C = upload(C, len, c.stream, &b);
+            p->S = upload(S, so, c.stream, &b);
+            p->U = upload(U, uo, c.stream, &b);
+            p->V = upload(V, so * (size_t)nt, c.stream, &b);
+            p->uoff = upload(uoff.data(), uoff.size(), c.stream, &b);
+            p->soff = upload(soff.data(), soff.size(), c.stream, &b);
+            p->rank = upload(rk.data(), rk.size(), c.stream, &b);
+            p->ubytes = uo * sizeof(double);
+            TRT_CHECK(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            pod_free(p);
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int trt_pod_reconstruct(trt_pod *p, int t1, int t2, double x, double x1, double x2)
+{
+    if (!p) return -1;
+    return pguarded(p->ctx, [&](trt::Ctx &c) {
+        if (t1 < 1 || t1 > p->nt || t2 < 1 || t2 > p->nt) throw trt::Error("data time index out of range");
+        if (t1 != t2 && x2 == x1) throw trt::Error("interpolation between two data times at the same time");
+        double *dst = c.arr(p->name.c_str());
+        const int blocks = (int)std::min<size_t>((p->len + 255) / 256, 8192);
+        hipLaunchKernelGGL(pod_kernel, dim3(blocks), dim3(256), 0, c.stream, p->C, p->S, p->U, p->V, p->uoff, p->soff,
+                           p->rank, p->nt, p->len, p->glen, p->gsum, t1 - 1, t2 - 1, x, x1, x2, dst);
+        TRT_CHECK(hipGetLastError());
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int trt_pod_timed(trt_pod *p, int reps, double *ms)
+{
+    if (!p) return -1;
+    return pguarded(p->ctx, [&](trt::Ctx &c) {
+        if (reps < 1 || !ms) throw trt::Error("bad argument");
+        double *dst = c.arr(p->name.c_str());
+        const int blocks = (int)std::min<size_t>((p->len + 255) / 256, 8192);
+        TRT_CHECK(hipEventRecord(c.ev0, c.stream));
+        for (int k = 0; k < reps; ++k)
+            hipLaunchKernelGGL(pod_kernel, dim3(blocks), dim3(256), 0, c.stream, p->C, p->S, p->U, p->V, p->uoff,
+                               p->soff, p->rank, p->nt, p->len, p->glen, p->gsum, 0, 0, 0.0, 0.0, 0.0, dst);
+        TRT_CHECK(hipGetLastError());
+        TRT_CHECK(hipEventRecord(c.ev1, c.stream));
+        TRT_CHECK(hipEventSynchronize(c.ev1));
+        float f = 0.f;
+        TRT_CHECK(hipEventElapsedTime(&f, c.ev0, c.ev1));
+        *ms = f / reps;
+    });
+}
+
+size_t trt_pod_basis_bytes(const trt_pod *p) { return p ? p->ubytes : 0; }
+
+int trt_pod_destroy(trt_pod *p)
+{
+    if (!p) return -1;
+    (void)hipStreamSynchronize(p->ctx->c.stream);
+    pod_free(p);
+    return 0;
+}
+
+}  // extern "C"

@@ -50,7 +50,8 @@ EXPORTS = [
     "trt_server_request", "trt_server_request_error", "trt_server_summary", "trt_server_output_size",
     "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
-    "trt_frecords_last_error", "trt_step_given_qdf",
+    "trt_frecords_last_error", "trt_step_given_qdf", "trt_pod_create", "trt_pod_reconstruct", "trt_pod_destroy",
+    "trt_pod_timed", "trt_pod_basis_bytes",
 ]
 
 _lib = None
@@ -102,6 +103,13 @@ def load(path: str = LIBPATH):
                                   C.POINTER(C.c_int)]
     L.trt_step.argtypes = [vp, C.POINTER(C.c_int * 4)]
     L.trt_step_given_qdf.argtypes = [vp, C.POINTER(C.c_int * 4)]
+    L.trt_pod_create.argtypes = [vp, C.POINTER(vp), C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp,
+                                 _dp]
+    L.trt_pod_reconstruct.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.trt_pod_destroy.argtypes = [vp]
+    L.trt_pod_timed.argtypes = [vp, C.c_int, _dp]
+    L.trt_pod_basis_bytes.argtypes = [vp]
+    L.trt_pod_basis_bytes.restype = C.c_size_t
     L.trt_time.argtypes = [vp]
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
@@ -384,6 +392,41 @@ class Batch:
         u, g, n = C.c_longlong(), C.c_int(), C.c_int()
         self._check(self.L.trt_batch_info(self.h, C.byref(u), C.byref(g), C.byref(n)), "batch_info")
         return u.value, g.value, n.value
+
+
+class PodBasis:
+    """A POD basis on the device reconstructing into one context array (POD_RECONSTRUCT,
+    MainSrc/POD_ROUTINES.f08:372-432).  Arrays in the reference's storage order."""
+
+    def __init__(self, ctx: "Context", name: str, C_, S, U, V, rank, nt: int, gsum: int = 0):
+        self.ctx, self.L = ctx, ctx.L
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F")) for a in (C_, S, U, V)]
+        rk = np.ascontiguousarray(np.asarray(rank, dtype=np.int32).ravel())
+        self._keep = arrs + [rk]
+        h = C.c_void_p()
+        ctx._check(self.L.trt_pod_create(ctx.h, C.byref(h), name.encode(), int(gsum), int(nt),
+                                         rk.ctypes.data_as(C.POINTER(C.c_int)), *[a.ctypes.data_as(_dp) for a in arrs]),
+                   "pod_create")
+        self.h = h
+
+    def reconstruct(self, t1: int, t2: int | None = None, x: float = 0.0, x1: float = 0.0, x2: float = 0.0):
+        """The field at data time t1 (1-based), or interpolated to x between t1 (at x1) and t2 (at x2)."""
+        self.ctx._check(self.L.trt_pod_reconstruct(self.h, int(t1), int(t1 if t2 is None else t2), float(x), float(x1),
+                                                   float(x2)), "pod_reconstruct")
+
+    def timed(self, reps: int = 10) -> float:
+        ms = C.c_double()
+        self.ctx._check(self.L.trt_pod_timed(self.h, int(reps), C.byref(ms)), "pod_timed")
+        return ms.value
+
+    @property
+    def basis_bytes(self) -> int:
+        return int(self.L.trt_pod_basis_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            self.L.trt_pod_destroy(self.h)
+            self.h = None
 
 
 def _io_check(rc: int, what: str) -> None:

@@ -136,6 +136,29 @@ int trt_step(trt_ctx *ctx, int its_out[4]);
  * MLOQD/GLOQD iteration with one RT iteration and no sweep (maxit_RTE = 1,
  * MainSrc/INPUTS.f08:94-104).  its_out as trt_step; its_out[3] (sweeps) is 0. */
 int trt_step_given_qdf(trt_ctx *ctx, int its_out[4]);
+
+/* POD reconstruction of one field of the ROM runs: POD_RECONSTRUCT,
+ * MainSrc/POD_ROUTINES.f08:372-432, with LINEAR_INTERPOLATION between two data times,
+ * MainSrc/GRID_FUNCTIONS.f08:819-836, as POD_RECONSTRUCT_fg calls them
+ * (POD_ROUTINES.f08:28-130) for data on the simulation grid (the data-grid map FMAP is not
+ * built).  trt_pod_create uploads a basis in the reference's storage -- C (centering
+ * vector, the size of array `name`), S (singular values), U (left vectors: per group
+ * rank(g) columns of size/N_g values, or rank(1) columns of the whole size for gsum 1),
+ * V (right vectors: per group nt x rank(g), t-major), rank (N_g values, or 1 for gsum 1)
+ * -- for reconstruction into the context array `name` (normally a *_next array of
+ * trt_step_given_qdf).  trt_pod_reconstruct writes the field at data time t1 (1-based),
+ * or, when t2 != t1, interpolated linearly to time x between data times x1 (t1) and x2
+ * (t2).  The basis is device memory of its own; destroy it before its context.  Errors:
+ * trt_last_error(ctx). */
+typedef struct trt_pod trt_pod;
+int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int nt, const int *rank,
+                   const double *C, const double *S, const double *U, const double *V);
+int trt_pod_reconstruct(trt_pod *pod, int t1, int t2, double x, double x1, double x2);
+int trt_pod_destroy(trt_pod *pod);
+/* timing (no counterpart in the reference): reps reconstructions at data time 1 on the
+ * context's stream, ms per reconstruction from HIP events; bytes of U */
+int trt_pod_timed(trt_pod *pod, int reps, double *ms);
+size_t trt_pod_basis_bytes(const trt_pod *pod);
 double trt_time(const trt_ctx *ctx);
 
 /* Checkpoint and resume.  trt_restart_out ≙ RESTART_OUT, MainSrc/OUTPUTS.f08:1624-1674

@@ -1,0 +1,83 @@
+"""POD reconstruction on the GPU (trt_pod_*) against the oracle's restatement, and a ROM
+step driven by it: bases built from the oracle's own snapshots, fields reconstructed into
+the *_next arrays on the device, trt_step_given_qdf against the oracle's same step."""
+import numpy as np
+import pytest
+
+import oracle.rom as rom
+
+pytestmark = pytest.mark.gpu
+
+QDF = ("fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy", "Cg_L", "Cg_B", "Cg_R",
+       "Cg_T")
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name,gsum,rank", [("fg_avg_xx_next", 0, None), ("fg_edgV_xy_next", 0, 3),
+                                            ("Cg_L_next", 1, 4), ("fg_edgH_yy_next", 1, None)])
+def test_pod_reconstruct_matches_oracle(trt, name, gsum, rank):
+    prob = trt.problem.small(nx=9, ny=7, order=4, ng=3)
+    ctx = trt.native.Context(prob)
+    n = ctx.size(name)
+    nt = 5
+    rng = np.random.default_rng(n + gsum)
+    X = rng.uniform(0.2, 0.5, (n, nt))
+    C, S, U, V, rk = rom.basis_from_snapshots(X, prob.ng, gsum, rank)
+    pod = trt.native.PodBasis(ctx, name, C, S, U, V, rk, nt, gsum)
+    for t in (1, 3, 5):
+        pod.reconstruct(t)
+        ref = rom.pod_reconstruct(C, S, U, V, rk, n, prob.ng, nt, t, gsum)
+        assert _rel(ctx.pull(name), ref) < 1e-14, (t, _rel(ctx.pull(name), ref))
+        if rank is None:
+            assert _rel(ctx.pull(name), X[:, t - 1]) < 1e-12
+    pod.reconstruct(2, 3, 0.27, 0.2, 0.3)
+    f2 = rom.pod_reconstruct(C, S, U, V, rk, n, prob.ng, nt, 2, gsum)
+    f3 = rom.pod_reconstruct(C, S, U, V, rk, n, prob.ng, nt, 3, gsum)
+    assert _rel(ctx.pull(name), rom.linear_interpolation(f2, f3, 0.27, 0.2, 0.3)) < 1e-14
+    with pytest.raises(trt.native.TrtError, match="out of range"):
+        pod.reconstruct(6)
+    pod.close()
+    with pytest.raises(trt.native.TrtError, match="no array"):
+        trt.native.PodBasis(ctx, "fg_avg_zz_next", C, S, U, V, rk, nt, gsum)
+    ctx.close()
+
+
+def test_rom_step_from_pod_bases(trt, oracle_mod):
+    """Snapshots of three full oracle steps -> per-group bases (full rank) -> the fields of
+    the second snapshot time reconstructed on the GPU -> one ROM step, GPU against oracle."""
+    prob = trt.problem.small(nx=9, ny=7, order=4, ng=3)
+    prob.use_line_search = 0
+    prob.conv_ho, prob.conv_lo, prob.conv_gr1 = 1e-11, 1e-12, 1e-13
+    snap = oracle_mod.Oracle(prob, threads=4)
+    X = {n: [] for n in QDF}
+    for _ in range(3):
+        snap.step()
+        for n in QDF:
+            X[n].append(np.asarray(snap[n]).ravel(order="F").copy())
+    o = oracle_mod.Oracle(prob, threads=4)
+    ctx = trt.native.Context(prob)
+    o.step()
+    ctx.step()
+    pods = []
+    for n in QDF:
+        Xn = np.stack(X[n], axis=1)
+        C, S, U, V, rk = rom.basis_from_snapshots(Xn, prob.ng, 0)
+        pod = trt.native.PodBasis(ctx, n + "_next", C, S, U, V, rk, 3, 0)
+        pod.reconstruct(2)
+        pods.append(pod)
+        ref = rom.pod_reconstruct(C, S, U, V, rk, Xn.shape[0], prob.ng, 3, 2, 0)
+        o[n + "_next"][...] = ref.reshape(np.asarray(o[n]).shape, order="F")
+        assert _rel(ctx.pull(n + "_next"), ref) < 1e-14
+    it_o = o.step_given_qdf()
+    it_g = ctx.step_given_qdf()
+    assert it_g[3] == 0 and it_g[0] == 1
+    for n in ("Temp", "E_avg", "MGQD_E_avg", "Eg_avg"):
+        ref = np.asarray(o[n]).ravel(order="F")
+        err = float(np.sqrt(np.sum((ctx.pull(n) - ref) ** 2) / np.sum(ref ** 2)))
+        assert err < 1e-8, (n, err, it_o, it_g)
+    for p in pods:
+        p.close()
+    ctx.close()
