@@ -46,6 +46,8 @@ __global__ void pod_kernel(const double *__restrict__ C, const double *__restric
         const double *v1 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t1 * rk;
         const double *v2 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t2 * rk;
         double a1 = 0.0, a2 = 0.0;
+        // unrolled so several U loads are in flight per thread (the sum keeps its order)
+#pragma unroll 8
         for (int r = 0; r < rk; ++r) {
             const double su = s[r] * u[(size_t)r * stride];
             a1 += su * v1[r];

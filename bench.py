@@ -179,6 +179,35 @@ def measure_dropin_sweep(trt, prob, reps: int = 2):
                     "I_avg, I_edgV, I_edgH, Ic_edgV, Ic_edgH into pageable host arrays"}
 
 
+def measure_pod(trt, prob, rank: int = 32, nt: int = 10, reps: int = 20):
+    """POD reconstruction (POD_RECONSTRUCT) of one QD-factor field of the ROM runs'
+    step at the problem's size: fg_avg_xx (N_x N_y N_g values), a per-group basis of
+    the given rank; synthetic basis.  Bytes per reconstruction: U + C read, the field
+    written."""
+    import numpy as np
+    ctx = trt.native.Context(prob, precision=0)
+    pod = None
+    try:
+        n = ctx.size("fg_avg_xx_next")
+        rng = np.random.default_rng(3)
+        ng = prob.ng
+        U = rng.standard_normal(n * rank)
+        C = rng.uniform(0.2, 0.4, n)
+        S = np.sort(rng.uniform(0.0, 1.0, ng * rank))[::-1].copy()
+        V = rng.standard_normal(ng * rank * nt)
+        pod = trt.native.PodBasis(ctx, "fg_avg_xx_next", C, S, U, V, np.full(ng, rank), nt, 0)
+        pod.reconstruct(1)
+        ms = pod.timed(reps)
+        b = pod.basis_bytes + 2 * 8 * n
+    finally:
+        if pod is not None:
+            pod.close()
+        ctx.close()
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"field": f"fg_avg_xx ({prob.nx}x{prob.ny}x{prob.ng})", "rank_per_group": rank, "ms": ms,
+            "bytes": b, "gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+
 def measure_restart(trt):
     """Checkpoint and resume (RESTART_OUT / RESTART_IN records) of the 128x128, S16,
     32-group FP64 state after one time step: 4.8 GB in 25 records, I_crn and RT_Src
@@ -259,6 +288,7 @@ def main(argv=None):
     ap.add_argument("--no-steps", action="store_true", help="skip the full MLQD time steps")
     ap.add_argument("--no-restart", action="store_true", help="skip the checkpoint/resume measurement")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-array (PCIe-inclusive) sweep")
+    ap.add_argument("--no-pod", action="store_true", help="skip the POD reconstruction measurement")
     ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -353,6 +383,9 @@ def main(argv=None):
     if not args.no_dropin and not args.only:
         print("[bench] drop-in TRANSPORT_SCB with host arrays (PCIe-inclusive)", file=sys.stderr, flush=True)
         out["dropin_sweep"] = measure_dropin_sweep(trt, prob)
+    if not args.no_pod and not args.only:
+        print("[bench] POD reconstruction (ROM runs)", file=sys.stderr, flush=True)
+        out["pod_reconstruct"] = measure_pod(trt, prob)
     if not args.no_restart and not args.only:
         print("[bench] checkpoint and resume (128x128, S16, 32 groups)", file=sys.stderr, flush=True)
         try:
