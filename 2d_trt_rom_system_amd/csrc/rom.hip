@@ -14,6 +14,7 @@
 // One pass over U: a thread per element, both time points accumulated together.
 #include "trt_internal.h"
 
+#include <cstdint>
 #include <numeric>
 
 struct trt_pod {
@@ -56,6 +57,63 @@ __global__ void pod_kernel(const double *__restrict__ C, const double *__restric
         const double y1 = a1 + C[i];
         out[i] = interp ? y1 + (x - x1) * ((a2 + C[i]) - y1) / (x2 - x1) : y1;
     }
+}
+
+// Two elements per thread with 16-byte loads, for an even group length: a pair
+// never straddles two groups and every pair starts 16-byte aligned (group blocks of
+// U start at multiples of glen).  Each element's sum keeps the scalar kernel's order.
+__global__ void pod_kernel2(const double *__restrict__ C, const double *__restrict__ S, const double *__restrict__ U,
+                            const double *__restrict__ V, const long long *__restrict__ uoff,
+                            const int *__restrict__ soff, const int *__restrict__ rank, int nt, size_t len,
+                            size_t glen, int gsum, int t1, int t2, double x, double x1, double x2,
+                            double *__restrict__ out)
+{
+    const bool interp = t1 != t2;
+    const size_t npair = len / 2;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < npair; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = 2 * k;
+        const int g = gsum ? 0 : (int)(i / glen);
+        const size_t ii = gsum ? i : i - (size_t)g * glen;
+        const size_t stride = gsum ? len : glen;
+        const int rk = rank[g], p3 = soff[g];
+        const double *u = U + uoff[g] + ii;
+        const double *s = S + p3;
+        const double *v1 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t1 * rk;
+        const double *v2 = V + (gsum ? 0 : (size_t)nt * p3) + (size_t)t2 * rk;
+        double a1x = 0.0, a1y = 0.0, a2x = 0.0, a2y = 0.0;
+#pragma unroll 8
+        for (int r = 0; r < rk; ++r) {
+            const double2 uu = *reinterpret_cast<const double2 *>(u + (size_t)r * stride);
+            const double sx = s[r] * uu.x, sy = s[r] * uu.y;
+            a1x += sx * v1[r];
+            a1y += sy * v1[r];
+            if (interp) {
+                a2x += sx * v2[r];
+                a2y += sy * v2[r];
+            }
+        }
+        const double2 cc = *reinterpret_cast<const double2 *>(C + i);
+        const double y1x = a1x + cc.x, y1y = a1y + cc.y;
+        double2 o;
+        o.x = interp ? y1x + (x - x1) * ((a2x + cc.x) - y1x) / (x2 - x1) : y1x;
+        o.y = interp ? y1y + (x - x1) * ((a2y + cc.y) - y1y) / (x2 - x1) : y1y;
+        *reinterpret_cast<double2 *>(out + i) = o;
+    }
+}
+
+void pod_launch(const trt_pod *p, double *dst, hipStream_t st, int t1, int t2, double x, double x1, double x2)
+{
+    const size_t seg = p->gsum ? p->len : p->glen;
+    if (seg % 2 == 0 && (reinterpret_cast<uintptr_t>(dst) % 16) == 0) {
+        const int blocks = (int)std::min<size_t>((p->len / 2 + 255) / 256, 8192);
+        hipLaunchKernelGGL(pod_kernel2, dim3(blocks), dim3(256), 0, st, p->C, p->S, p->U, p->V, p->uoff, p->soff,
+                           p->rank, p->nt, p->len, p->glen, p->gsum, t1, t2, x, x1, x2, dst);
+    } else {
+        const int blocks = (int)std::min<size_t>((p->len + 255) / 256, 8192);
+        hipLaunchKernelGGL(pod_kernel, dim3(blocks), dim3(256), 0, st, p->C, p->S, p->U, p->V, p->uoff, p->soff,
+                           p->rank, p->nt, p->len, p->glen, p->gsum, t1, t2, x, x1, x2, dst);
+    }
+    TRT_CHECK(hipGetLastError());
 }
 
 template <typename T>
@@ -147,11 +205,7 @@ int trt_pod_reconstruct(trt_pod *p, int t1, int t2, double x, double x1, double 
     return pguarded(p->ctx, [&](trt::Ctx &c) {
         if (t1 < 1 || t1 > p->nt || t2 < 1 || t2 > p->nt) throw trt::Error("data time index out of range");
         if (t1 != t2 && x2 == x1) throw trt::Error("interpolation between two data times at the same time");
-        double *dst = c.arr(p->name.c_str());
-        const int blocks = (int)std::min<size_t>((p->len + 255) / 256, 8192);
-        hipLaunchKernelGGL(pod_kernel, dim3(blocks), dim3(256), 0, c.stream, p->C, p->S, p->U, p->V, p->uoff, p->soff,
-                           p->rank, p->nt, p->len, p->glen, p->gsum, t1 - 1, t2 - 1, x, x1, x2, dst);
-        TRT_CHECK(hipGetLastError());
+        pod_launch(p, c.arr(p->name.c_str()), c.stream, t1 - 1, t2 - 1, x, x1, x2);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -162,12 +216,8 @@ int trt_pod_timed(trt_pod *p, int reps, double *ms)
     return pguarded(p->ctx, [&](trt::Ctx &c) {
         if (reps < 1 || !ms) throw trt::Error("bad argument");
         double *dst = c.arr(p->name.c_str());
-        const int blocks = (int)std::min<size_t>((p->len + 255) / 256, 8192);
         TRT_CHECK(hipEventRecord(c.ev0, c.stream));
-        for (int k = 0; k < reps; ++k)
-            hipLaunchKernelGGL(pod_kernel, dim3(blocks), dim3(256), 0, c.stream, p->C, p->S, p->U, p->V, p->uoff,
-                               p->soff, p->rank, p->nt, p->len, p->glen, p->gsum, 0, 0, 0.0, 0.0, 0.0, dst);
-        TRT_CHECK(hipGetLastError());
+        for (int k = 0; k < reps; ++k) pod_launch(p, dst, c.stream, 0, 0, 0.0, 0.0, 0.0);
         TRT_CHECK(hipEventRecord(c.ev1, c.stream));
         TRT_CHECK(hipEventSynchronize(c.ev1));
         float f = 0.f;
