@@ -14,6 +14,7 @@
 // One pass over U: a thread per element, both time points accumulated together.
 #include "trt_internal.h"
 
+#include <cmath>
 #include <cstdint>
 #include <numeric>
 
@@ -29,7 +30,45 @@ struct trt_pod {
     size_t ubytes = 0;
 };
 
+// DMD reconstruction (DMD_RECONSTRUCT, MainSrc/DMD_ROUTINES.f08:28-103): the real part of
+// sum_r B_r W_r[i] exp(L_r t), plus C[i]; W in U's block layout, complex (re, im) pairs.
+// The factors exp(L_r t) are formed on the host per call (rank values).
+struct trt_dmd {
+    trt_ctx *ctx = nullptr;
+    std::string name;
+    int gsum = 0, ng = 0;
+    size_t len = 0, glen = 0;
+    std::vector<double> L, B;                // host: (re, im) pairs, sum of ranks
+    std::vector<int> rank_h;
+    double *C = nullptr, *W = nullptr, *B_dev = nullptr, *E = nullptr;
+    long long *woff = nullptr;
+    int *soff = nullptr, *rank = nullptr;
+};
+
 namespace {
+
+__global__ void dmd_kernel(const double *__restrict__ C, const double2 *__restrict__ W, const double2 *__restrict__ B,
+                           const double2 *__restrict__ E, const long long *__restrict__ woff,
+                           const int *__restrict__ soff, const int *__restrict__ rank, size_t len, size_t glen,
+                           int gsum, double *__restrict__ out)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        const int g = gsum ? 0 : (int)(i / glen);
+        const size_t ii = gsum ? i : i - (size_t)g * glen;
+        const size_t stride = gsum ? len : glen;
+        const int rk = rank[g], p3 = soff[g];
+        const double2 *w = W + woff[g] + ii;
+        double acc = 0.0;
+#pragma unroll 4
+        for (int r = 0; r < rk; ++r) {
+            const double2 b = B[p3 + r], e = E[p3 + r], wv = w[(size_t)r * stride];
+            // (B W) exp(L t), left to right as the reference multiplies
+            const double bwr = b.x * wv.x - b.y * wv.y, bwi = b.x * wv.y + b.y * wv.x;
+            acc += bwr * e.x - bwi * e.y;
+        }
+        out[i] = acc + C[i];
+    }
+}
 
 __global__ void pod_kernel(const double *__restrict__ C, const double *__restrict__ S, const double *__restrict__ U,
                            const double *__restrict__ V, const long long *__restrict__ uoff,
@@ -227,6 +266,91 @@ int trt_pod_timed(trt_pod *p, int reps, double *ms)
 }
 
 size_t trt_pod_basis_bytes(const trt_pod *p) { return p ? p->ubytes : 0; }
+
+int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, const int *rank, const double *C,
+                   const double *L, const double *B, const double *W)
+{
+    return pguarded(ctx, [&](trt::Ctx &c) {
+        if (!out || !name || !rank || !C || !L || !B || !W) throw trt::Error("null argument");
+        *out = nullptr;
+        const size_t len = trt_array_size(ctx, name);
+        if (len == 0 || trt::derived_size(c, name)) throw trt::Error(std::string("no array ") + name + " to reconstruct into");
+        if (gsum != 0 && gsum != 1) throw trt::Error("DMDgsum must be 0 or 1");
+        const int ng = gsum ? 1 : c.ng;
+        if (len % (size_t)ng) throw trt::Error(std::string(name) + " does not split into N_g equal group blocks");
+        std::vector<int> rk(rank, rank + ng);
+        for (int r : rk)
+            if (r < 0) throw trt::Error("negative rank");
+        const size_t glen = len / ng;
+        std::vector<long long> woff(ng);
+        std::vector<int> soff(ng);
+        size_t wo = 0, so = 0;
+        for (int g = 0; g < ng; ++g) {
+            woff[g] = (long long)wo;
+            soff[g] = (int)so;
+            wo += glen * (size_t)rk[g];
+            so += (size_t)rk[g];
+        }
+        trt_dmd *d = new trt_dmd;
+        auto cleanup = [](trt_dmd *q) {
+            for (void *m : {(void *)q->C, (void *)q->W, (void *)q->B_dev, (void *)q->E, (void *)q->woff,
+                            (void *)q->soff, (void *)q->rank})
+                if (m) (void)hipFree(m);
+            delete q;
+        };
+        try {
+            d->ctx = ctx; d->name = name; d->gsum = gsum; d->ng = ng; d->len = len; d->glen = glen;
+            d->L.assign(L, L + 2 * so);
+            d->B.assign(B, B + 2 * so);
+            d->rank_h = rk;
+            d->C = upload(C, len, c.stream, nullptr);
+            d->W = upload(W, 2 * wo, c.stream, nullptr);
+            d->B_dev = upload(B, 2 * so, c.stream, nullptr);
+            d->E = upload(B, 2 * so, c.stream, nullptr);      // overwritten per reconstruction
+            d->woff = upload(woff.data(), woff.size(), c.stream, nullptr);
+            d->soff = upload(soff.data(), soff.size(), c.stream, nullptr);
+            d->rank = upload(rk.data(), rk.size(), c.stream, nullptr);
+            TRT_CHECK(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            cleanup(d);
+            throw;
+        }
+        *out = d;
+    });
+}
+
+int trt_dmd_reconstruct(trt_dmd *d, double time)
+{
+    if (!d) return -1;
+    return pguarded(d->ctx, [&](trt::Ctx &c) {
+        // exp(L t) per mode: exp(Re L t) (cos(Im L t) + i sin(Im L t))
+        const size_t n = d->L.size() / 2;
+        std::vector<double> e(2 * n);
+        for (size_t k = 0; k < n; ++k) {
+            const double m = std::exp(d->L[2 * k] * time), a = d->L[2 * k + 1] * time;
+            e[2 * k] = m * std::cos(a);
+            e[2 * k + 1] = m * std::sin(a);
+        }
+        if (n) TRT_CHECK(hipMemcpyAsync(d->E, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        const int blocks = (int)std::min<size_t>((d->len + 255) / 256, 8192);
+        hipLaunchKernelGGL(dmd_kernel, dim3(blocks), dim3(256), 0, c.stream, d->C, reinterpret_cast<const double2 *>(d->W),
+                           reinterpret_cast<const double2 *>(d->B_dev), reinterpret_cast<const double2 *>(d->E),
+                           d->woff, d->soff, d->rank, d->len, d->glen, d->gsum, c.arr(d->name.c_str()));
+        TRT_CHECK(hipGetLastError());
+        TRT_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int trt_dmd_destroy(trt_dmd *d)
+{
+    if (!d) return -1;
+    (void)hipStreamSynchronize(d->ctx->c.stream);
+    for (void *m : {(void *)d->C, (void *)d->W, (void *)d->B_dev, (void *)d->E, (void *)d->woff, (void *)d->soff,
+                    (void *)d->rank})
+        if (m) (void)hipFree(m);
+    delete d;
+    return 0;
+}
 
 int trt_pod_destroy(trt_pod *p)
 {

@@ -51,7 +51,7 @@ EXPORTS = [
     "trt_server_output", "trt_autotune_tile", "trt_autotune_shapes", "trt_autotune_last_error",
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
     "trt_frecords_last_error", "trt_step_given_qdf", "trt_pod_create", "trt_pod_reconstruct", "trt_pod_destroy",
-    "trt_pod_timed", "trt_pod_basis_bytes",
+    "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
 ]
 
 _lib = None
@@ -110,6 +110,9 @@ def load(path: str = LIBPATH):
     L.trt_pod_timed.argtypes = [vp, C.c_int, _dp]
     L.trt_pod_basis_bytes.argtypes = [vp]
     L.trt_pod_basis_bytes.restype = C.c_size_t
+    L.trt_dmd_create.argtypes = [vp, C.POINTER(vp), C.c_char_p, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp]
+    L.trt_dmd_reconstruct.argtypes = [vp, C.c_double]
+    L.trt_dmd_destroy.argtypes = [vp]
     L.trt_time.argtypes = [vp]
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
@@ -426,6 +429,31 @@ class PodBasis:
     def close(self):
         if self.h:
             self.L.trt_pod_destroy(self.h)
+            self.h = None
+
+
+class DmdBasis:
+    """A DMD basis on the device reconstructing into one context array (DMD_RECONSTRUCT,
+    MainSrc/DMD_ROUTINES.f08:28-103).  L, B, W complex; W in PodBasis's U layout."""
+
+    def __init__(self, ctx: "Context", name: str, C_, L_, B, W, rank, gsum: int = 0):
+        self.ctx, self.L = ctx, ctx.L
+        c = np.ascontiguousarray(np.asarray(C_, dtype=np.float64).ravel())
+        cx = [np.ascontiguousarray(np.asarray(a, dtype=np.complex128).ravel()).view(np.float64) for a in (L_, B, W)]
+        rk = np.ascontiguousarray(np.asarray(rank, dtype=np.int32).ravel())
+        self._keep = [c, rk] + cx
+        h = C.c_void_p()
+        ctx._check(self.L.trt_dmd_create(ctx.h, C.byref(h), name.encode(), int(gsum),
+                                         rk.ctypes.data_as(C.POINTER(C.c_int)), c.ctypes.data_as(_dp),
+                                         *[a.ctypes.data_as(_dp) for a in cx]), "dmd_create")
+        self.h = h
+
+    def reconstruct(self, time: float):
+        self.ctx._check(self.L.trt_dmd_reconstruct(self.h, float(time)), "dmd_reconstruct")
+
+    def close(self):
+        if self.h:
+            self.L.trt_dmd_destroy(self.h)
             self.h = None
 
 

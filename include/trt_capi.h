@@ -159,6 +159,17 @@ int trt_pod_destroy(trt_pod *pod);
  * context's stream, ms per reconstruction from HIP events; bytes of U */
 int trt_pod_timed(trt_pod *pod, int reps, double *ms);
 size_t trt_pod_basis_bytes(const trt_pod *pod);
+/* DMD reconstruction of one field of the ROM runs: DMD_RECONSTRUCT,
+ * MainSrc/DMD_ROUTINES.f08:28-103, as DMD_RECONSTRUCT_fg calls it (:131): the real part
+ * of sum_r B_r W_r exp(L_r t), plus C.  L, B (eigenvalues, amplitudes; sum of ranks) and W
+ * (modes, in trt_pod_create's U layout) are complex, stored as (re, im) double pairs as
+ * Fortran's COMPLEX*16; rank as for trt_pod_create.  trt_dmd_reconstruct writes the field
+ * at time t (the reference passes Time - Start_Time_d, ALGORITHMS.f08:617). */
+typedef struct trt_dmd trt_dmd;
+int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, const int *rank, const double *C,
+                   const double *L, const double *B, const double *W);
+int trt_dmd_reconstruct(trt_dmd *dmd, double time);
+int trt_dmd_destroy(trt_dmd *dmd);
 double trt_time(const trt_ctx *ctx);
 
 /* Checkpoint and resume.  trt_restart_out ≙ RESTART_OUT, MainSrc/OUTPUTS.f08:1624-1674

@@ -7,10 +7,14 @@ concatenated over the groups, V of a group laid out t-major, rank(g) values per 
 time), PODgsum 1 one basis for the whole vector.  LINEAR_INTERPOLATION
 (MainSrc/GRID_FUNCTIONS.f08:819-836): y1 + (x - x1)(y2 - y1)/(x2 - x1) per element.
 
+DMD_RECONSTRUCT (MainSrc/DMD_ROUTINES.f08:28-103): the real part of sum_r B_r W_r
+exp(L_r t), accumulated as (B W) exp(L t) in the loop order of :48-92, plus C.
+
 Parity pinning: tests/test_oracle_rom.py checks that a full-rank basis built from a
 snapshot matrix by numpy's SVD gives every snapshot back (which fixes the storage orders
 of U, S and V against the algebra X - C = U diag(S) V^T) and that interpolation at a
-midpoint gives the mean of the two snapshots.
+midpoint gives the mean of the two snapshots; and that DMD modes and amplitudes of a
+linear system dx/dt = A x (the eigendecomposition of A) reproduce scipy's expm(A t) x0.
 """
 from __future__ import annotations
 
@@ -58,3 +62,24 @@ def basis_from_snapshots(X, ng, gsum, rank=None):
         Vs.append(vt[:rk, :].T.ravel())             # t-major: rk values per data time
         ranks.append(rk)
     return C, np.concatenate(Ss), np.concatenate(Us), np.concatenate(Vs), np.array(ranks, dtype=np.int32)
+
+
+def dmd_reconstruct(C, L, B, W, rank, len_, ng, t, gsum):
+    """Field at time t (DMD_ROUTINES.f08:48-96)."""
+    dat = np.zeros(len_, dtype=np.complex128)
+    if gsum == 0:
+        glen = len_ // ng
+        for g in range(ng):
+            rk = int(rank[g])
+            p3 = int(np.sum(rank[:g]))
+            p2 = p3 * glen
+            for r in range(rk):
+                e = np.exp(L[p3 + r] * t)
+                blk = slice(g * glen, (g + 1) * glen)
+                dat[blk] = dat[blk] + (B[p3 + r] * W[p2:p2 + glen]) * e
+                p2 += glen
+    else:
+        for r in range(int(rank[0])):
+            e = np.exp(L[r] * t)
+            dat = dat + (B[r] * W[r * len_:(r + 1) * len_]) * e
+    return dat.real + C

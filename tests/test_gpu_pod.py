@@ -81,3 +81,26 @@ def test_rom_step_from_pod_bases(trt, oracle_mod):
     for p in pods:
         p.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("name,gsum", [("fg_edgV_xy_next", 0), ("Cg_L_next", 1), ("fg_avg_yy_next", 0)])
+def test_dmd_reconstruct_matches_oracle(trt, name, gsum):
+    prob = trt.problem.small(nx=9, ny=7, order=4, ng=3)
+    ctx = trt.native.Context(prob)
+    n = ctx.size(name)
+    ng = 1 if gsum else prob.ng
+    glen = n // ng
+    rng = np.random.default_rng(n + 7 * gsum)
+    rank = rng.integers(1, 5, ng)
+    tot = int(rank.sum())
+    L = rng.uniform(-2.0, 0.5, tot) + 1j * rng.uniform(-3.0, 3.0, tot)
+    B = rng.standard_normal(tot) + 1j * rng.standard_normal(tot)
+    W = rng.standard_normal(glen * tot) + 1j * rng.standard_normal(glen * tot)
+    C = rng.uniform(0.2, 0.4, n)
+    dmd = trt.native.DmdBasis(ctx, name, C, L, B, W, rank, gsum)
+    for t in (0.0, 0.013, 0.4):
+        dmd.reconstruct(t)
+        ref = rom.dmd_reconstruct(C, L, B, W, rank, n, prob.ng, t, gsum)
+        assert _rel(ctx.pull(name), ref) < 1e-13, (t, _rel(ctx.pull(name), ref))
+    dmd.close()
+    ctx.close()

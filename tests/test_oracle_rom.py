@@ -38,3 +38,31 @@ def test_truncation_and_interpolation():
     f4 = rom.pod_reconstruct(C, S, U, V, rk, ng * glen, ng, nt, 4, 0)
     mid = rom.linear_interpolation(f3, f4, 0.25, 0.2, 0.3)
     assert np.allclose(mid, 0.5 * (f3 + f4), atol=1e-14)
+
+
+@pytest.mark.parametrize("gsum", [0, 1])
+def test_dmd_of_a_linear_system_reproduces_expm(gsum):
+    """For dx/dt = A x with A = W diag(L) W^-1 (real A, so complex-conjugate modes), the
+    DMD reconstruction with amplitudes B = W^-1 x0 is expm(A t) x0, group by group."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(11 + gsum)
+    ng, glen = (1, 12) if gsum else (3, 6)
+    C = rng.uniform(0.1, 0.2, ng * glen)
+    Ls, Bs, Ws, ranks, want = [], [], [], [], {}
+    As, x0s = [], []
+    for g in range(ng):
+        A = rng.standard_normal((glen, glen)) * 0.5
+        x0 = rng.standard_normal(glen)
+        lam, vec = np.linalg.eig(A)
+        b = np.linalg.solve(vec, x0)
+        Ls.append(lam)
+        Bs.append(b)
+        Ws.append(vec.T.ravel())                     # rank columns of glen values
+        ranks.append(glen)
+        As.append(A)
+        x0s.append(x0)
+    L, B, W = np.concatenate(Ls), np.concatenate(Bs), np.concatenate(Ws)
+    for t in (0.0, 0.3, 1.1):
+        f = rom.dmd_reconstruct(C, L, B, W, np.array(ranks), ng * glen, ng, t, gsum)
+        ref = np.concatenate([expm(A * t) @ x0 for A, x0 in zip(As, x0s)]) + C
+        assert np.allclose(f, ref, rtol=1e-10, atol=1e-10), t
