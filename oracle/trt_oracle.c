@@ -22,6 +22,7 @@
  */
 #include "trt_oracle.h"
 
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
