@@ -47,6 +47,32 @@ struct trt_dmd {
 
 namespace {
 
+// BCg_next -> Cg_L/B/R/T_next: per group the data vector holds the left side (N_y
+// values), the bottom (N_x), the right (N_y), the top (N_x) (POD_ROUTINES.f08:279-352,
+// with the data grid the simulation grid)
+__global__ void bcg_split_kernel(const double *__restrict__ bc, double *__restrict__ L, double *__restrict__ B,
+                                 double *__restrict__ R, double *__restrict__ T, int nx, int ny, int ng)
+{
+    const int per = 2 * (nx + ny);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < per * ng; k += gridDim.x * blockDim.x) {
+        const int g = k / per, j = k - g * per;
+        const double v = bc[k];
+        if (j < ny) L[j + ny * g] = v;
+        else if (j < ny + nx) B[(j - ny) + nx * g] = v;
+        else if (j < 2 * ny + nx) R[(j - ny - nx) + ny * g] = v;
+        else T[(j - 2 * ny - nx) + nx * g] = v;
+    }
+}
+
+void bcg_split_if(trt::Ctx &c, const std::string &name)
+{
+    if (name != "BCg_next") return;
+    const int n = 2 * (c.nx + c.ny) * c.ng;
+    hipLaunchKernelGGL(bcg_split_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.arr("BCg_next"),
+                       c.arr("Cg_L_next"), c.arr("Cg_B_next"), c.arr("Cg_R_next"), c.arr("Cg_T_next"), c.nx, c.ny, c.ng);
+    TRT_CHECK(hipGetLastError());
+}
+
 __global__ void dmd_kernel(const double *__restrict__ C, const double2 *__restrict__ W, const double2 *__restrict__ B,
                            const double2 *__restrict__ E, const long long *__restrict__ woff,
                            const int *__restrict__ soff, const int *__restrict__ rank, size_t len, size_t glen,
@@ -245,6 +271,7 @@ int trt_pod_reconstruct(trt_pod *p, int t1, int t2, double x, double x1, double 
         if (t1 < 1 || t1 > p->nt || t2 < 1 || t2 > p->nt) throw trt::Error("data time index out of range");
         if (t1 != t2 && x2 == x1) throw trt::Error("interpolation between two data times at the same time");
         pod_launch(p, c.arr(p->name.c_str()), c.stream, t1 - 1, t2 - 1, x, x1, x2);
+        bcg_split_if(c, p->name);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -337,6 +364,7 @@ int trt_dmd_reconstruct(trt_dmd *d, double time)
                            reinterpret_cast<const double2 *>(d->B_dev), reinterpret_cast<const double2 *>(d->E),
                            d->woff, d->soff, d->rank, d->len, d->glen, d->gsum, c.arr(d->name.c_str()));
         TRT_CHECK(hipGetLastError());
+        bcg_split_if(c, d->name);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }

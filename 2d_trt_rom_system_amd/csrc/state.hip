@@ -246,7 +246,7 @@ size_t state_bytes(const trt_problem &p)
 {
     const size_t na = (size_t)p.nx * p.ny, nv = (size_t)(p.nx + 1) * p.ny, nh = (size_t)p.nx * (p.ny + 1);
     const size_t nbd = (size_t)p.nx + p.ny;
-    size_t n = (27 * na + 14 * nv + 14 * nh + 8 * nbd) * p.ng;       // per-group arrays of init_state
+    size_t n = (27 * na + 14 * nv + 14 * nh + 10 * nbd) * p.ng;      // per-group arrays of init_state
     n += 40 * na + 6 * nv + 6 * nh + 6 * nbd + 3 * (size_t)p.nm + p.ng + 1 + p.nmat + p.nx + p.ny;
     n += std::max<size_t>(4096, 8 * (size_t)std::max(p.nx, p.ny) * p.ng + 4096) + 4 * (size_t)p.ng;   // scratch, bcin
     return n * 8 + na * sizeof(int);
@@ -441,6 +441,8 @@ void init_state(Ctx &c)
     for (auto n : {"fg_edgH_yy_next", "fg_edgH_xy_next"}) c.reg(n, nx, ny + 1, ng);
     for (auto n : {"Cg_L_next", "Cg_R_next"}) c.reg(n, ny, ng);
     for (auto n : {"Cg_B_next", "Cg_T_next"}) c.reg(n, nx, ng);
+    // C_g of all four sides as POD/DMD_RECONSTRUCT_BCg hold it, per group [L | B | R | T]
+    c.reg("BCg_next", 2 * (nx + ny), ng);
     const char *grey[] = {"HO_E_avg", "MGQD_E_avg", "Temp", "Temp_Old", "E_avg", "E_avg_old", "KapE_Bar",
                           "KapE_Bar_old", "KapE_Bar_dT", "KapE_bar_MGQDold", "GQD_Src", "GQD_Src_old", "Gold_Hat",
                           "Rhat_old", "DC_xx", "DL_xx", "DR_xx", "DC_yy", "DB_yy", "DT_yy", "DL_xy", "DR_xy", "DB_xy",

@@ -155,6 +155,10 @@ int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int 
                    const double *C, const double *S, const double *U, const double *V);
 int trt_pod_reconstruct(trt_pod *pod, int t1, int t2, double x, double x1, double x2);
 int trt_pod_destroy(trt_pod *pod);
+/* For C_g (POD_RECONSTRUCT_BCg / DMD_RECONSTRUCT_BCg, MainSrc/POD_ROUTINES.f08:253-353) the
+ * basis reconstructs into BCg_next, shape (2(N_x+N_y), N_g): per group the left side
+ * (N_y values), the bottom (N_x), the right (N_y), the top (N_x); each reconstruction into
+ * it is split into Cg_L_next, Cg_B_next, Cg_R_next, Cg_T_next. */
 /* timing (no counterpart in the reference): reps reconstructions at data time 1 on the
  * context's stream, ms per reconstruction from HIP events; bytes of U */
 int trt_pod_timed(trt_pod *pod, int reps, double *ms);

@@ -83,3 +83,15 @@ def dmd_reconstruct(C, L, B, W, rank, len_, ng, t, gsum):
             e = np.exp(L[r] * t)
             dat = dat + (B[r] * W[r * len_:(r + 1) * len_]) * e
     return dat.real + C
+
+
+def split_bcg(bc, nx, ny, ng):
+    """Cg_L, Cg_B, Cg_R, Cg_T (Fortran shapes) from the boundary data vector of
+    POD_RECONSTRUCT_BCg (POD_ROUTINES.f08:279-352): per group L (N_y), B (N_x), R (N_y),
+    T (N_x), on the simulation grid."""
+    per = 2 * (nx + ny)
+    L, B, R, T = (np.zeros((ny, ng)), np.zeros((nx, ng)), np.zeros((ny, ng)), np.zeros((nx, ng)))
+    for g in range(ng):
+        v = bc[g * per:(g + 1) * per]
+        L[:, g], B[:, g], R[:, g], T[:, g] = v[:ny], v[ny:ny + nx], v[ny + nx:2 * ny + nx], v[2 * ny + nx:]
+    return L, B, R, T

@@ -104,3 +104,23 @@ def test_dmd_reconstruct_matches_oracle(trt, name, gsum):
         assert _rel(ctx.pull(name), ref) < 1e-13, (t, _rel(ctx.pull(name), ref))
     dmd.close()
     ctx.close()
+
+
+def test_bcg_reconstruction_splits_into_the_four_sides(trt):
+    prob = trt.problem.small(nx=9, ny=7, order=4, ng=3)
+    ctx = trt.native.Context(prob)
+    n = ctx.size("BCg_next")
+    assert n == 2 * (prob.nx + prob.ny) * prob.ng
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0.3, 0.6, (n, 4))
+    C, S, U, V, rk = rom.basis_from_snapshots(X, prob.ng, 0)
+    pod = trt.native.PodBasis(ctx, "BCg_next", C, S, U, V, rk, 4, 0)
+    pod.reconstruct(3)
+    bc = rom.pod_reconstruct(C, S, U, V, rk, n, prob.ng, 4, 3, 0)
+    assert _rel(ctx.pull("BCg_next"), bc) < 1e-14
+    # the split is a copy: bitwise against the split of the device's own vector
+    for side, ref in zip(("Cg_L_next", "Cg_B_next", "Cg_R_next", "Cg_T_next"),
+                         rom.split_bcg(ctx.pull("BCg_next"), prob.nx, prob.ny, prob.ng)):
+        assert np.array_equal(ctx.pull(side), ref.ravel(order="F")), side
+    pod.close()
+    ctx.close()
