@@ -98,6 +98,7 @@ void ctx_destroy(trt_ctx *ctx)
 {
     Ctx &c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    trt::rom_release_all(c);
     trt::pk_destroy(c);
     trt::direct_destroy(c);
     trt::nd_destroy(c);
@@ -185,6 +186,8 @@ int trt_push(trt_ctx *ctx, const char *name, const double *host, size_t n)
         TRT_CHECK(hipMemcpyAsync(c.arr(name), host, n * 8, hipMemcpyHostToDevice, c.stream));
         // the sweep reads KapE and RT_Src through its per-(cell, group) pair array
         if (!strcmp(name, "KapE") || !strcmp(name, "RT_Src")) trt::rebuild_sc(c);
+        // a C_g boundary vector reconstructed by the caller reaches the step through Cg_*_next
+        trt::bcg_split_if(c, name);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }

@@ -64,15 +64,6 @@ __global__ void bcg_split_kernel(const double *__restrict__ bc, double *__restri
     }
 }
 
-void bcg_split_if(trt::Ctx &c, const std::string &name)
-{
-    if (name != "BCg_next") return;
-    const int n = 2 * (c.nx + c.ny) * c.ng;
-    hipLaunchKernelGGL(bcg_split_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.arr("BCg_next"),
-                       c.arr("Cg_L_next"), c.arr("Cg_B_next"), c.arr("Cg_R_next"), c.arr("Cg_T_next"), c.nx, c.ny, c.ng);
-    TRT_CHECK(hipGetLastError());
-}
-
 __global__ void dmd_kernel(const double *__restrict__ C, const double2 *__restrict__ W, const double2 *__restrict__ B,
                            const double2 *__restrict__ E, const long long *__restrict__ woff,
                            const int *__restrict__ soff, const int *__restrict__ rank, size_t len, size_t glen,
@@ -216,6 +207,64 @@ int pguarded(trt_ctx *ctx, F &&f)
 
 }  // namespace
 
+namespace trt {
+
+void bcg_split_if(Ctx &c, const std::string &name)
+{
+    if (name != "BCg_next") return;
+    const int n = 2 * (c.nx + c.ny) * c.ng;
+    hipLaunchKernelGGL(bcg_split_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.arr("BCg_next"),
+                       c.arr("Cg_L_next"), c.arr("Cg_B_next"), c.arr("Cg_R_next"), c.arr("Cg_T_next"), c.nx, c.ny, c.ng);
+    TRT_CHECK(hipGetLastError());
+}
+
+}  // namespace trt
+
+namespace {
+
+void dmd_free(trt_dmd *d)
+{
+    for (void *m : {(void *)d->C, (void *)d->W, (void *)d->B_dev, (void *)d->E, (void *)d->woff, (void *)d->soff,
+                    (void *)d->rank})
+        if (m) (void)hipFree(m);
+    delete d;
+}
+
+// a basis may only write a registered FP64 context array (not I_crn / I_crn_old,
+// which are stored in the sweep's own layout and precision)
+size_t target_size(trt_ctx *ctx, const char *name)
+{
+    const trt::Ctx &c = ctx->c;
+    const size_t len = trt_array_size(ctx, name);
+    if (len == 0 || trt::derived_size(c, name) || c.arrays.find(name) == c.arrays.end())
+        throw trt::Error(std::string("no array ") + name + " to reconstruct into");
+    return len;
+}
+
+template <typename T>
+void unregister(std::vector<T *> &v, T *x)
+{
+    for (size_t k = 0; k < v.size(); ++k)
+        if (v[k] == x) {
+            v.erase(v.begin() + k);
+            return;
+        }
+}
+
+}  // namespace
+
+namespace trt {
+
+void rom_release_all(Ctx &c)
+{
+    for (trt_pod *p : c.pods) pod_free(p);
+    for (trt_dmd *d : c.dmds) dmd_free(d);
+    c.pods.clear();
+    c.dmds.clear();
+}
+
+}  // namespace trt
+
 extern "C" {
 
 int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int nt, const int *rank,
@@ -224,8 +273,7 @@ int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int 
     return pguarded(ctx, [&](trt::Ctx &c) {
         if (!out || !name || !rank || !C || !S || !U || !V) throw trt::Error("null argument");
         *out = nullptr;
-        const size_t len = trt_array_size(ctx, name);
-        if (len == 0 || trt::derived_size(c, name)) throw trt::Error(std::string("no array ") + name + " to reconstruct into");
+        const size_t len = target_size(ctx, name);
         if (gsum != 0 && gsum != 1) throw trt::Error("PODgsum must be 0 or 1");
         if (nt < 1) throw trt::Error("the basis needs at least one data time");
         const int ng = gsum ? 1 : c.ng;
@@ -260,6 +308,7 @@ int trt_pod_create(trt_ctx *ctx, trt_pod **out, const char *name, int gsum, int 
             pod_free(p);
             throw;
         }
+        c.pods.push_back(p);
         *out = p;
     });
 }
@@ -271,7 +320,7 @@ int trt_pod_reconstruct(trt_pod *p, int t1, int t2, double x, double x1, double 
         if (t1 < 1 || t1 > p->nt || t2 < 1 || t2 > p->nt) throw trt::Error("data time index out of range");
         if (t1 != t2 && x2 == x1) throw trt::Error("interpolation between two data times at the same time");
         pod_launch(p, c.arr(p->name.c_str()), c.stream, t1 - 1, t2 - 1, x, x1, x2);
-        bcg_split_if(c, p->name);
+        trt::bcg_split_if(c, p->name);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -300,8 +349,7 @@ int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, cons
     return pguarded(ctx, [&](trt::Ctx &c) {
         if (!out || !name || !rank || !C || !L || !B || !W) throw trt::Error("null argument");
         *out = nullptr;
-        const size_t len = trt_array_size(ctx, name);
-        if (len == 0 || trt::derived_size(c, name)) throw trt::Error(std::string("no array ") + name + " to reconstruct into");
+        const size_t len = target_size(ctx, name);
         if (gsum != 0 && gsum != 1) throw trt::Error("DMDgsum must be 0 or 1");
         const int ng = gsum ? 1 : c.ng;
         if (len % (size_t)ng) throw trt::Error(std::string(name) + " does not split into N_g equal group blocks");
@@ -319,12 +367,6 @@ int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, cons
             so += (size_t)rk[g];
         }
         trt_dmd *d = new trt_dmd;
-        auto cleanup = [](trt_dmd *q) {
-            for (void *m : {(void *)q->C, (void *)q->W, (void *)q->B_dev, (void *)q->E, (void *)q->woff,
-                            (void *)q->soff, (void *)q->rank})
-                if (m) (void)hipFree(m);
-            delete q;
-        };
         try {
             d->ctx = ctx; d->name = name; d->gsum = gsum; d->ng = ng; d->len = len; d->glen = glen;
             d->L.assign(L, L + 2 * so);
@@ -339,9 +381,10 @@ int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, cons
             d->rank = upload(rk.data(), rk.size(), c.stream, nullptr);
             TRT_CHECK(hipStreamSynchronize(c.stream));
         } catch (...) {
-            cleanup(d);
+            dmd_free(d);
             throw;
         }
+        c.dmds.push_back(d);
         *out = d;
     });
 }
@@ -364,7 +407,7 @@ int trt_dmd_reconstruct(trt_dmd *d, double time)
                            reinterpret_cast<const double2 *>(d->B_dev), reinterpret_cast<const double2 *>(d->E),
                            d->woff, d->soff, d->rank, d->len, d->glen, d->gsum, c.arr(d->name.c_str()));
         TRT_CHECK(hipGetLastError());
-        bcg_split_if(c, d->name);
+        trt::bcg_split_if(c, d->name);
         TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -373,10 +416,8 @@ int trt_dmd_destroy(trt_dmd *d)
 {
     if (!d) return -1;
     (void)hipStreamSynchronize(d->ctx->c.stream);
-    for (void *m : {(void *)d->C, (void *)d->W, (void *)d->B_dev, (void *)d->E, (void *)d->woff, (void *)d->soff,
-                    (void *)d->rank})
-        if (m) (void)hipFree(m);
-    delete d;
+    unregister(d->ctx->c.dmds, d);
+    dmd_free(d);
     return 0;
 }
 
@@ -384,6 +425,7 @@ int trt_pod_destroy(trt_pod *p)
 {
     if (!p) return -1;
     (void)hipStreamSynchronize(p->ctx->c.stream);
+    unregister(p->ctx->c.pods, p);
     pod_free(p);
     return 0;
 }

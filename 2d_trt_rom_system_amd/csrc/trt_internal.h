@@ -132,6 +132,10 @@ struct Ctx {
     DirectWork *direct = nullptr;
     NDPlan *nd = nullptr;
     int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback; 2: block LU
+    // POD and DMD bases created on this context: they use its stream and arrays, so
+    // the context frees any still open when it is destroyed
+    std::vector<trt_pod *> pods;
+    std::vector<trt_dmd *> dmds;
 
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
@@ -182,6 +186,11 @@ void push_icrn(Ctx &c, int which, const double *host);
 // checks the whole file before it changes any device array
 void restart_out(Ctx &c, const char *path);
 void restart_in(Ctx &c, const char *path);
+// C_g of the ROM runs: BCg_next (per group: left N_y, bottom N_x, right N_y, top N_x
+// values) split into Cg_L/B/R/T_next; a no-op for any other array name
+void bcg_split_if(Ctx &c, const std::string &name);
+// frees the POD/DMD bases still registered on the context (csrc/rom.hip)
+void rom_release_all(Ctx &c);
 void sweep_setup(Ctx &c);
 // persistent sweep (sweep_pk.hip)
 bool pk_supported(const Ctx &c);

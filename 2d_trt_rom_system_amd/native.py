@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -229,9 +230,14 @@ class Context:
         if rc != 0:
             raise TrtError(f"trt_ctx_create failed ({rc}): {self.L.trt_last_error(None).decode()}")
         self.h = h
+        self._bases = weakref.WeakSet()     # POD/DMD bases created on this context
 
     def close(self):
         if getattr(self, "h", None):
+            # a basis uses the context's stream: close it first (the library also frees
+            # any basis still open when the context is destroyed)
+            for b in list(getattr(self, "_bases", ())):
+                b.close()
             self.L.trt_ctx_destroy(self.h)
             self.h = None
 
@@ -290,8 +296,13 @@ class Context:
         self._check(self.L.trt_step(self.h, C.byref(it)), "step")
         return tuple(it)
 
-    def step_given_qdf(self):
-        """The ROM runs' time step: QD factors and C_g pushed by the caller, no sweep."""
+    def step_given_qdf(self, direc_diff: int = 0):
+        """The ROM runs' time step: QD factors and C_g pushed by the caller, no sweep.
+        direc_diff=1 zeroes the reconstructed off-diagonal face factors first, as the
+        reference does for Direc_Diff = 1 (MainSrc/ALGORITHMS.f08:593-596)."""
+        if direc_diff == 1:
+            for n in ("fg_edgV_xy_next", "fg_edgH_xy_next"):
+                self.push(n, np.zeros(self.size(n)))
         it = (C.c_int * 4)()
         self._check(self.L.trt_step_given_qdf(self.h, C.byref(it)), "step_given_qdf")
         return tuple(it)
@@ -411,6 +422,7 @@ class PodBasis:
                                          rk.ctypes.data_as(C.POINTER(C.c_int)), *[a.ctypes.data_as(_dp) for a in arrs]),
                    "pod_create")
         self.h = h
+        ctx._bases.add(self)
 
     def reconstruct(self, t1: int, t2: int | None = None, x: float = 0.0, x1: float = 0.0, x2: float = 0.0):
         """The field at data time t1 (1-based), or interpolated to x between t1 (at x1) and t2 (at x2)."""
@@ -427,9 +439,13 @@ class PodBasis:
         return int(self.L.trt_pod_basis_bytes(self.h))
 
     def close(self):
-        if self.h:
-            self.L.trt_pod_destroy(self.h)
+        if getattr(self, "h", None):
+            if self.ctx.h:                  # else the context's destroy has freed it
+                self.L.trt_pod_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class DmdBasis:
@@ -447,14 +463,19 @@ class DmdBasis:
                                          rk.ctypes.data_as(C.POINTER(C.c_int)), c.ctypes.data_as(_dp),
                                          *[a.ctypes.data_as(_dp) for a in cx]), "dmd_create")
         self.h = h
+        ctx._bases.add(self)
 
     def reconstruct(self, time: float):
         self.ctx._check(self.L.trt_dmd_reconstruct(self.h, float(time)), "dmd_reconstruct")
 
     def close(self):
-        if self.h:
-            self.L.trt_dmd_destroy(self.h)
+        if getattr(self, "h", None):
+            if self.ctx.h:
+                self.L.trt_dmd_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def _io_check(rc: int, what: str) -> None:

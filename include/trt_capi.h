@@ -134,7 +134,12 @@ int trt_step(trt_ctx *ctx, int its_out[4]);
  * _next).  The step keeps the current factors as the old level's, as the reference does at
  * its start, copies the _next arrays in where the reconstruction writes, and runs the
  * MLOQD/GLOQD iteration with one RT iteration and no sweep (maxit_RTE = 1,
- * MainSrc/INPUTS.f08:94-104).  its_out as trt_step; its_out[3] (sweeps) is 0. */
+ * MainSrc/INPUTS.f08:94-104).  its_out as trt_step; its_out[3] (sweeps) is 0.
+ * Direc_Diff = 1 runs zero fg_edgV_xy and fg_edgH_xy after the reconstruction
+ * (ALGORITHMS.f08:593-596): such a caller pushes zeros to fg_edgV_xy_next and
+ * fg_edgH_xy_next after reconstructing (the step copies them in unchanged).  A push of
+ * BCg_next (the reference's C_g boundary vector, POD_ROUTINES.f08:279-352) is split into
+ * Cg_L/B/R/T_next at once, as a reconstruction into BCg_next is. */
 int trt_step_given_qdf(trt_ctx *ctx, int its_out[4]);
 
 /* POD reconstruction of one field of the ROM runs: POD_RECONSTRUCT,
