@@ -63,6 +63,8 @@ extern "C" {
 
 namespace trt {
 
+void set_thread_error(const std::string &msg) { g_create_error = msg; }
+
 trt_ctx *ctx_create(const trt_problem *p, bool pooled, Arena *arena)
 {
     trt_ctx *ctx = nullptr;

@@ -342,11 +342,19 @@ __global__ void conv_partial(const double *x, const double *xo, const double *xo
     if (threadIdx.x == 0) for (int k = 0; k < 4; ++k) part[k * gridDim.x + blockIdx.x] = s[k][0];
 }
 
+int conv_blocks(size_t n) { return std::min(nb(n), 128); }
+
+void conv_partial_launch(const double *x, const double *xo, const double *xo2, int n, int conv_type, double *part,
+                         int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(conv_partial, dim3(blocks), dim3(256), 0, s, x, xo, xo2, n, conv_type, part);
+}
+
 void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, int n, int conv_type, double eps,
                  int its, double *norm, double *rho, int *conv)
 {
-    const int blocks = std::min(nb(n), 128);
-    hipLaunchKernelGGL(conv_partial, dim3(blocks), dim3(256), 0, c.stream, x, xo, xo2, n, conv_type, c.scratch);
+    const int blocks = conv_blocks((size_t)n);
+    conv_partial_launch(x, xo, xo2, n, conv_type, c.scratch, blocks, c.stream);
     TRT_CHECK(hipMemcpyAsync(c.hbuf, c.scratch, sizeof(double) * 4 * blocks, hipMemcpyDeviceToHost, c.stream));
     TRT_CHECK(hipStreamSynchronize(c.stream));
     double a = 0, b = 0, cc = 0, mn = 1.7976931348623157e308;

@@ -176,6 +176,14 @@ void collapse_mg_ef(Ctx &c);
 void cbar_calc(Ctx &c);
 void grey_coefs(Ctx &c);
 int egp_fv_newt(Ctx &c, int mgqd_it);
+// CONVERGENCE's one-pass reduction: partial blocks for n values, and the launch that
+// writes 4 x blocks partials (max |x - xo|, max |xo - xo2|, max |x| or max |1 - xo/x|,
+// min |x|) to part on stream s
+int conv_blocks(size_t n);
+void conv_partial_launch(const double *x, const double *xo, const double *xo2, int n, int conv_type, double *part,
+                         int blocks, hipStream_t s);
+// the message trt_last_error(NULL) returns on this thread
+void set_thread_error(const std::string &msg);
 void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, int n, int conv_type,
                  double eps, int its, double *norm, double *rho, int *conv);
 // ho_form 0: full-order step; 1: the ROM runs' step (QD factors and C_g from the context, no sweep)

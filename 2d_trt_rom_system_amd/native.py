@@ -53,6 +53,7 @@ EXPORTS = [
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
     "trt_frecords_last_error", "trt_step_given_qdf", "trt_pod_create", "trt_pod_reconstruct", "trt_pod_destroy",
     "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
+    "trt_convergence_noalloc",
 ]
 
 _lib = None
@@ -94,6 +95,7 @@ def load(path: str = LIBPATH):
     L.trt_array_size.restype = C.c_size_t
     L.trt_push.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
     L.trt_pull.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
+    L.trt_convergence_noalloc.argtypes = [vp, vp, vp, C.c_size_t, C.c_int, C.c_double, C.c_int, vp, vp]
     for f in ("trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
               "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_collapse_mg_ef", "trt_cbar_calc",
               "trt_grey_coefs", "trt_gqd_in_calc"):

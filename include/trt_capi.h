@@ -323,6 +323,19 @@ int trt_server_summary(const trt_server *s, trt_server_stats *out);
 size_t trt_server_output_size(const trt_server *s, int id, const char *name);
 int trt_server_output(const trt_server *s, int id, const char *name, double *host, size_t n);
 
+/* No counterpart in the reference: CONVERGENCE (MainSrc/CONVERGENCE_CHECKS.f08:7-44) for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  x, x_old, x_old2: n values each in the caller's
+ * device memory; out: 3 doubles of the caller's device memory receiving {norm, rho,
+ * converged (1.0 or 0.0)}, the values trt_convergence returns; stream: a hipStream_t (NULL:
+ * the default stream).  The call only enqueues two kernels on the stream: it allocates
+ * nothing, copies nothing to the host and does not synchronise.  The reduction's partials
+ * sit in a fixed device array of 4 KiB per GPU held until the process ends; n is limited
+ * to 2147483647.  Calls that overlap on one GPU share that array and race.  Errors
+ * (negative return): the message is trt_last_error(NULL). */
+int trt_convergence_noalloc(const double *x, const double *x_old, const double *x_old2, size_t n, int conv_type,
+                            double eps, int its, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
