@@ -51,14 +51,26 @@ class _Problem(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        build()
-        L = C.CDLL(LIB)
+def build_mutant(mutant: int, outdir: str) -> str:
+    """The oracle compiled with one deliberate mistake (ORC_MUTANT = mutant, see the
+    ORC_MUTANT switches in trt_oracle.c) into outdir: the pin tests check that each
+    mistake makes them fail.  Never used by anything but tests."""
+    path = os.path.join(outdir, f"liboracle_mutant{mutant}.so")
+    subprocess.check_call(["gcc", "-O2", "-fopenmp", "-fPIC", "-std=c99", "-shared", f"-DORC_MUTANT={int(mutant)}",
+                           "-o", path, os.path.join(HERE, "trt_oracle.c"), "-lm"])
+    return path
+
+
+def lib(path: str | None = None):
+    """The oracle library (or, with path, a mutant build of it)."""
+    key = path or LIB
+    if key not in _libs:
+        if path is None:
+            build()
+        L = C.CDLL(key)
         dp = C.POINTER(C.c_double)
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(_Problem)]
@@ -100,8 +112,8 @@ def lib():
         L.orc_qd_build_csr.restype = C.c_int
         L.orc_qd_fv.argtypes = [C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
         L.orc_qd_fv.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _dptr(a):
@@ -111,8 +123,8 @@ def _dptr(a):
 class Oracle:
     """One oracle state (the reference driver's arrays) for a Problem."""
 
-    def __init__(self, prob, threads: int | None = None):
-        L = lib()
+    def __init__(self, prob, threads: int | None = None, lib_path: str | None = None):
+        L = lib(lib_path)
         self.prob = prob
         self._keep = []
 

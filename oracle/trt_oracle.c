@@ -19,6 +19,13 @@
  *    module there, TEMP_FUNCTIONS.f08:131-134).
  *  - RT_Src is stored per (cell, group) because MATERIAL_UPDATE writes the same
  *    value to all four corners and all angles (UPDATES.f08:141-150).
+ *
+ * Pins (tests/test_oracle_*.py): closed forms, invariants and brute-force solves for
+ * every routine.  The grey-level routines (GREY_COEFS, Cbar_Calc, OLD_GREY_COEFS,
+ * COLLAPSE_MG_EF, GQD_In_Calc) are pinned by the group-sum identity -- the collapsed
+ * multigroup solution satisfies the grey equations built from their output -- and by
+ * discrete energy conservation; ORC_MUTANT builds show that planted index, weight and
+ * time-level mistakes fail those pins.
  */
 #include "trt_oracle.h"
 
@@ -30,6 +37,13 @@
 #include <stdint.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* ORC_MUTANT: deliberate mistakes compiled in only by the pin tests
+ * (tests/test_oracle_grey_pins.py, through oracle.build_mutant) to show that the
+ * pins catch them.  0 -- the oracle itself -- in every other build. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 #endif
 
 static const double PI = 3.14159265358979323846264338327950288;
@@ -625,8 +639,8 @@ void orc_old_grey_coefs(orc_state *s)
         s->Gold_Hat[k] = t;
     }
     for (size_t k = 0; k < n; ++k) {
-        s->Gold_Hat[k] = s->Gold_Hat[k] + s->A[k] * s->E_avg_old[k] / (th * dt);
-        s->Rhat_old[k] = cv / (th * dt) * s->Temp_Old[k]
+        s->Gold_Hat[k] = s->Gold_Hat[k] + s->A[k] * (ORC_MUTANT == 7 ? s->E_avg : s->E_avg_old)[k] / (th * dt);
+        s->Rhat_old[k] = (ORC_MUTANT == 8 ? 2.0 * cv : cv) / (th * dt) * s->Temp_Old[k]
                          + ((1.0 - th) / th) * (c * s->KapE_Bar_old[k] * s->E_avg_old[k] - s->GQD_Src_old[k]);
     }
 }
@@ -990,7 +1004,7 @@ void orc_collapse_mg_ef(orc_state *s)
     for (size_t k = 0; k < n; ++k) { double t = 0; for (int g = 0; g < ng; ++g) t += s->Eg_avg[k + g * n]; s->MGQD_E_avg[k] = t; }
     for (size_t k = 0; k < nv; ++k) {
         double t = 0, f = 0;
-        for (int g = 0; g < ng; ++g) { t += s->Eg_edgV[k + g * nv]; f += s->Fxg_edgV[k + g * nv]; }
+        for (int g = 0; g < ng; ++g) { t += s->Eg_edgV[k + g * nv]; f += (ORC_MUTANT == 6 ? s->Fxg_edgV_old : s->Fxg_edgV)[k + g * nv]; }
         s->MGQD_E_edgV[k] = t; s->MGQD_Fx_edgV[k] = f;
     }
     for (size_t k = 0; k < nh; ++k) {
@@ -1008,7 +1022,7 @@ void orc_cbar_calc(orc_state *s)
     for (int j = 0; j < ny; ++j) {
         double cl = 0, cr = 0, s1 = 0, s2 = 0;
         for (int g = 0; g < ng; ++g) {
-            double q = s->Eg_edgV[g * nv + IX2(0, j, nx + 1)] - s->Eg_in_L[j + g * ny];
+            double q = s->Eg_edgV[g * nv + IX2(0, j, nx + 1)] - (ORC_MUTANT == 5 ? 0.0 : s->Eg_in_L[j + g * ny]);
             cl += s->Cg_L[j + g * ny] * q; s1 += q;
             q = s->Eg_edgV[g * nv + IX2(nx, j, nx + 1)] - s->Eg_in_R[j + g * ny];
             cr += s->Cg_R[j + g * ny] * q; s2 += q;
@@ -1047,15 +1061,15 @@ void orc_grey_coefs(orc_state *s)
                 s->KapE_Bar[k] += s->KapE[kg] * s->Eg_avg[kg];
                 s->DC_xx[k] += s->fg_avg_xx[kg] * s->Eg_avg[kg] / tk;
                 s->DL_xx[k] += s->fg_edgV_xx[vL] * s->Eg_edgV[vL] / tk;
-                s->DR_xx[k] += s->fg_edgV_xx[vR] * s->Eg_edgV[vR] / tk;
+                s->DR_xx[k] += s->fg_edgV_xx[ORC_MUTANT == 1 ? vL : vR] * s->Eg_edgV[vR] / tk;
                 s->DC_yy[k] += s->fg_avg_yy[kg] * s->Eg_avg[kg] / tk;
                 s->DB_yy[k] += s->fg_edgH_yy[hB] * s->Eg_edgH[hB] / tk;
                 s->DT_yy[k] += s->fg_edgH_yy[hT] * s->Eg_edgH[hT] / tk;
                 s->DL_xy[k] += s->fg_edgV_xy[vL] * s->Eg_edgV[vL] / tk;
                 s->DR_xy[k] += s->fg_edgV_xy[vR] * s->Eg_edgV[vR] / tk;
                 s->DB_xy[k] += s->fg_edgH_xy[hB] * s->Eg_edgH[hB] / tk;
-                s->DT_xy[k] += s->fg_edgH_xy[hT] * s->Eg_edgH[hT] / tk;
-                s->PL[k] += (Fdt * A / (2.0 * th * c * dt) * s->Fxg_edgV_old[vL] + s->Pold_L[kg]) / tk;
+                s->DT_xy[k] += s->fg_edgH_xy[ORC_MUTANT == 2 ? hB : hT] * s->Eg_edgH[hT] / tk;
+                s->PL[k] += (Fdt * A / (2.0 * th * c * dt) * s->Fxg_edgV_old[ORC_MUTANT == 3 ? vR : vL] + s->Pold_L[kg]) / tk;
                 s->PR[k] += (Fdt * A / (2.0 * th * c * dt) * s->Fxg_edgV_old[vR] + s->Pold_R[kg]) / tk;
                 s->PB[k] += (Fdt * A / (2.0 * th * c * dt) * s->Fyg_edgH_old[hB] + s->Pold_B[kg]) / tk;
                 s->PT[k] += (Fdt * A / (2.0 * th * c * dt) * s->Fyg_edgH_old[hT] + s->Pold_T[kg]) / tk;
@@ -1073,7 +1087,7 @@ void orc_grey_coefs(orc_state *s)
             }
             s->KapE_Bar[k] /= s1;
             s->DC_xx[k] /= s1; s->DL_xx[k] /= s2; s->DR_xx[k] /= s3;
-            s->DC_yy[k] /= s1; s->DB_yy[k] /= s4; s->DT_yy[k] /= s5;
+            s->DC_yy[k] /= (ORC_MUTANT == 4 ? s2 : s1); s->DB_yy[k] /= s4; s->DT_yy[k] /= s5;
             s->DL_xy[k] /= s2; s->DR_xy[k] /= s3; s->DB_xy[k] /= s4; s->DT_xy[k] /= s5;
         }
 }
