@@ -48,6 +48,10 @@ void validate(const trt_problem *p)
     if (p->precision == 2 && p->sweep_mode == 1)
         throw Error("precision 2 runs on the tile engine (sweep_mode 0, 2, 3 or 4)");
     if (p->sweep_mode < 0 || p->sweep_mode > 4) throw Error("sweep_mode must be 0, 1, 2, 3 or 4");
+    if (p->grey_emission != 0 && p->grey_emission != 1) throw Error("grey_emission must be 0 or 1");
+    if (p->no_grey != 0 && p->no_grey != 1) throw Error("no_grey must be 0 or 1");
+    if (p->use_safety_search && !(p->line_src > 1.0))
+        throw Error("the safety search needs a line-search factor line_src above 1");
     if (p->qd_solver < 0 || p->qd_solver > 2)
         throw Error("qd_solver must be 0 (nested dissection), 1 (BiCGSTAB) or 2 (block LU)");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
@@ -262,6 +266,15 @@ int trt_convergence(trt_ctx *ctx, const char *x, const char *x_old, const char *
         if (n == 0 || n != trt_array_size(ctx, x_old) || n != trt_array_size(ctx, x_old2))
             throw Error("convergence arrays must exist and have equal sizes");
         trt::convergence(c, c.arr(x), c.arr(x_old), c.arr(x_old2), (int)n, conv_type, eps, its, norm, rho, converged);
+    });
+}
+
+int trt_meb_solve(trt_ctx *ctx, const char *eg)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!eg) throw Error("null argument");
+        trt::meb_solve(c, eg);
+        TRT_CHECK(hipStreamSynchronize(c.stream));
     });
 }
 

@@ -15,6 +15,7 @@
 // face.  All N_g group systems of MLOQD_FV are solved as one batch by a
 // Jacobi-preconditioned BiCGSTAB; the grey Newton system is a batch of one.
 #include "qd_view.h"
+#include "physics_device.h"
 
 #include <chrono>
 
@@ -1030,6 +1031,11 @@ void grey_coefs(Ctx &c)
 struct NewtArgs {
     int nx, ny, nbd, damp;
     double c, th, dt, cv, chi, qfac, Fin, Ein;
+    // grey_emission 1: Q_bar from the group sources (opacity model, groups, multipliers)
+    int gem, ng, opm;
+    double cu;
+    const double *nu, *kmult;
+    const int *mat;
     const double *A, *dx, *dy, *Rh, *Gh;
     const double *DCxx, *DLxx, *DRxx, *DCyy, *DByy, *DTyy, *DLxy, *DRxy, *DBxy, *DTxy, *PL, *PR, *PB, *PT;
     const double *CbL, *CbB, *CbR, *CbT, *EinL, *EinB, *EinR, *EinT, *FinL, *FinB, *FinR, *FinT;
@@ -1038,6 +1044,34 @@ struct NewtArgs {
     double *dT, *dEa, *dEv, *dEh, *dFx, *dFy;
     double *coef, *bnd;
 };
+
+// grey_emission 1 (a labelled deviation, off by default): the grey emission consistent
+// with MATERIAL_UPDATE's group sources (UPDATES.f08:130-152), Q_bar(T) = sum_g 4 pi
+// kappa_g(T) mult B_g(T) in cell k, and its T-derivative by a central difference of step
+// 1e-4 T (the derivative steers the Newton iteration; its fixed point does not depend on it)
+__device__ double qbar_groups(const NewtArgs &a, size_t k, double T)
+{
+    const double mult = a.kmult[a.mat[k]];
+    double q = 0.0;
+    for (int g = 0; g < a.ng; ++g) {
+        const double nl = a.nu[g], nh = a.nu[g + 1];
+        const double kap = a.opm == 0 ? d_kapB(T, nl, nh) : d_kap_mid(T, nl, nh);
+        q += 4.0 * PI * (kap * mult) * d_bg_planck(T, nl, nh, a.cu);
+    }
+    return q;
+}
+
+__device__ void qbar(const NewtArgs &a, size_t k, double T, double *Qb, double *QbdT)
+{
+    if (a.gem == 1) {
+        const double h = 1e-4 * T;
+        *Qb = qbar_groups(a, k, T);
+        *QbdT = (qbar_groups(a, k, T + h) - qbar_groups(a, k, T - h)) / (2.0 * h);
+    } else {
+        *Qb = a.qfac * T;        // GL:439-440
+        *QbdT = a.qfac;
+    }
+}
 
 // residuals and cell coefficients (GL:437-513)
 __global__ void newt_cells(NewtArgs a)
@@ -1048,7 +1082,8 @@ __global__ void newt_cells(NewtArgs a)
         const size_t vL = i + (size_t)(a.nx + 1) * j, vR = vL + 1, hB = i + (size_t)a.nx * j, hT = hB + a.nx;
         const double A = a.A[k], dx = a.dx[i], dy = a.dy[j], c = a.c, th = a.th, dt = a.dt, cv = a.cv;
         const double T = a.T[k], Ea = a.Ea[k], KB = a.KB[k];
-        const double Qb = a.qfac * T, QbdT = a.qfac;
+        double Qb, QbdT;
+        qbar(a, k, T, &Qb, &QbdT);
         a.Qb[k] = Qb;
         double kdt = a.KBdT[k];
         if (a.damp) {                                   // GL:444-454
@@ -1201,8 +1236,32 @@ __global__ void newt_update(NewtArgs a)
         a.T[k] += a.dT[k];
         a.Ea[k] += a.dEa[k];
         a.KB[k] = a.KB[k] + a.KBdT[k] * a.dT[k];
-        a.Qb[k] = a.qfac * a.T[k];
+        a.Qb[k] = a.gem == 1 ? qbar_groups(a, k, a.T[k]) : a.qfac * a.T[k];   // GL:766
     }
+}
+
+// Positivity safety search (GL:643-752): per checked value x with step d, the number of
+// times the step length must be divided by line_src before x + d / LS reaches the lower
+// bound, the step length built by repeated multiplication as the reference builds it.
+// The reference divides every delta by each cell's LS in turn, so the total division is
+// line_src^(largest count); counts are combined by atomicMax into *emax.  A value already
+// below its bound never reaches it (the reference loops forever): the count is capped at
+// kSafeCap and reported.
+constexpr int kSafeCap = 4096;
+__global__ void safety_count_kernel(const double *x, const double *d, size_t n, double bound, double line_src,
+                                    int *emax)
+{
+    int best = 0;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        double LS = 1.0;
+        int p = 0;
+        while (x[k] + d[k] / LS < bound && p < kSafeCap) {
+            LS *= line_src;
+            ++p;
+        }
+        best = max(best, p);
+    }
+    if (best > 0) atomicMax(emax, best);
 }
 
 // dKapE_Bar/dT by finite difference over MLOQD iterates (GL:402-417)
@@ -1218,7 +1277,6 @@ __global__ void kbdt_kernel(size_t n, const double *T, const double *Tm, const d
 
 int egp_fv_newt(Ctx &c, int mgqd_it)
 {
-    if (c.p.use_safety_search) throw Error("the positivity safety search (Use_Safety_Search) is not supported");
     if (c.grey.nsys == 0) qd_alloc(c, c.grey, 1);
     const int nx = c.nx, ny = c.ny;
     const size_t nc = (size_t)nx * ny, nv = (size_t)(nx + 1) * ny, nh = (size_t)nx * (ny + 1);
@@ -1236,6 +1294,8 @@ int egp_fv_newt(Ctx &c, int mgqd_it)
     a.c = cc; a.th = c.p.theta; a.dt = c.p.dt; a.cv = c.p.cv; a.chi = c.p.chi;
     a.qfac = 15.0 * c.p.kap0 * cc * aR / (PI * PI * PI * PI);
     a.Fin = 1.0; a.Ein = 1.0;
+    a.gem = c.p.grey_emission; a.ng = c.ng; a.opm = c.p.opacity_model; a.cu = c.p.comp_unit;
+    a.nu = c.nu; a.kmult = c.kmult; a.mat = c.mat;
     a.A = c.A; a.dx = c.dx; a.dy = c.dy; a.Rh = c.arr("Rhat_old"); a.Gh = c.arr("Gold_Hat");
     a.DCxx = c.arr("DC_xx"); a.DLxx = c.arr("DL_xx"); a.DRxx = c.arr("DR_xx");
     a.DCyy = c.arr("DC_yy"); a.DByy = c.arr("DB_yy"); a.DTyy = c.arr("DT_yy");
@@ -1310,6 +1370,30 @@ int egp_fv_newt(Ctx &c, int mgqd_it)
                 for (int q = 0; q < 6; ++q)
                     hipLaunchKernelGGL(scale_kernel, dim3(nb(sz[q])), dim3(256), 0, c.stream, del[q], sz[q], S);
         }
+        if (c.p.use_safety_search) {
+            // T and E_avg (cells), E_edgV, E_edgH against their lower bounds (GL:643-752)
+            int *emax = reinterpret_cast<int *>(part + 12 * PB);
+            TRT_CHECK(hipMemsetAsync(emax, 0, sizeof(int), c.stream));
+            const double *xs[4] = {a.T, a.Ea, a.Ev, a.Eh};
+            const double *ds[4] = {a.dT, a.dEa, a.dEv, a.dEh};
+            const size_t ns[4] = {nc, nc, nv, nh};
+            const double bnd[4] = {c.p.T_bound_low, c.p.E_bound_low, c.p.E_bound_low, c.p.E_bound_low};
+            for (int q = 0; q < 4; ++q)
+                hipLaunchKernelGGL(safety_count_kernel, dim3(std::min(nb(ns[q]), 1024)), dim3(256), 0, c.stream, xs[q],
+                                   ds[q], ns[q], bnd[q], c.p.line_src, emax);
+            int e = 0;
+            TRT_CHECK(hipMemcpyAsync(&e, emax, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            TRT_CHECK(hipStreamSynchronize(c.stream));
+            if (e >= kSafeCap)
+                throw Error("EGP_FV_NEWT safety search: a temperature or energy density is already below its lower "
+                            "bound, so no step length reaches it");
+            if (e > 0) {
+                double S = 1.0;
+                for (int t = 0; t < e; ++t) S *= c.p.line_src;
+                for (int q = 0; q < 6; ++q)
+                    hipLaunchKernelGGL(scale_kernel, dim3(nb(sz[q])), dim3(256), 0, c.stream, del[q], sz[q], S);
+            }
+        }
         hipLaunchKernelGGL(newt_update, dim3(nb(nc)), dim3(256), 0, c.stream, a);
         for (int q = 2; q < 6; ++q)
             hipLaunchKernelGGL(axpy1_kernel, dim3(nb(sz[q])), dim3(256), 0, c.stream, val[q], del[q], sz[q]);
@@ -1328,6 +1412,39 @@ int egp_fv_newt(Ctx &c, int mgqd_it)
         }
     }
     return its;
+}
+
+// ---------------------------------------------------------------------------
+// MEB_SOLVE, UPDATES.f08:189-231: closed-form grey temperature (use_grey = 0)
+// ---------------------------------------------------------------------------
+__global__ void meb_kernel(double *T, const double *Eg, const double *KapE, const double *Told, size_t n, int ng,
+                           double fac, double dt, double cv)
+{
+    const double cc = C_CONST;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        double E = 0.0, kb = 0.0;
+        for (int g = 0; g < ng; ++g) {
+            E += Eg[k + g * n];
+            kb += KapE[k + g * n] * Eg[k + g * n];
+        }
+        kb /= E;
+        T[k] = fac * (dt * cc * kb * E + cv * Told[k]);
+    }
+}
+
+void meb_solve(Ctx &c, const char *eg)
+{
+    const size_t n = (size_t)c.nx * c.ny;
+    auto it = c.arrays.find(eg);
+    if (it == c.arrays.end() || it->second.size() != n * c.ng)
+        throw Error(std::string("MEB_SOLVE needs a group energy array (N_x,N_y,N_g), not ") + eg);
+    // (c^2 h^3)/(c^2 h^3 cv cu + 8 pi dt kap0 / erg^4) ... * cu, with the hard-coded c (UP:196)
+    const double cc = C_CONST, h3 = H_PLANCK * H_PLANCK * H_PLANCK, e4 = ERG * ERG * ERG * ERG;
+    const double fac = (cc * cc * h3) / (cc * cc * h3 * c.p.cv * c.p.comp_unit + 8.0 * PI * c.p.dt * c.p.kap0 / e4)
+                       * c.p.comp_unit;
+    hipLaunchKernelGGL(meb_kernel, dim3(nb(n)), dim3(256), 0, c.stream, c.arr("Temp"), it->second.p, c.arr("KapE"),
+                       c.arr("Temp_Old"), n, c.ng, fac, c.p.dt, c.p.cv);
+    TRT_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1388,12 +1505,16 @@ int step(Ctx &c, int its_out[4], int ho_form)
             mg_its++;
             mloqd_fv(c, nullptr);            // AL:686-689
             collapse_mg_ef(c);               // AL:692-693
-            cbar_calc(c);                    // AL:709
-            cp(c, "KapE_bar_MGQDold", "KapE_Bar");   // AL:712
-            grey_coefs(c);                   // AL:713-715
-            cp(c, "E_avg", "MGQD_E_avg"); cp(c, "E_edgV", "MGQD_E_edgV"); cp(c, "E_edgH", "MGQD_E_edgH");
-            cp(c, "Fx_edgV", "MGQD_Fx_edgV"); cp(c, "Fy_edgH", "MGQD_Fy_edgH");   // AL:717-721
-            newt_total += egp_fv_newt(c, mg_its);    // AL:724-728
+            if (c.p.no_grey) {
+                meb_solve(c, "Eg_avg");      // AL:767-770
+            } else {
+                cbar_calc(c);                // AL:709
+                cp(c, "KapE_bar_MGQDold", "KapE_Bar");   // AL:712
+                grey_coefs(c);               // AL:713-715
+                cp(c, "E_avg", "MGQD_E_avg"); cp(c, "E_edgV", "MGQD_E_edgV"); cp(c, "E_edgH", "MGQD_E_edgH");
+                cp(c, "Fx_edgV", "MGQD_Fx_edgV"); cp(c, "Fy_edgH", "MGQD_Fy_edgH");   // AL:717-721
+                newt_total += egp_fv_newt(c, mg_its);    // AL:724-728
+            }
             material_update(c);              // AL:774
             int tc = 0, ec = 0;
             double tn, tr, en, er;

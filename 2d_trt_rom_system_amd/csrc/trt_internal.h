@@ -176,6 +176,8 @@ void collapse_mg_ef(Ctx &c);
 void cbar_calc(Ctx &c);
 void grey_coefs(Ctx &c);
 int egp_fv_newt(Ctx &c, int mgqd_it);
+// MEB_SOLVE (use_grey = 0) from the group energies in the named array
+void meb_solve(Ctx &c, const char *eg);
 // CONVERGENCE's one-pass reduction: partial blocks for n values, and the launch that
 // writes 4 x blocks partials (max |x - xo|, max |xo - xo2|, max |x| or max |1 - xo/x|,
 // min |x|) to part on stream s

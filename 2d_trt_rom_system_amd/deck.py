@@ -20,7 +20,8 @@ Deliberate differences, each listed in Deck.warnings when it applies:
     multiplied by 1e8; the reference multiplies ns by 10 and divides sec by 1e8
     (:987-990, :1007-1010), which makes '2e-3 ns' 2e-2 sh, 100 times too long.
   * E_low_bnd / T_low_bnd: the reference stores T_low_bnd's value in E_Bound_Low
-    (:571-579); it is reproduced, since the GPU path rejects the safety search either way.
+    (:571-579); it is reproduced (T_Bound_Low then keeps its default, and the safety
+    search of EGP_FV_NEWT checks the energy densities against that value).
 """
 from __future__ import annotations
 
@@ -189,6 +190,7 @@ class Deck:
                       chi=self.chi, line_src=self.line_src, use_line_search=int(self.use_line_search),
                       use_safety_search=int(self.use_safety_search), kapE_dT_flag=int(self.kapE_dT_flag),
                       E_bound_low=self.E_bound_low, T_bound_low=self.T_bound_low, threads=self.threads,
+                      no_grey=0 if self.use_grey == 1 else 1,
                       name=f"deck:{self.test}", quadrature=self.quadrature)
         return p
 

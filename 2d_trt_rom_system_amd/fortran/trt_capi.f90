@@ -23,6 +23,8 @@ MODULE TRT_CAPI
     INTEGER(C_INT) :: lin_maxit
     INTEGER(C_INT) :: sweep_mode
     INTEGER(C_INT) :: qd_solver      ! 0 nested dissection, 1 BiCGSTAB, 2 block LU
+    INTEGER(C_INT) :: grey_emission  ! 0 analytic (reference), 1 sum over groups
+    INTEGER(C_INT) :: no_grey        ! 1: use_grey = 0 (MEB_SOLVE)
   END TYPE trt_problem
 
   ! the context of the running problem, created by TRT_GPU_INIT
@@ -129,6 +131,11 @@ MODULE TRT_CAPI
       REAL(C_DOUBLE), VALUE :: eps
       REAL(C_DOUBLE) :: norm, rho
       INTEGER(C_INT) :: conv
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_meb_solve(ctx, eg) BIND(C, name="trt_meb_solve")
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: eg(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION trt_step(ctx, its) BIND(C, name="trt_step")
       IMPORT :: C_INT, C_PTR
@@ -269,7 +276,7 @@ SUBROUTINE TRT_GPU_INIT(Omega_x, Omega_y, quad_weight, Nu_g, Delx, Dely, Mat, Ka
   p%kapE_dT_flag = MERGE(1, 0, kapE_dT_flag)
   p%E_bound_low = E_Bound_Low; p%T_bound_low = T_Bound_Low
   p%threads = 1; p%precision = precision; p%tile_x = 0; p%tile_y = 0; p%lin_rtol = 0d0; p%lin_maxit = 0; p%sweep_mode = 0
-  p%qd_solver = 0
+  p%qd_solver = 0; p%grey_emission = 0; p%no_grey = 0
   CALL TRT_CHECK(trt_ctx_create(TRT_CTX, p), 'trt_ctx_create')
   DEALLOCATE(mat0, ox, oy, w, nu, dx, dy, km)
 

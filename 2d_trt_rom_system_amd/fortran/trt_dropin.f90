@@ -562,6 +562,20 @@ SUBROUTINE MATERIAL_UPDATE(RT_Src,MGQD_Src,KapE,KapB,KapR,Bg,Temp,Comp_Unit,Nu_g
   DEALLOCATE(src)
 END SUBROUTINE MATERIAL_UPDATE
 
+! MEB_SOLVE, MainSrc/UPDATES.f08:189-231 (Bg, Delt, cV, Kap0, Comp_Unit are the context's; Eg_HO is
+! pushed as the context's Eg_avg, the array the MLOQD loop passes, ALGORITHMS.f08:769)
+SUBROUTINE MEB_SOLVE(Temp,Eg_HO,Bg,KapE,Temp_old,Delt,cV,Kap0,Comp_Unit)
+  REAL*8,INTENT(INOUT):: Temp(:,:)
+  REAL*8,INTENT(IN):: Eg_HO(:,:,:), Bg(:,:,:), KapE(:,:,:)
+  REAL*8,INTENT(IN):: Temp_old(:,:), Delt, cV, Kap0, Comp_Unit
+
+  CALL TRT_PUSH_ARRAY('Eg_avg', Eg_HO, SIZE(Eg_HO))
+  CALL TRT_PUSH_ARRAY('KapE', KapE, SIZE(KapE))
+  CALL TRT_PUSH_ARRAY('Temp_Old', Temp_old, SIZE(Temp_old))
+  CALL TRT_CHECK(trt_meb_solve(TRT_CTX, 'Eg_avg'//C_NULL_CHAR), 'MEB_SOLVE')
+  CALL TRT_PULL_ARRAY('Temp', Temp, SIZE(Temp))
+END SUBROUTINE MEB_SOLVE
+
 END MODULE UPDATES
 
 !==================================================================================================================================!

@@ -35,6 +35,7 @@ class TrtProblem(C.Structure):
         ("threads", C.c_int),
         ("precision", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
         ("lin_rtol", C.c_double), ("lin_maxit", C.c_int), ("sweep_mode", C.c_int), ("qd_solver", C.c_int),
+        ("grey_emission", C.c_int), ("no_grey", C.c_int),
     ]
 
 
@@ -53,7 +54,7 @@ EXPORTS = [
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
     "trt_frecords_last_error", "trt_step_given_qdf", "trt_pod_create", "trt_pod_reconstruct", "trt_pod_destroy",
     "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
-    "trt_convergence_noalloc",
+    "trt_convergence_noalloc", "trt_meb_solve",
 ]
 
 _lib = None
@@ -95,6 +96,7 @@ def load(path: str = LIBPATH):
     L.trt_array_size.restype = C.c_size_t
     L.trt_push.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
     L.trt_pull.argtypes = [vp, C.c_char_p, _dp, C.c_size_t]
+    L.trt_meb_solve.argtypes = [vp, C.c_char_p]
     L.trt_convergence_noalloc.argtypes = [vp, vp, vp, C.c_size_t, C.c_int, C.c_double, C.c_int, vp, vp]
     for f in ("trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
               "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_collapse_mg_ef", "trt_cbar_calc",
@@ -201,6 +203,8 @@ def make_problem_struct(prob, precision: int = 0, tile=(0, 0), lin_rtol: float =
     s.lin_rtol, s.lin_maxit = lin_rtol, lin_maxit
     s.sweep_mode = sweep_mode
     s.qd_solver = qd_solver
+    s.grey_emission = int(getattr(prob, "grey_emission", 0))
+    s.no_grey = int(getattr(prob, "no_grey", 0))
     return s, keep
 
 

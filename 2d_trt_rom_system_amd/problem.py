@@ -224,6 +224,10 @@ class Problem:
     E_bound_low: float = -1.7976931348623157e308
     T_bound_low: float = -1.7976931348623157e308
     threads: int = 1
+    # options beyond the reference's defaults
+    grey_emission: int = 0           # 0 = analytic grey emission (GLOQD_SOLVES.f08:439-440);
+                                     # 1 = sum_g 4 pi kappa_g B_g, consistent with the group sources
+    no_grey: int = 0                 # 1 = use_grey 0: MEB_SOLVE instead of the grey Newton (AL:767-770)
     name: str = "custom"
     quadrature: Optional[str] = None  # a named set of the input deck (abu36, abu20, 1d_test) instead of LQn
 

@@ -63,6 +63,13 @@ typedef struct {
                                        1 = batched BiCGSTAB with the direct solve as fallback;
                                        2 = block LU over grid rows (the slow direct solve that
                                        backs up the other two) */
+    int grey_emission;              /* grey emission of EGP_FV_NEWT: 0 = the reference's analytic
+                                       Q_bar = 15 kap0 c aR T / pi^4 (GLOQD_SOLVES.f08:439-440,766);
+                                       1 = sum_g 4 pi kappa_g(T) B_g(T) with the cell's opacity
+                                       multiplier, consistent with the group sources (a labelled
+                                       deviation from the reference)                   */
+    int no_grey;                    /* 0 = use_grey 1 (the grey Newton solve); 1 = use_grey 0: MEB_SOLVE
+                                       in the MLOQD loop (MainSrc/ALGORITHMS.f08:767-770) */
 } trt_problem;
 
 /* Context life cycle.  trt_ctx_create allocates the device-resident state and
@@ -115,7 +122,13 @@ int trt_collapse_mg_ef(trt_ctx *ctx);
 int trt_cbar_calc(trt_ctx *ctx);
 int trt_grey_coefs(trt_ctx *ctx);
 int trt_gqd_in_calc(trt_ctx *ctx);
-/* EGP_FV_NEWT, MainSrc/GLOQD_SOLVES.f08:305-860; its receives the Newton count. */
+/* MEB_SOLVE, MainSrc/UPDATES.f08:189-231: Temp from the group energies named eg ("Eg_avg" in
+ * the MLOQD loop, "HO_Eg_avg" for tr_no_qd, ALGORITHMS.f08:666,769), KapE and Temp_Old. */
+int trt_meb_solve(trt_ctx *ctx, const char *eg);
+/* EGP_FV_NEWT, MainSrc/GLOQD_SOLVES.f08:305-860; its receives the Newton count.  With
+ * use_safety_search the positivity search of GL:643-752 runs once per Newton iteration
+ * (the reference's Safe_Search loop has no exit); a bound that no step length reaches
+ * (a value already below it) is an error. */
 int trt_egp_fv_newt(trt_ctx *ctx, int mgqd_it, int *its);
 /* CONVERGENCE, MainSrc/CONVERGENCE_CHECKS.f08:7-44, on device arrays x, x_old, x_old2
  * named as in trt_pull. */

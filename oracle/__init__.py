@@ -48,6 +48,7 @@ class _Problem(C.Structure):
         ("use_line_search", C.c_int), ("use_safety_search", C.c_int), ("kapE_dT_flag", C.c_int),
         ("E_bound_low", C.c_double), ("T_bound_low", C.c_double),
         ("threads", C.c_int),
+        ("grey_emission", C.c_int), ("no_grey", C.c_int),
     ]
 
 
@@ -80,7 +81,7 @@ def lib(path: str | None = None):
         L.orc_iarray.restype = C.POINTER(C.c_int)
         L.orc_iarray.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int * 4)]
         for f in ("orc_material_update", "orc_transport_scb", "orc_collapse_intensities", "orc_fg_calc",
-                  "orc_cg_calc", "orc_mgqd_in_calc", "orc_gqd_in_calc", "orc_old_mgqd_coefs",
+                  "orc_cg_calc", "orc_mgqd_in_calc", "orc_gqd_in_calc", "orc_old_mgqd_coefs", "orc_meb_solve",
                   "orc_old_grey_coefs", "orc_mloqd_fv", "orc_collapse_mg_ef", "orc_cbar_calc", "orc_grey_coefs"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = None
@@ -154,6 +155,7 @@ class Oracle:
                                                                   prob.kapE_dT_flag)
         s.E_bound_low, s.T_bound_low = prob.E_bound_low, prob.T_bound_low
         s.threads = threads if threads is not None else prob.threads
+        s.grey_emission, s.no_grey = int(getattr(prob, "grey_emission", 0)), int(getattr(prob, "no_grey", 0))
         self._p = s
         self.h = L.orc_create(C.byref(s))
         self.L = L

@@ -235,6 +235,48 @@ void orc_material_update(orc_state *s)
     }
 }
 
+/* grey_emission 1 (a deviation from the reference, off by default): the grey
+ * emission consistent with the group sources of MATERIAL_UPDATE (UPDATES.f08:130-152),
+ * Q_bar(T) = sum_g 4 pi kappa_g(T) mult B_g(T) in cell k, and its T-derivative by a
+ * central difference of step 1e-4 T (the derivative steers the Newton iteration only;
+ * its fixed point does not depend on it). */
+static double qbar_groups(const orc_state *s, size_t k, double T)
+{
+    const double cu = s->p.comp_unit, mult = s->kmult[s->mat[k]];
+    double q = 0.0;
+    for (int g = 0; g < s->ng; ++g) {
+        const double nl = s->nu[g], nh = s->nu[g + 1];
+        const double kap = s->p.opacity_model == 0 ? orc_kapB(T, nl, nh) : orc_kap_midpoint(T, nl, nh);
+        q += 4.0 * PI * (kap * mult) * orc_Bg_planck(T, nl, nh, cu);
+    }
+    return q;
+}
+
+static double qbar_groups_dT(const orc_state *s, size_t k, double T)
+{
+    const double h = 1e-4 * T;
+    return (qbar_groups(s, k, T + h) - qbar_groups(s, k, T - h)) / (2.0 * h);
+}
+
+/* MEB_SOLVE, UPDATES.f08:189-231 (use_grey = 0): T from the group energies, the
+ * E_g-weighted absorption opacity and the analytic emission, in closed form. */
+void orc_meb_solve(orc_state *s)
+{
+    const int nx = s->nx, ny = s->ny, ng = s->ng;
+    const double h = H_PLANCK, c = 2.99792458e2, dt = s->p.dt, cv = s->p.cv, kap0 = s->p.kap0, cu = s->p.comp_unit;
+    const size_t n = (size_t)nx * ny;
+    for (size_t k = 0; k < n; ++k) {
+        double E = 0.0, kb = 0.0;
+        for (int g = 0; g < ng; ++g) {
+            E += s->Eg_avg[k + g * n];
+            kb += s->KapE[k + g * n] * s->Eg_avg[k + g * n];
+        }
+        kb /= E;
+        s->Temp[k] = (c * c * h * h * h) / (c * c * h * h * h * cv * cu + 8.0 * PI * dt * kap0 / (ERG * ERG * ERG * ERG))
+                     * (dt * c * kb * E + cv * s->Temp_Old[k]) * cu;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* GAUSS_COMPLETE restated: 4x4 elimination with partial pivoting           */
 /* ------------------------------------------------------------------------ */
@@ -1173,10 +1215,15 @@ int orc_egp_fv_newt(orc_state *s, int mgqd_it)
         its++;
         memcpy(dT_o, dT, n * 8); memcpy(dEa_o, dEa, n * 8); memcpy(dEv_o, dEv, nv * 8);
         memcpy(dEh_o, dEh, nh * 8); memcpy(dFx_o, dFx, nv * 8); memcpy(dFy_o, dFy, nh * 8);
-        /* GL:437-442: analytic grey emission */
+        /* GL:437-442: analytic grey emission (or, as an option, the group sum) */
         for (size_t k = 0; k < n; ++k) {
-            Qb[k] = 15.0 * kap0 * c * aR * T[k] / pi4;
-            QbdT[k] = 15.0 * kap0 * c * aR / pi4;
+            if (s->p.grey_emission == 1) {
+                Qb[k] = qbar_groups(s, k, T[k]);
+                QbdT[k] = qbar_groups_dT(s, k, T[k]);
+            } else {
+                Qb[k] = 15.0 * kap0 * c * aR * T[k] / pi4;
+                QbdT[k] = 15.0 * kap0 * c * aR / pi4;
+            }
         }
         /* GL:444-454 */
         if (mgqd_it > 1 && s->p.kapE_dT_flag) {
@@ -1298,23 +1345,27 @@ int orc_egp_fv_newt(orc_state *s, int mgqd_it)
             line_search(norm2(dFx, nv), norm2(dFx_o, nv), line_src, LS_Maxit, &D);
             line_search(norm2(dFy, nh), norm2(dFy_o, nh), line_src, LS_Maxit, &D);
         }
-        /* GL:643-752 */
+        /* GL:643-752, one pass: the reference's Safe_Search loop has no exit.  A value
+         * already below its bound is never reached (the reference's inner loop does not
+         * end either); the search stops after SAFE_CAP divisions there. */
         if (s->p.use_safety_search) {
+            enum { SAFE_CAP = 4096 };
             double LS = 1.0;
+            int cnt;
             for (int j = 0; j < ny; ++j)
                 for (int i = 0; i < nx; ++i) {
                     const size_t k = IX2(i, j, nx);
-                    while (T[k] + dT[k] / LS < s->p.T_bound_low) LS *= line_src;
+                    for (cnt = 0; T[k] + dT[k] / LS < s->p.T_bound_low && cnt < SAFE_CAP; ++cnt) LS *= line_src;
                     if (LS > 1.0) { ls_reset(LS, &D); LS = 1.0; }
-                    while (Ea[k] + dEa[k] / LS < s->p.E_bound_low) LS *= line_src;
+                    for (cnt = 0; Ea[k] + dEa[k] / LS < s->p.E_bound_low && cnt < SAFE_CAP; ++cnt) LS *= line_src;
                     if (LS > 1.0) { ls_reset(LS, &D); LS = 1.0; }
                 }
             for (size_t k = 0; k < nv; ++k) {
-                while (Ev[k] + dEv[k] / LS < s->p.E_bound_low) LS *= line_src;
+                for (cnt = 0; Ev[k] + dEv[k] / LS < s->p.E_bound_low && cnt < SAFE_CAP; ++cnt) LS *= line_src;
                 if (LS > 1.0) { ls_reset(LS, &D); LS = 1.0; }
             }
             for (size_t k = 0; k < nh; ++k) {
-                while (Eh[k] + dEh[k] / LS < s->p.E_bound_low) LS *= line_src;
+                for (cnt = 0; Eh[k] + dEh[k] / LS < s->p.E_bound_low && cnt < SAFE_CAP; ++cnt) LS *= line_src;
                 if (LS > 1.0) { ls_reset(LS, &D); LS = 1.0; }
             }
         }
@@ -1325,7 +1376,8 @@ int orc_egp_fv_newt(orc_state *s, int mgqd_it)
         }
         for (size_t k = 0; k < nh; ++k) { Eh[k] += dEh[k]; Fy[k] += dFy[k]; }
         for (size_t k = 0; k < nv; ++k) { Ev[k] += dEv[k]; Fx[k] += dFx[k]; }
-        for (size_t k = 0; k < n; ++k) Qb[k] = 15.0 * kap0 * c * aR * T[k] / pi4;
+        for (size_t k = 0; k < n; ++k)
+            Qb[k] = s->p.grey_emission == 1 ? qbar_groups(s, k, T[k]) : 15.0 * kap0 * c * aR * T[k] / pi4;
         /* GL:773-793 */
         const double e1 = s->p.conv_gr1, e2 = s->p.conv_gr2;
         converged = 1;
@@ -1632,15 +1684,19 @@ int orc_step_ex(orc_state *s, int it_out[4], int ho_form)
             mg_its++;
             orc_mloqd_fv(s);                     /* AL:686-689 */
             orc_collapse_mg_ef(s);               /* AL:692-693 */
-            orc_cbar_calc(s);                    /* AL:709 */
-            memcpy(s->KapE_bar_MGQDold, s->KapE_Bar, n * 8);   /* AL:712 */
-            orc_grey_coefs(s);                   /* AL:713-715 */
-            memcpy(s->E_avg, s->MGQD_E_avg, n * 8);            /* AL:717-721 */
-            memcpy(s->E_edgV, s->MGQD_E_edgV, (size_t)(nx + 1) * ny * 8);
-            memcpy(s->E_edgH, s->MGQD_E_edgH, (size_t)nx * (ny + 1) * 8);
-            memcpy(s->Fx_edgV, s->MGQD_Fx_edgV, (size_t)(nx + 1) * ny * 8);
-            memcpy(s->Fy_edgH, s->MGQD_Fy_edgH, (size_t)nx * (ny + 1) * 8);
-            newt_total += orc_egp_fv_newt(s, mg_its);          /* AL:724-728 */
+            if (s->p.no_grey) {
+                orc_meb_solve(s);                /* AL:767-770 */
+            } else {
+                orc_cbar_calc(s);                /* AL:709 */
+                memcpy(s->KapE_bar_MGQDold, s->KapE_Bar, n * 8);   /* AL:712 */
+                orc_grey_coefs(s);               /* AL:713-715 */
+                memcpy(s->E_avg, s->MGQD_E_avg, n * 8);            /* AL:717-721 */
+                memcpy(s->E_edgV, s->MGQD_E_edgV, (size_t)(nx + 1) * ny * 8);
+                memcpy(s->E_edgH, s->MGQD_E_edgH, (size_t)nx * (ny + 1) * 8);
+                memcpy(s->Fx_edgV, s->MGQD_Fx_edgV, (size_t)(nx + 1) * ny * 8);
+                memcpy(s->Fy_edgH, s->MGQD_Fy_edgH, (size_t)nx * (ny + 1) * 8);
+                newt_total += orc_egp_fv_newt(s, mg_its);      /* AL:724-728 */
+            }
             orc_material_update(s);              /* AL:774 */
             double tn, tr, en, er;
             const int tc = orc_convergence(s->p.conv_type, s->p.conv_lo, s->Temp, s->Temp_MGQDold, s->Temp_MGQDold2,

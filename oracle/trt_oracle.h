@@ -34,6 +34,12 @@ typedef struct {
     int use_line_search, use_safety_search, kapE_dT_flag;
     double E_bound_low, T_bound_low;
     int threads;
+    /* options beyond the reference's defaults */
+    int grey_emission;              /* 0 = the reference's analytic grey emission 15 kap0 c aR T / pi^4
+                                       (GLOQD_SOLVES.f08:439-440); 1 = sum_g 4 pi kappa_g B_g, consistent
+                                       with the group sources (a labelled deviation)                 */
+    int no_grey;                    /* 0 = grey Newton (use_grey = 1); 1 = use_grey = 0: MEB_SOLVE in the
+                                       MLOQD loop (ALGORITHMS.f08:767-770)                            */
 } orc_problem;
 
 typedef struct orc_state orc_state;
@@ -55,6 +61,8 @@ int *orc_iarray(orc_state *s, const char *name, int dims[4]);
 
 /* one reference routine per call, operating on the state's arrays */
 void orc_material_update(orc_state *s);
+/* MEB_SOLVE, UPDATES.f08:189-231: closed-form grey temperature from Eg_avg */
+void orc_meb_solve(orc_state *s);
 void orc_transport_scb(orc_state *s);
 void orc_collapse_intensities(orc_state *s);
 void orc_fg_calc(orc_state *s);
