@@ -1,0 +1,204 @@
+"""The benchmark workloads themselves in GPU tests (through the C ABI):
+
+* the 256x256-cell, S16, 32-group problem with its 8x8 lognormal(0, 1.5) opacity map
+  (five group blocks of 7,7,7,7,4 groups, the full persistent grid, 191-diagonal
+  dependency chains): MATERIAL_UPDATE + TRANSPORT_SCB + COLLAPSE_INTENSITIES + fg_Calc +
+  Cg_Calc against the oracle, FP64 (1e-12) and the FP32 sweep (2e-6);
+* the low-order solves at that size, where the oracle's direct solve cannot go: one
+  MLOQD_FV against the un-reduced balances and boundary conditions of
+  MLOQD_SOLVES.f08:280-339, one EGP_FV_NEWT against the grey residuals of
+  GLOQD_SOLVES.f08:920-994;
+* draws of the batched distribution (64..512 grids, S8..S16, 16..64 groups): batched
+  equal to alone bitwise, a Tgen_QDf step against the oracle on the smallest draws, and a
+  served request equal alone and in a mix of real draws;
+* the FP32 sweep's whole-step accuracy: 10 steps of the 20x20 problem, E and T within
+  1e-5 relative L2 of the oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from residuals import grey_residuals, mloqd_residuals
+
+pytestmark = pytest.mark.gpu
+
+
+def _getter(ctx, prob):
+    nx, ny, ng = prob.nx, prob.ny, prob.ng
+
+    def get(name):
+        n = ctx.size(name)
+        if "edgV" in name:
+            base = (nx + 1, ny)
+        elif "edgH" in name:
+            base = (nx, ny + 1)
+        elif name == "Delx":
+            base = (nx,)
+        elif name == "Dely":
+            base = (ny,)
+        elif name.startswith(("Cg_", "Eg_in_", "Fg_in_", "Cb_", "E_in_", "F_in_")):
+            base = (ny,) if name[-1] in "LR" else (nx,)
+        else:
+            base = (nx, ny)
+        per = int(np.prod(base))
+        return ctx.pull(name, base + (ng,) if n == per * ng else base)
+    return get
+
+
+def _cfg1_temperature(prob, seed=5):
+    """A temperature field spanning 1 eV .. 1 keV (every group carries energy somewhere)."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(0.0, np.log(1000.0), (prob.nx, prob.ny)))
+
+
+@pytest.mark.parametrize("precision,tol,ftol", [(0, 1e-12, 1e-11), (1, 2e-6, 1e-5)])
+def test_cfg1_sweep_matches_oracle(trt, oracle_mod, precision, tol, ftol):
+    """tol bounds intensities and moments; ftol the ratios f = H/(c E) and C_g, which
+    amplify the moments' rounding where a group's energy is a cancelling sum of signed
+    corner intensities (max |f| ~ 1, so ftol is also their absolute error)."""
+    prob = trt.problem.cfg1()
+    assert (prob.nx, prob.nm, prob.ng) == (256, 144, 32)
+    T = _cfg1_temperature(prob)
+    o = oracle_mod.Oracle(prob, threads=16)
+    ctx = trt.native.Context(prob, precision=precision)
+    o["Temp"][...] = T
+    ctx.push("Temp", T)
+    for x in (o, ctx):
+        x.call("material_update")
+        x.call("transport_scb")
+    Ig = ctx.pull("I_crn", o["I_crn"].shape)
+    assert rel_err(Ig, o["I_crn"]) < tol
+    del Ig
+    for x in (o, ctx):
+        x.call("collapse_intensities")
+        x.call("fg_calc")
+        x.call("cg_calc")
+    for name in ("HO_Eg_avg", "HO_Eg_edgV", "HO_Eg_edgH", "HO_Fxg_edgV", "HO_Fyg_edgH", "HO_E_avg"):
+        assert rel_err(ctx.pull(name), o[name].ravel(order="F")) < tol, name
+    for name in ("fg_avg_xx", "fg_avg_yy", "fg_avg_xy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy",
+                 "Cg_L", "Cg_B", "Cg_R", "Cg_T"):
+        assert np.max(np.abs(ctx.pull(name) - o[name].ravel(order="F"))) < ftol, name
+    ctx.close()
+
+
+def test_cfg1_low_order_residuals_full_size(trt):
+    prob = trt.problem.cfg1()
+    prob.use_line_search = 0
+    prob.conv_gr1 = 1e-13
+    ctx = trt.native.Context(prob)
+    get = _getter(ctx, prob)
+    ctx.push("Temp", _cfg1_temperature(prob))
+    for r in ("material_update", "transport_scb", "collapse_intensities", "fg_calc", "cg_calc", "mgqd_in_calc",
+              "old_mgqd_coefs", "old_grey_coefs"):
+        ctx.call(r)
+    kits = ctx.mloqd_fv()
+    assert len(kits) == prob.ng
+    worst = {}
+    for g in range(prob.ng):
+        for k, v in mloqd_residuals(get, prob, g).items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    assert max(worst.values()) < 1e-10, worst
+    for r in ("collapse_mg_ef", "cbar_calc", "grey_coefs"):
+        ctx.call(r)
+    for a, b in (("E_avg", "MGQD_E_avg"), ("E_edgV", "MGQD_E_edgV"), ("E_edgH", "MGQD_E_edgH"),
+                 ("Fx_edgV", "MGQD_Fx_edgV"), ("Fy_edgH", "MGQD_Fy_edgH")):
+        ctx.push(a, get(b))
+    its = ctx.egp_fv_newt(1)
+    assert 1 <= its < prob.maxit_gloqd
+    res = grey_residuals(get, prob)
+    assert max(res.values()) < 1e-9, res
+    ctx.close()
+
+
+def _draws(trt, n, pick):
+    return [trt.problem.cfg3_problem(20240611, i) for i in pick]
+
+
+def test_batched_draws_equal_alone(trt):
+    """Draws 0, 18, 26, 43 and 5 (64x64 and 128x128, S12/S16, 16..64 groups, their own
+    lognormal(0, 1.5) maps and boundary temperatures), FP32 sweep as in the batched runs:
+    one persistent launch over all of them equals each swept alone, bit for bit."""
+    probs = _draws(trt, 5, (0, 18, 26, 43, 5))
+    alone = []
+    for p in probs:
+        c = trt.native.Context(p, precision=1)
+        c.call("material_update")
+        c.call("transport_scb")
+        alone.append(c.pull("I_crn"))
+        c.close()
+    ctxs = [trt.native.Context(p, precision=1) for p in probs]
+    for c in ctxs:
+        c.call("material_update")
+    with trt.native.Batch(ctxs, graph=True) as b:
+        b.sweep_timed(1)
+    for c, a, p in zip(ctxs, alone, probs):
+        assert np.array_equal(c.pull("I_crn"), a), p.name
+        c.close()
+
+
+@pytest.mark.parametrize("idx", [26, 0])
+def test_tgen_step_on_draws_matches_oracle(trt, oracle_mod, idx):
+    """Tgen_QDf steps (ALGORITHMS.f08:1031-1095) of a 64x64 draw, FP64 and FP32 sweep."""
+    p = trt.problem.cfg3_problem(20240611, idx)
+    for precision, tol in ((0, 1e-12), (1, 2e-6)):
+        ctx = trt.native.Context(p, precision=precision)
+        o2 = oracle_mod.Oracle(p, threads=16)
+        for _ in range(2):
+            ctx.tgen_step()
+            o2.call("material_update")
+            o2["I_crn_old"][...] = o2["I_crn"]
+            for r in ("transport_scb", "collapse_intensities", "fg_calc", "cg_calc"):
+                o2.call(r)
+        for name in ("HO_Eg_avg", "HO_Eg_edgV", "HO_Eg_edgH", "HO_E_avg"):
+            assert rel_err(ctx.pull(name), o2[name].ravel(order="F")) < tol, (idx, precision, name)
+        # f = H / (c E) is large and ill-conditioned where a group's energy is a cancelling sum
+        # of signed corner intensities (|f| reaches 1e3 on these draws): its error is measured
+        # weighted by the energy, i.e. on c f E = H, to the moments' tolerance
+        for name, en in (("fg_avg_xx", "HO_Eg_avg"), ("fg_avg_yy", "HO_Eg_avg"), ("fg_edgV_xx", "HO_Eg_edgV"),
+                         ("fg_edgH_yy", "HO_Eg_edgH"), ("fg_edgH_xy", "HO_Eg_edgH")):
+            e = o2[en].ravel(order="F")
+            ref = o2[name].ravel(order="F")
+            err = np.max(np.abs((ctx.pull(name) - ref) * e)) / np.max(np.abs(ref * e))
+            assert err < tol, (idx, precision, name, err)
+        # C_g, a ratio of outgoing half-range sums, likewise weighted by the boundary energy
+        eT = np.asarray(o2["HO_Eg_edgH"])[:, -1, :]
+        ref = np.asarray(o2["Cg_T"])
+        got = ctx.pull("Cg_T", ref.shape)
+        assert np.max(np.abs((got - ref) * eT)) / np.max(np.abs(ref * eT)) < tol, (idx, precision)
+        ctx.close()
+
+
+def test_served_draw_same_alone_and_in_mix(trt):
+    """A 64x64 draw served alone and arriving while larger draws run."""
+    probs = _draws(trt, 4, (26, 5, 18, 12))
+    outs = ("Temp", "HO_E_avg", "HO_Eg_avg", "fg_avg_xx", "fg_edgH_xy", "Cg_L")
+    with trt.serve.Server(40 << 30) as srv:
+        a = srv.submit(probs[0], steps=4)
+        srv.run()
+        alone = {n: srv.output(a, n) for n in outs}
+    with trt.serve.Server(40 << 30) as srv:
+        others = [srv.submit(probs[1], 12, 0.0), srv.submit(probs[2], 10, 0.0)]
+        a = srv.submit(probs[0], 4, 0.005)
+        others.append(srv.submit(probs[3], 6, 0.006))
+        srv.run()
+        st = srv.request(a)
+        assert st["status"] == 0 and st["steps"] == 4
+        assert all(srv.request(i)["status"] == 0 for i in others)
+        for n in outs:
+            assert np.array_equal(srv.output(a, n), alone[n]), n
+
+
+def test_fp32_whole_steps_within_1e5(trt, oracle_mod):
+    """The precision split (FP32 sweep, FP64 moments and low-order): 10 MLQD steps of the
+    20x20 problem, E and T within 1e-5 relative L2 of the FP64 oracle at every step."""
+    prob = trt.problem.cfg0()
+    o = oracle_mod.Oracle(prob, threads=4)
+    ctx = trt.native.Context(prob, precision=1)
+    A = np.outer(prob.dx, prob.dy).ravel(order="F")
+    for s in range(10):
+        o.step()
+        ctx.step()
+        for n in ("E_avg", "Temp"):
+            ref = np.asarray(o[n]).ravel(order="F")
+            err = float(np.sqrt(np.sum(A * (ctx.pull(n) - ref) ** 2) / np.sum(A * ref ** 2)))
+            assert err < 1e-5, (s, n, err)
+    ctx.close()
