@@ -288,8 +288,10 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     // 2x2 (FP32 storage with FP64 arithmetic), the fastest measured on the 256x256, S16,
     // 32-group problem
     const bool engine = p.sweep_mode != 1;
+    // default tiles: FP32 4x4, FP64 2x4, FP32 storage with FP64 arithmetic 2x2 (the per-row
+    // form of sweep_mode 1: 4x4); trt_autotune_tile times the others
     sg.TX = p.tile_x > 0 ? p.tile_x : (!engine ? 4 : p.precision == 1 ? 4 : 2);
-    sg.TY = p.tile_y > 0 ? p.tile_y : (!engine ? 4 : p.precision == 0 ? 4 : 2);
+    sg.TY = p.tile_y > 0 ? p.tile_y : (!engine ? 4 : p.precision == 2 ? 2 : 4);
     sg.NTX = (p.nx + sg.TX - 1) / sg.TX;
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }
@@ -302,8 +304,9 @@ size_t sweep_bytes(const trt_problem &p, SweepGeom *out)
     const size_t nc = (size_t)p.nx * p.ny, NL = (size_t)p.ng * sg.K;
     const size_t ti = p.precision ? 4 : 8;
     size_t b = 2 * nc * 4 * NL * 4 * ti;                                   // I_crn, I_crn_old
-    b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * p.ny * NL * ti;         // XV
-    b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * p.nx * NL * ti;         // XH
+    b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * sg.NTY * sg.TY * NL * ti;   // XV (whole tiles)
+    b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * sg.NTX * sg.TX * NL * ti;   // XH
+    b += (size_t)p.nm * 6 * 8;                                             // moment weight table
     b += nc * p.ng * 2 * 8;                                                // sc
     if (p.precision) b += nc * p.ng * 2 * 4;                               // FP32 copy of sc
     b += (nc * 4 + (size_t)(p.nx + 1) * p.ny * 4 + (size_t)p.nx * (p.ny + 1) * 5) * 4 * p.ng * 8;
@@ -326,8 +329,10 @@ void sweep_setup(Ctx &c)
     c.icrn_bytes = nc * 4 * NL * 4 * ti;
     c.Icrn[0] = c.dalloc(c.icrn_bytes);
     c.Icrn[1] = c.dalloc(c.icrn_bytes);
-    c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * c.ny * NL * ti));
-    c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * c.nx * NL * ti));
+    // face exchange: sized for whole tiles (the tile engine's packed layout), which also
+    // covers the per-row layout of sweep_mode 1
+    c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * sg.NTY * sg.TY * NL * ti));
+    c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * sg.NTX * sg.TX * NL * ti));
     c.sc = static_cast<double *>(c.dalloc(nc * c.ng * 2 * 8));
     if (c.precision) c.scf = static_cast<float *>(c.dalloc(nc * c.ng * 2 * 4));
     c.Pavg = static_cast<double *>(c.dalloc(nc * 4 * c.ng * 4 * 8));

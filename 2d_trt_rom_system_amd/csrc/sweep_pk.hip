@@ -22,14 +22,19 @@
 // sweep coefficients of all TX x TY cells and the incoming faces -- so a tile
 // costs one memory round trip, then solves the cells in upwind order with the
 // corner-edge intensities in registers.  The corner averages of every (cell,
-// direction) are staged in LDS; after one block barrier the block sums them
-// over the directions of each group (S adjacent lanes per (cell, group) pair,
-// ceil(K/S) directions each, against a per-quadrant table of weighted
-// direction cosines, combined with cross-lane shuffles) and writes the FP64
+// direction) are staged in LDS; after one block barrier wave w of the block
+// sums a quarter of the directions for every (cell, group) pair of the tile, a
+// lane per pair, against the weighted direction cosines, which are scalar loads
+// (the direction index is the same across the wave); waves 1..3 hand their sums
+// to wave 0 through LDS, which adds them in wave order and writes the FP64
 // quadrant partial moments.
 //
-// Faces between tiles go through exchange buffers with agent-coherent (sc1)
-// stores and loads.  The producer waits for its stores to complete
+// Faces between tiles go through exchange buffers as one record per (tile,
+// lane) -- 2 TY values across a V boundary, 2 TX across an H boundary -- moved
+// in 16-byte chunks laid out chunk-major, so every chunk instruction of a wave
+// covers 64 consecutive 16-byte records.  The accesses carry the device-scope
+// cache policy (sc1) of agent-scope atomics, so they stay coherent across the
+// XCDs' L2 caches.  The producer waits for its stores to complete
 // (s_waitcnt vmcnt(0)) before the barrier after which one lane publishes the
 // completion flag, and the consumer reads the faces only after it has seen the
 // flag; no L2 writeback or invalidate is needed for the rest of the data,
@@ -50,7 +55,8 @@ constexpr int NMOM = 13;
 
 struct PkProb {
     int nx, ny, ng, K, gblk, NTX, NTY;
-    int S, ks, npair;      // moment reduction: lanes per (cell, group) pair, directions per lane, pairs per tile
+    int kw, npair, xsw;    // moment reduction: directions per wave, (cell, group) pairs per tile, doubles of
+                           // the cross-wave sums ahead of the staging area in LDS (0: they reuse it)
     int seg, nseg;         // column segments: tiles per segment, segments per tile column
     double inv_cdt, eps;
     const double *dx, *dy, *ox, *oy, *w;
@@ -60,6 +66,7 @@ struct PkProb {
     void *Inew;
     void *XV, *XH;
     double *Pavg, *PV, *PH;
+    const double *ctab;    // [m][6]: w, w ox^2, w oy^2, w ox oy, w ox, w oy, over the stored intensities' factor
 };
 
 struct PkArgs {
@@ -122,11 +129,55 @@ __device__ __forceinline__ void store4(GLOBAL_AS double *o, double a, double b, 
     *reinterpret_cast<GLOBAL_AS d2v *>(o + 2) = y;
 }
 
-// face exchange: agent-coherent accesses
+// The same pointer in every lane (a value read from LDS the compiler cannot prove
+// uniform): keep it scalar, so buffer descriptors built from it stay in SGPRs.
 template <typename T>
-__device__ __forceinline__ T xload(GLOBAL_AS const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-template <typename T>
-__device__ __forceinline__ void xstore(GLOBAL_AS T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ T *uniform_ptr(const T *p)
+{
+    const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
+
+// Face exchange.  A slice holds the records of one tile boundary line:
+// [tile][16-byte chunk][lane]; a record is N values of one lane.  Buffer accesses
+// with the SC1 cache-policy bit, the device-scope coherence of agent-scope atomics.
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+constexpr int kBufDword3 = 0x00020000;   // buffer descriptor word 3 of gfx9 (raw, 32-bit elements)
+constexpr int kSc1 = 16;                 // cache-policy bit SC1
+
+template <typename TI, int N>
+__device__ __forceinline__ void xload_rec(const TI *slice, unsigned bytes, unsigned tile, unsigned NL, unsigned lane,
+                                          TI (&v)[N])
+{
+    static_assert((N * sizeof(TI)) % 16 == 0, "face records are whole 16-byte chunks");
+    constexpr int NC = (int)(N * sizeof(TI) / 16);
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        uniform_ptr(slice), 0, (int)__builtin_amdgcn_readfirstlane(bytes), kBufDword3);
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+        const unsigned off = ((tile * NC + ch) * NL + lane) * 16u;
+        const u4v w = __builtin_bit_cast(u4v, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, kSc1));
+        __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * ch, &w, 16);
+    }
+}
+
+template <typename TI, int N>
+__device__ __forceinline__ void xstore_rec(TI *slice, unsigned bytes, unsigned tile, unsigned NL, unsigned lane,
+                                           const TI (&v)[N])
+{
+    constexpr int NC = (int)(N * sizeof(TI) / 16);
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        uniform_ptr(slice), 0, (int)__builtin_amdgcn_readfirstlane(bytes), kBufDword3);
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+        u4v w;
+        __builtin_memcpy(&w, reinterpret_cast<const char *>(v) + 16 * ch, 16);
+        const unsigned off = ((tile * NC + ch) * NL + lane) * 16u;
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, kSc1);
+    }
+}
 
 // 1/x: hardware reciprocal, refined by Newton-Raphson steps (r <- r + r (1 - x r)),
 // each of which squares the relative error: one step for FP32, two for FP64.
@@ -146,8 +197,15 @@ __device__ __forceinline__ double recip(double x)
     return fma(r, e, r);
 }
 
-// s_waitcnt vmcnt(0): every global load and store of this wave has completed
-__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt vmcnt(0): every global load and store of this wave has completed.  The
+// compiler barriers keep every store of the tile (the face records are buffer
+// stores the compiler does not order against the wait by itself) ahead of it.
+__device__ __forceinline__ void wait_vm_stores()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    asm volatile("" ::: "memory");
+}
 
 // Bounded wait for an upwind tile.  A wait that runs out sets the error word, and
 // every later wait of the launch sees it and returns at once, so a broken
@@ -163,18 +221,6 @@ __device__ __forceinline__ void wait_flag(const int *f, int epoch, int limit, in
             atomicOr(err, 1);
             break;
         }
-    }
-}
-
-// (weights times isc: the factor of the stored intensities leaves with the weights)
-__device__ __forceinline__ void load_ctab(const PkProb &P, double *ctab, int q, double isc)
-{
-    for (int t = threadIdx.x; t < P.K; t += PK_THREADS) {
-        const int mm = q * P.K + t;
-        const double wm = gptr(P.w)[mm], ox = gptr(P.ox)[mm], oy = gptr(P.oy)[mm];
-        double *cc = ctab + t * 6;
-        cc[0] = wm * isc; cc[1] = wm * ox * ox * isc; cc[2] = wm * oy * oy * isc; cc[3] = wm * ox * oy * isc;
-        cc[4] = wm * ox * isc; cc[5] = wm * oy * isc;
     }
 }
 
@@ -207,8 +253,8 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
                              pmask == 0 ? 3 : pmask == 1 ? 2 : pmask == 2 ? 0 : 1};
     GLOBAL_AS const TI *Iold = gptr(static_cast<const TI *>(P.Iold));
     GLOBAL_AS TI *Inew = gptr(static_cast<TI *>(P.Inew));
-    GLOBAL_AS TI *XV = gptr(static_cast<TI *>(P.XV));
-    GLOBAL_AS TI *XH = gptr(static_cast<TI *>(P.XH));
+    TI *XV = static_cast<TI *>(P.XV);
+    TI *XH = static_cast<TI *>(P.XH);
     typedef typename Vec2<TI>::type TI2;
     GLOBAL_AS const TI2 *scv = reinterpret_cast<GLOBAL_AS const TI2 *>(gptr(P.sc));
     const TC inv_cdt = P.inv_cdt;
@@ -254,29 +300,34 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     const TI bcy = (TI)(gptr(P.bcin)[(py ? 1 : 3) * ng + gg] * iscale<TI>());
 
     sync();
+    asm volatile("" ::: "memory");   // the face loads stay behind the upwind waits
 
+    // incoming faces: the records the left and lower tiles wrote (rows and columns
+    // beyond a partial tile's edge are carried but unused)
+    constexpr int NV = 2 * TY, NH = 2 * TX;
+    const unsigned vslice = (unsigned)((size_t)P.NTY * NL * NV * sizeof(TI));
+    const unsigned hslice = (unsigned)((size_t)P.NTX * NL * NH * sizeof(TI));
     TI vbr[TY], vtr[TY], hl[TX], hr[TX];
+    {
+        TI pv[NV];
 #pragma unroll
-    for (int r = 0; r < TY; ++r) {
-        vbr[r] = bcx; vtr[r] = bcx;
-        if (txl > 0 && r < jh && act) {
-            const size_t base = ((size_t)(q * (P.NTX - 1) + txl - 1) * (2 * ny) + 2 * (j0 + r)) * NL + L;
-            vbr[r] = xload(XV + base);
-            vtr[r] = xload(XV + base + NL);
-        }
-    }
+        for (int t = 0; t < NV; ++t) pv[t] = bcx;
+        if (txl > 0 && act)
+            xload_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl - 1) * P.NTY * NL * NV, vslice, (unsigned)tyl,
+                              (unsigned)NL, (unsigned)L, pv);
 #pragma unroll
-    for (int ii = 0; ii < TX; ++ii) {
-        hl[ii] = bcy; hr[ii] = bcy;
-        if (tyl > 0 && ii < iw && act) {
-            if (hin_regs) {
-                hl[ii] = hc[ii][0];
-                hr[ii] = hc[ii][1];
-            } else {
-                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl - 1) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-                hl[ii] = xload(XH + base);
-                hr[ii] = xload(XH + base + NL);
-            }
+        for (int r = 0; r < TY; ++r) { vbr[r] = pv[2 * r]; vtr[r] = pv[2 * r + 1]; }
+        TI ph[NH];
+#pragma unroll
+        for (int t = 0; t < NH; ++t) ph[t] = bcy;
+        if (tyl > 0 && act && !hin_regs)
+            xload_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl - 1) * P.NTX * NL * NH, hslice, (unsigned)txl,
+                              (unsigned)NL, (unsigned)L, ph);
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            hl[ii] = ph[2 * ii];
+            hr[ii] = ph[2 * ii + 1];
+            if (tyl > 0 && hin_regs) { hl[ii] = hc[ii][0]; hr[ii] = hc[ii][1]; }
         }
     }
 
@@ -346,21 +397,22 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             st[(1 * TX * TY + cidx) * PK_THREADS + tid] = (TI)((x1 + x2) * (TC)0.5);
             st[(2 * TX * TY + cidx) * PK_THREADS + tid] = (TI)((x3 + x2) * (TC)0.5);
         }
-        if (txl < P.NTX - 1 && act) {
-            const size_t base = ((size_t)(q * (P.NTX - 1) + txl) * (2 * ny) + 2 * (j0 + r)) * NL + L;
-            xstore(XV + base, (TI)vb);
-            xstore(XV + base + NL, (TI)vt);
-        }
+        vbr[r] = (TI)vb;                 // outgoing V faces of the row
+        vtr[r] = (TI)vt;
+    }
+    if (txl < P.NTX - 1 && act) {
+        TI pv[NV];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) { pv[2 * r] = vbr[r]; pv[2 * r + 1] = vtr[r]; }
+        xstore_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl) * P.NTY * NL * NV, vslice, (unsigned)tyl, (unsigned)NL,
+                           (unsigned)L, pv);
     }
     if (tyl < P.NTY - 1 && act && hout_xh) {
+        TI ph[NH];
 #pragma unroll
-        for (int ii = 0; ii < TX; ++ii) {
-            if (ii < iw) {
-                const size_t base = ((size_t)(q * (P.NTY - 1) + tyl) * (2 * nx) + 2 * (i0 + ii)) * NL + L;
-                xstore(XH + base, (TI)hlc[ii]);
-                xstore(XH + base + NL, (TI)hrc[ii]);
-            }
-        }
+        for (int ii = 0; ii < TX; ++ii) { ph[2 * ii] = (TI)hlc[ii]; ph[2 * ii + 1] = (TI)hrc[ii]; }
+        xstore_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl) * P.NTX * NL * NH, hslice, (unsigned)txl, (unsigned)NL,
+                           (unsigned)L, ph);
     }
 #pragma unroll
     for (int ii = 0; ii < TX; ++ii) {
@@ -369,80 +421,102 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     }
 }
 
-// v of the lane whose index differs in bit 0 (XOR 1) or bit 1 (XOR 2): a DPP quad
-// permute of both halves, done in the VALU without a round trip through the LDS
-// unit (as ds_bpermute, what __shfl_xor compiles to); same partner, same value.
-template <int CTRL>
-__device__ __forceinline__ double quad_perm(double v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-constexpr int DPP_XOR1 = 0xB1;   // quad_perm [1,0,3,2]
-constexpr int DPP_XOR2 = 0x4E;   // quad_perm [2,3,0,1]
+typedef const __attribute__((address_space(4))) double CDBL;
 
 // Sum the staged corner averages of a tile over the directions of each group and
-// write the quadrant partial moments of its cells and downwind faces.
+// write the quadrant partial moments of its cells and downwind faces.  Wave w takes
+// directions [w kw, (w+1) kw), a lane per (cell, group) pair (64 pairs per pass); the
+// direction index is uniform across the wave, so the weights are scalar loads from
+// the problem's constant table.  Waves 1..3 leave their 13 sums in xs; wave 0 adds
+// them in wave order (a fixed order: results do not depend on timing) and writes the
+// partials.  With P.xsw == 0 the sums of at most two passes are formed before xs
+// overwrites the staging area; otherwise xs has its own LDS area, one pass at a time.
+constexpr int MAX_OVERLAP_PASSES = 2;
+
 template <typename TS, int TX, int TY, bool px, bool py>
-__device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, const double *ctab, int q, int gb, int txl,
-                                            int tyl)
+__device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, double *xs, int q, int gb, int txl, int tyl)
 {
-    const int tid = threadIdx.x;
-    const int S = P.S, K = P.K, ng = P.ng, nx = P.nx, ny = P.ny, gblk = P.gblk;
-    const int s = tid & (S - 1), pofs = tid / S, pstride = PK_THREADS / S;
+    const int tid = threadIdx.x, ln = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny, gblk = P.gblk;
+    const int k0 = wv * P.kw, k1 = min(K, k0 + P.kw);
     const int i0 = txl * TX, iw = min(TX, nx - i0);
     const int j0 = tyl * TY, jh = min(TY, ny - j0);
     const int g0 = gb * gblk;
     const double eps = P.eps * iscale<TS>();   // (the weights carry 1 / iscale)
-    for (int pb = 0; pb < P.npair; pb += pstride) {
-        const int p = pb + pofs;
-        const int cidx = p / gblk, gl = p - cidx * gblk;
-        const int r = cidx / TX, ii = cidx - r * TX;
-        const int g = g0 + gl;
-        const bool ok = p < P.npair && ii < iw && r < jh && g < ng;
-        double m[NMOM];
+    CDBL *ct = (CDBL *)(uniform_ptr(P.ctab)) + (size_t)q * K * 6;
+    // pair p of the tile: (cell cidx = (r, ii), group g); ok if it lies inside the grid
+    auto pair_of = [&](int p, int &cidx, int &gl, int &r, int &ii, int &g) {
+        cidx = p / gblk; gl = p - cidx * gblk;
+        r = cidx / TX; ii = cidx - r * TX;
+        g = g0 + gl;
+        return p < P.npair && ii < iw && r < jh && g < ng;
+    };
+    auto accumulate = [&](int p, double (&m)[NMOM]) {
 #pragma unroll
         for (int v = 0; v < NMOM; ++v) m[v] = 0.0;
-        if (ok) {
-            const int k0 = s * P.ks, k1 = min(K, k0 + P.ks);
-            const TS *sa = st + (0 * TX * TY + cidx) * PK_THREADS + gl * K;
-            const TS *sv = st + (1 * TX * TY + cidx) * PK_THREADS + gl * K;
-            const TS *sh = st + (2 * TX * TY + cidx) * PK_THREADS + gl * K;
-            for (int kk = k0; kk < k1; ++kk) {
-                const double ia = (double)sa[kk] + eps, iv = (double)sv[kk] + eps, ih = (double)sh[kk] + eps;
-                const double *cc = ctab + kk * 6;   // w, w ox^2, w oy^2, w ox oy, w ox, w oy
-                m[0] += cc[1] * ia; m[1] += cc[2] * ia; m[2] += cc[3] * ia; m[3] += cc[0] * ia;
-                m[4] += cc[1] * iv; m[5] += cc[3] * iv; m[6] += cc[4] * iv; m[7] += cc[0] * iv;
-                m[8] += cc[2] * ih; m[9] += cc[3] * ih; m[10] += cc[4] * ih; m[11] += cc[5] * ih;
-                m[12] += cc[0] * ih;
+        int cidx, gl, r, ii, g;
+        if (!pair_of(p, cidx, gl, r, ii, g)) return;
+        const TS *sa = st + (0 * TX * TY + cidx) * PK_THREADS + gl * K;
+        const TS *sv = st + (1 * TX * TY + cidx) * PK_THREADS + gl * K;
+        const TS *sh = st + (2 * TX * TY + cidx) * PK_THREADS + gl * K;
+        for (int kk = k0; kk < k1; ++kk) {
+            const double ia = (double)sa[kk] + eps, iv = (double)sv[kk] + eps, ih = (double)sh[kk] + eps;
+            CDBL *cc = ct + kk * 6;   // w, w ox^2, w oy^2, w ox oy, w ox, w oy
+            m[0] += cc[1] * ia; m[1] += cc[2] * ia; m[2] += cc[3] * ia; m[3] += cc[0] * ia;
+            m[4] += cc[1] * iv; m[5] += cc[3] * iv; m[6] += cc[4] * iv; m[7] += cc[0] * iv;
+            m[8] += cc[2] * ih; m[9] += cc[3] * ih; m[10] += cc[4] * ih; m[11] += cc[5] * ih;
+            m[12] += cc[0] * ih;
+        }
+    };
+    auto finish = [&](int p, const double *xp, double (&m)[NMOM]) {   // wave 0: add waves 1..3, store
+        int cidx, gl, r, ii, g;
+        if (!pair_of(p, cidx, gl, r, ii, g)) return;
+#pragma unroll
+        for (int w = 1; w < 4; ++w)
+#pragma unroll
+            for (int v = 0; v < NMOM; ++v) m[v] += xp[((w - 1) * NMOM + v) * 64 + ln];
+        const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+        const size_t c = (size_t)i + (size_t)nx * j;
+        const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
+        const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
+        store4(gptr(P.Pavg) + ((c * 4 + q) * ng + g) * 4, m[0], m[1], m[2], m[3]);
+        store4(gptr(P.PV) + ((fV * 4 + q) * ng + g) * 4, m[4], m[5], m[6], m[7]);
+        GLOBAL_AS double *o = gptr(P.PH) + ((fH * 4 + q) * ng + g) * 5;
+        o[0] = m[8]; o[1] = m[9]; o[2] = m[10]; o[3] = m[11]; o[4] = m[12];
+    };
+    constexpr int XSP = 3 * NMOM * 64;   // doubles of one pass in xs
+    if (P.xsw == 0) {
+        const int npass = (P.npair + 63) / 64;   // at most MAX_OVERLAP_PASSES here
+        double m[MAX_OVERLAP_PASSES][NMOM];
+#pragma unroll
+        for (int t = 0; t < MAX_OVERLAP_PASSES; ++t) accumulate(t < npass ? t * 64 + ln : P.npair, m[t]);
+        __syncthreads();                 // every wave has read the staged values xs overwrites
+        if (wv > 0) {
+#pragma unroll
+            for (int t = 0; t < MAX_OVERLAP_PASSES; ++t)
+                if (t < npass)
+#pragma unroll
+                    for (int v = 0; v < NMOM; ++v) xs[t * XSP + ((wv - 1) * NMOM + v) * 64 + ln] = m[t][v];
+        }
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int t = 0; t < MAX_OVERLAP_PASSES; ++t)
+                if (t < npass) finish(t * 64 + ln, xs + t * XSP, m[t]);
+        }
+    } else {
+        for (int pb = 0; pb < P.npair; pb += 64) {
+            double m[NMOM];
+            accumulate(pb + ln, m);
+            if (wv > 0) {
+#pragma unroll
+                for (int v = 0; v < NMOM; ++v) xs[((wv - 1) * NMOM + v) * 64 + ln] = m[v];
             }
-        }
-        // butterfly over the S lanes of a pair (every lane of the block is active here)
-        if (S > 1) {
-#pragma unroll
-            for (int v = 0; v < NMOM; ++v) m[v] += quad_perm<DPP_XOR1>(m[v]);
-        }
-        if (S > 2) {
-#pragma unroll
-            for (int v = 0; v < NMOM; ++v) m[v] += quad_perm<DPP_XOR2>(m[v]);
-        }
-        if (S > 4) {
-#pragma unroll
-            for (int v = 0; v < NMOM; ++v) m[v] += __shfl_xor(m[v], 4);
-        }
-        if (ok) {
-            const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
-            const int j = py ? j0 + r : ny - 1 - (j0 + r);
-            const size_t c = (size_t)i + (size_t)nx * j;
-            const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
-            const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
-            if (s == 0) store4(gptr(P.Pavg) + ((c * 4 + q) * ng + g) * 4, m[0], m[1], m[2], m[3]);
-            if (s == (1 & (S - 1))) store4(gptr(P.PV) + ((fV * 4 + q) * ng + g) * 4, m[4], m[5], m[6], m[7]);
-            if (s == (2 & (S - 1))) {
-                GLOBAL_AS double *o = gptr(P.PH) + ((fH * 4 + q) * ng + g) * 5;
-                o[0] = m[8]; o[1] = m[9]; o[2] = m[10]; o[3] = m[11]; o[4] = m[12];
-            }
+            __syncthreads();
+            if (wv == 0) finish(pb + ln, xs, m);
+            if (pb + 64 < P.npair) __syncthreads();   // xs is written again by the next pass
         }
     }
 }
@@ -460,14 +534,13 @@ __device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int g
 }
 
 template <typename TI, int TX, int TY>
-__device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, const double *ctab, int q, int gb, int txl,
-                                            int tyl)
+__device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, double *xs, int q, int gb, int txl, int tyl)
 {
     switch (q) {
-    case 0: reduce_tile<TI, TX, TY, true, true>(P, st, ctab, q, gb, txl, tyl); break;
-    case 1: reduce_tile<TI, TX, TY, false, true>(P, st, ctab, q, gb, txl, tyl); break;
-    case 2: reduce_tile<TI, TX, TY, false, false>(P, st, ctab, q, gb, txl, tyl); break;
-    default: reduce_tile<TI, TX, TY, true, false>(P, st, ctab, q, gb, txl, tyl); break;
+    case 0: reduce_tile<TI, TX, TY, true, true>(P, st, xs, q, gb, txl, tyl); break;
+    case 1: reduce_tile<TI, TX, TY, false, true>(P, st, xs, q, gb, txl, tyl); break;
+    case 2: reduce_tile<TI, TX, TY, false, false>(P, st, xs, q, gb, txl, tyl); break;
+    default: reduce_tile<TI, TX, TY, true, false>(P, st, xs, q, gb, txl, tyl); break;
     }
 }
 
@@ -516,15 +589,15 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
             if (tid == 0) s_pi = pi;
         }
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
-        double *ctab = dyn;
-        TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        double *xs = dyn;                                           // cross-wave sums
+        TI *st = reinterpret_cast<TI *>(dyn + P.xsw);             // staged corner averages
+        if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
         TI hc[TX][2];
         tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
             if (tid == 0) {
                 if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
                 if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
             }
-            load_ctab(P, ctab, q, 1.0 / iscale<TI>());
             __syncthreads();
         }, hc, false, true);
         wait_vm_stores();                 // this wave's face stores have completed
@@ -534,7 +607,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
             s_next = nxt;
             s_item = nit;
         }
-        tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+        tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
         __syncthreads();
         cur = __builtin_amdgcn_readfirstlane(s_next);
     }
@@ -551,15 +624,13 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, 
     const int t = bid % nt_d;
     const int q = bid / nt_d;
     const int txl = tx_lo + t, tyl = d - txl;
-    double *ctab = dyn;
-    TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+    double *xs = dyn;
+    TI *st = reinterpret_cast<TI *>(dyn + P.xsw);
+    if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
     TI hc[TX][2];
-    tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() {
-        load_ctab(P, ctab, q, 1.0 / iscale<TI>());
-        __syncthreads();
-    }, hc, false, true);
+    tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() { __syncthreads(); }, hc, false, true);
     __syncthreads();
-    tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+    tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
 }
 
 // Column-segment form: a work item is (quadrant, group block, tile column, segment
@@ -607,8 +678,9 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
             if (tid == 0) s_pi = pi;
         }
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
-        double *ctab = dyn;
-        TI *st = reinterpret_cast<TI *>(dyn + (size_t)P.K * 6);
+        double *xs = dyn;
+        TI *st = reinterpret_cast<TI *>(dyn + P.xsw);
+        if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
         const int t0 = sg * P.seg, t1 = min(t0 + P.seg, P.NTY);
         TI hc[TX][2];
         for (int tyl = t0; tyl < t1; ++tyl) {
@@ -618,13 +690,12 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
                     if (txl > 0) wait_flag(a.flags + fl - 1, epoch, a.spin_limit, err);
                     if (tyl == t0 && tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
                 }
-                if (tyl == t0) load_ctab(P, ctab, q, 1.0 / iscale<TI>());
                 __syncthreads();
             }, hc, tyl > t0, tyl == t1 - 1);
             wait_vm_stores();             // this wave's face stores have completed
             __syncthreads();
             if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            tile_reduce<TI, TX, TY>(P, st, ctab, q, gb, txl, tyl);
+            tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
             __syncthreads();
         }
         if (tid == 0) {
@@ -796,6 +867,44 @@ static int spin_limit()
     return 1 << 20;
 }
 
+// Doubles of LDS the cross-wave moment sums need ahead of the staging area: none when
+// the tile's pairs fit MAX_OVERLAP_PASSES passes and the staging area can hold their
+// sums once read; else one pass's worth.
+constexpr int XS_WORDS = 3 * NMOM * 64;
+static int xs_words(int npair, size_t staging_bytes)
+{
+    const int npass = (npair + 63) / 64;
+    return (npass <= MAX_OVERLAP_PASSES && staging_bytes >= (size_t)npass * XS_WORDS * 8) ? 0 : XS_WORDS;
+}
+
+static size_t staging_bytes(int tx, int ty, int precision)
+{
+    return (size_t)3 * tx * ty * PK_THREADS * (precision ? 4 : 8);
+}
+
+// The moment weights of every direction, w, w ox^2, w oy^2, w ox oy, w ox, w oy, over the
+// factor of the stored intensities, in device memory (read by scalar loads).
+static const double *moment_table(Ctx &c)
+{
+    std::vector<double> w(c.nm), ox(c.nm), oy(c.nm), tab((size_t)c.nm * 6);
+    TRT_CHECK(hipMemcpy(w.data(), c.w, c.nm * sizeof(double), hipMemcpyDeviceToHost));
+    TRT_CHECK(hipMemcpy(ox.data(), c.ox, c.nm * sizeof(double), hipMemcpyDeviceToHost));
+    TRT_CHECK(hipMemcpy(oy.data(), c.oy, c.nm * sizeof(double), hipMemcpyDeviceToHost));
+    const double isc = c.precision ? 1.0 / kIScale32 : 1.0;
+    for (int m = 0; m < c.nm; ++m) {
+        double *t = &tab[(size_t)m * 6];
+        t[0] = w[m] * isc;
+        t[1] = w[m] * ox[m] * ox[m] * isc;
+        t[2] = w[m] * oy[m] * oy[m] * isc;
+        t[3] = w[m] * ox[m] * oy[m] * isc;
+        t[4] = w[m] * ox[m] * isc;
+        t[5] = w[m] * oy[m] * isc;
+    }
+    double *d = static_cast<double *>(c.dalloc(tab.size() * sizeof(double)));
+    TRT_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    return d;
+}
+
 static void pk_setup(Ctx &c)
 {
     const SweepGeom &sg = c.sg;
@@ -806,10 +915,10 @@ static void pk_setup(Ctx &c)
     PkProb P{};
     P.nx = c.nx; P.ny = c.ny; P.ng = c.ng; P.K = sg.K; P.gblk = sg.gblk; P.NTX = sg.NTX; P.NTY = sg.NTY;
     P.npair = sg.TX * sg.TY * sg.gblk;
-    int S = 8;
-    while (S > 1 && (S * P.npair > PK_THREADS || S > sg.K)) S >>= 1;
-    P.S = S;
-    P.ks = (sg.K + S - 1) / S;
+    P.kw = (sg.K + 3) / 4;
+    const size_t stb = staging_bytes(sg.TX, sg.TY, c.precision);
+    P.xsw = xs_words(P.npair, stb);
+    P.ctab = moment_table(c);
     P.inv_cdt = 1.0 / (c.p.c * c.p.dt);
     P.eps = 1e-25 / c.p.comp_unit;
     P.dx = c.dx; P.dy = c.dy; P.ox = c.ox; P.oy = c.oy; P.w = c.w;
@@ -817,8 +926,7 @@ static void pk_setup(Ctx &c)
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
     P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
-    const size_t ti = c.precision ? 4 : 8;
-    pk->lds = (size_t)sg.K * 6 * 8 + (size_t)3 * sg.TX * sg.TY * PK_THREADS * ti;
+    pk->lds = (size_t)P.xsw * 8 + stb;
     // The default form (sweep_mode 0) takes column segments of 4 or 2 tiles when their
     // wavefront (items of one diagonal) still offers twice the resident workgroups, as
     // batches do: on large grids segments save face traffic, on small ones the segment's
@@ -1033,10 +1141,10 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
         // the segment wavefronts of all members still offer twice the resident
         // workgroups, else single tiles.
         {
-            const size_t ti0 = b->precision ? 4 : 8;
-            int maxk0 = 0;
-            for (Ctx *c : ctxs) maxk0 = std::max(maxk0, c->sg.K);
-            const size_t lds0 = (size_t)maxk0 * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti0;
+            const size_t stb0 = staging_bytes(b->tx, b->ty, b->precision);
+            int xsw0 = 0;
+            for (Ctx *c : ctxs) xsw0 = std::max(xsw0, xs_words(b->tx * b->ty * c->sg.gblk, stb0));
+            const size_t lds0 = (size_t)xsw0 * 8 + stb0;
             const long long g2 = 2LL * resident_grid(b->k->seg, lds0, 1 << 30);
             int seg = 1;
             if (seg_req > 0) seg = seg_req;
@@ -1054,7 +1162,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
             for (int i = 0; i < n; ++i) b->seg_of[i] = std::min(seg, ctxs[i]->sg.NTY);
         }
         std::vector<int> base(n + 1, 0);
-        int maxd = 0, maxk = 0;
+        int maxd = 0, maxxs = 0;
         size_t expect = 0;
         for (int i = 0; i < n; ++i) {
             Ctx &c = *ctxs[i];
@@ -1064,7 +1172,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
             const int nseg = (sg.NTY + b->seg_of[i] - 1) / b->seg_of[i];
             maxd = std::max(maxd, num_diags(sg, nseg, b->segs) - 1);
             expect += (size_t)4 * sg.nblk_g * sg.NTX * (b->segs ? nseg : sg.NTY);
-            maxk = std::max(maxk, sg.K);
+            maxxs = std::max(maxxs, c.pk->host[0].xsw);
             b->nbt = std::max(b->nbt, 4 * c.ng * (c.nx + c.ny));
             b->updates += (long long)c.nx * c.ny * c.nm * c.ng;
         }
@@ -1086,8 +1194,7 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
         b->queue = static_cast<int *>(batch_alloc(*b, 4 * sizeof(int)));
         TRT_CHECK(hipMemset(b->queue, 0, 4 * sizeof(int)));
         b->probs = static_cast<PkProb *>(batch_alloc(*b, (size_t)n * sizeof(PkProb)));
-        const size_t ti = b->precision ? 4 : 8;
-        b->lds = (size_t)maxk * 6 * 8 + (size_t)3 * b->tx * b->ty * PK_THREADS * ti;
+        b->lds = (size_t)maxxs * 8 + staging_bytes(b->tx, b->ty, b->precision);
         b->kern = b->segs ? b->k->seg : b->k->pk;
         b->grid = resident_grid(b->kern, b->lds, b->nitems);
         if (pooled) b->stream = cache_stream_get();

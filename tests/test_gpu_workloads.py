@@ -50,11 +50,13 @@ def _cfg1_temperature(prob, seed=5):
     return np.exp(rng.uniform(0.0, np.log(1000.0), (prob.nx, prob.ny)))
 
 
-@pytest.mark.parametrize("precision,tol,ftol", [(0, 1e-12, 1e-11), (1, 2e-6, 1e-5)])
+@pytest.mark.parametrize("precision,tol,ftol", [(0, 1e-12, 1e-11), (1, 4e-6, 1e-5)])
 def test_cfg1_sweep_matches_oracle(trt, oracle_mod, precision, tol, ftol):
     """tol bounds intensities and moments; ftol the ratios f = H/(c E) and C_g, which
     amplify the moments' rounding where a group's energy is a cancelling sum of signed
-    corner intensities (max |f| ~ 1, so ftol is also their absolute error)."""
+    corner intensities (max |f| ~ 1, so ftol is also their absolute error).  In FP32 the
+    corner solves' rounding (6e-8 per operation) accumulates along upwind chains of up to
+    511 cells here: 4e-6 (about 2e-6 observed), where the 20-cell problems stay in 2e-6."""
     prob = trt.problem.cfg1()
     assert (prob.nx, prob.nm, prob.ng) == (256, 144, 32)
     T = _cfg1_temperature(prob)
