@@ -550,8 +550,16 @@ __device__ __forceinline__ void tile_reduce(const PkProb &P, const TI *st, doubl
 constexpr int PKPROB_WORDS = (int)(sizeof(PkProb) / 8);
 static_assert(sizeof(PkProb) % 8 == 0, "PkProb is copied as 8-byte words");
 
+// Waves per SIMD the persistent tile kernel is compiled for: the FP32 4x4 tile needs a
+// few registers above the budget of 3 (168 VGPRs) and is held to it without spilling;
+// the others keep the compiler's choice (the column-segment kernel would spill).
 template <typename TI, typename TC, int TX, int TY>
-__global__ __launch_bounds__(PK_THREADS) void sweep_pk(PkArgs a)
+struct PkOccupancy {
+    static constexpr int waves = (sizeof(TI) == 4 && sizeof(TC) == 4 && TX == 4 && TY == 4) ? 3 : 1;
+};
+
+template <typename TI, typename TC, int TX, int TY>
+__global__ __launch_bounds__(PK_THREADS, (PkOccupancy<TI, TC, TX, TY>::waves)) void sweep_pk(PkArgs a)
 {
     extern __shared__ double dyn[];
     __shared__ int s_next;
