@@ -1,6 +1,6 @@
 """Driver for counter profiles of the sweep (run under rocprofv3).
 
-    python3 2d_trt_rom_system_amd/prof_sweep.py --precision 1 --sweeps 2
+    python3 2d_trt_rom_system_amd/prof_sweep.py --precision 1 --tile 4 4 --sweeps 2
 
 runs, on the 256x256-cell, S16, 32-group problem: one plain device copy of
 --copy-mib MiB (calib_copy_kernel, the reference for byte counters: it reads
@@ -81,6 +81,7 @@ def summarize(fetch_csv: str, write_csv: str, copy_mib: int, kernel_prefix: str)
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--tile", type=int, nargs=2, default=(0, 0), help="tile shape (0 0: the default)")
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--steps", type=int, default=0, help="full time steps instead of sweeps (uniform material)")
     ap.add_argument("--copy-mib", type=int, default=1024)
@@ -91,4 +92,4 @@ if __name__ == "__main__":
     elif a.steps:
         run_steps(a.precision, a.steps)
     else:
-        run(a.precision, a.sweeps, a.copy_mib)
+        run(a.precision, a.sweeps, a.copy_mib, tuple(a.tile))

@@ -24,7 +24,7 @@ from . import native, problem
 class Server:
     """A trt_server: submit requests, run them, read stats and outputs."""
 
-    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, precision: int = 1, tile=(4, 2)):
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, precision: int = 1, tile=(4, 4)):
         self.L = native.load()
         self.precision = precision
         self.tile = tile
@@ -106,7 +106,7 @@ def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, n))
 
 
-def run_load(rate: float, n: int, steps: int = 20, budget_gib: float = 200.0, precision: int = 1, tile=(4, 2),
+def run_load(rate: float, n: int, steps: int = 20, budget_gib: float = 200.0, precision: int = 1, tile=(4, 4),
              base_seed: int = 20240611, arrival_seed: int = 7, max_active: int = 64, problems=None) -> dict:
     """Serve n requests (problems of the batched distribution unless given)
     arriving as a Poisson process; returns latency and throughput figures."""
@@ -146,7 +146,7 @@ def main(argv=None):
     ap.add_argument("--budget-gib", type=float, default=200.0)
     ap.add_argument("--precision", type=int, default=1)
     a = ap.parse_args(argv)
-    tile = (4, 2) if a.precision else (2, 4)
+    tile = (4, 4) if a.precision else (2, 4)
     for r in a.rates:
         print(f"[serve] rate {r}/s, {a.requests} requests", file=sys.stderr, flush=True)
         print(json.dumps(run_load(r, a.requests, a.steps, a.budget_gib, a.precision, tile)), flush=True)

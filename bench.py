@@ -36,8 +36,14 @@ def alg_bytes(prob, precision: int) -> int:
     return nc * prob.nm * prob.ng * 8 * s_i + nc * prob.ng * (2 * 8 + 12 * 8)
 
 
-def measure(trt, prob, precision: int, steps: int, warmup: int):
-    ctx = trt.native.Context(prob, precision=precision)
+def measure(trt, prob, precision: int, steps: int, warmup: int, tune: bool = True):
+    """The sweep of one problem in one precision form, with the tile shape the autotuner
+    (trt_autotune_tile) finds fastest for it on this device."""
+    table = {}
+    tile = (0, 0)
+    if tune:
+        tile, _, table = trt.native.autotune_tile([prob], precision, sweeps=3)
+    ctx = trt.native.Context(prob, precision=precision, tile=tile)
     try:
         if warmup > 0:
             ctx.sweep_timed(warmup)
@@ -63,6 +69,7 @@ def measure(trt, prob, precision: int, steps: int, warmup: int):
         "achieved_gbs": b / (kern / nsw * 1e-3) / 1e9,
         "kernel": kernel_name(precision, mode, tx, ty),
         "sweep_mode": mode, "tile": [tx, ty], "grid": grid,
+        "autotune_ms_per_sweep": {f"{t[0]}x{t[1]}": round(v, 3) for t, v in table.items()},
     }
 
 
@@ -74,15 +81,18 @@ def kernel_name(precision: int, mode: int, tx: int, ty: int) -> str:
     return f"{form}<{ti}, {tc}, {tx}, {ty}>" if form else "sweep_kernel"
 
 
-def traffic_from_profiles(precision: int):
-    """HBM bytes per sweep-kernel launch from the committed rocprofv3 counter summary, if any."""
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of this sweep kernel (template name, tile shape included) from
+    the committed rocprofv3 counter summary, if it holds that kernel."""
     path = os.path.join(ROOT, "profiles", "sweep_cfg1_hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    key = {0: "fp64", 1: "fp32", 2: "fp32s"}[precision]
-    return d.get(key, {}).get("hbm_bytes_per_launch")
+    for v in d.values():
+        if isinstance(v, dict) and v.get("kernel") == kernel:
+            return v.get("hbm_bytes_per_launch")
+    return None
 
 
 def batch_mix(trt, budget_gib: float, base_seed: int = 20240611):
@@ -139,8 +149,63 @@ def measure_steps(trt, precision: int, steps: int = 10):
         dt = (time.perf_counter() - t0) / steps
     finally:
         ctx.close()
+    sweeps = sum(i[3] for i in its)
     return {"s_per_step": dt, "steps": steps, "iterations_per_step": its[-1],
-            "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps"}
+            "end_to_end_updates_per_s": prob.updates_per_sweep() * sweeps / (dt * steps),
+            "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps; end-to-end updates/s = sweep updates "
+                    "of the timed steps over their whole wall time (low-order solves included)"}
+
+
+def draws(trt, n: int = 64, steps: int = 20, base_seed: int = 20240611):
+    return [trt.problem.cfg3_problem(base_seed, i, nsteps=steps) for i in range(n)]
+
+
+def measure_waves(trt, budget_gib: float = 200.0, steps: int = 20, n: int = 64):
+    """The batched distribution in full: all 64 draws (64..512 grids, S8..S16, 16..64 groups,
+    their own lognormal(0, 1.5) maps and boundary temperatures) submitted at once to the
+    request loop, which admits them in waves as the HBM budget frees up and sweeps every
+    admitted problem in one persistent launch per step (FP32 sweep, FP64 moments); 20
+    Tgen_QDf steps each (sweep + moments + QD factors; the maps stop full MLQD steps)."""
+    sv = trt.serve.run_load(1e9, n, steps=steps, budget_gib=budget_gib, precision=1, tile=(4, 4),
+                            problems=draws(trt, n, steps))
+    sv["note"] = (f"{n} draws of the batched distribution submitted at t=0 under a {budget_gib:g} GiB budget, "
+                  f"{steps} Tgen_QDf steps each, FP32 sweep, 4x4 tiles; updates_per_s = all sweep updates over the "
+                  "wall time (admission, context creation and output copies included)")
+    return sv
+
+
+def measure_serving(trt, rates, n: int, steps: int = 20, budget_gib: float = 200.0):
+    """Poisson arrivals at each rate (requests/s) of requests drawn from the batched
+    distribution, FP32 continuous batching; then the same requests solved one at a time in
+    plain FP64 (a context per request, its steps run back to back), the reference point."""
+    out = {"rates": []}
+    for r in rates:
+        print(f"[bench] request loop, {n} requests at {r:g}/s", file=sys.stderr, flush=True)
+        sv = trt.serve.run_load(r, n, steps=steps, budget_gib=budget_gib, precision=1, tile=(4, 4),
+                                problems=draws(trt, n, steps))
+        out["rates"].append(sv)
+    print(f"[bench] {n} requests one at a time, plain FP64", file=sys.stderr, flush=True)
+    lat, upd = [], 0
+    t_all = time.perf_counter()
+    for p in draws(trt, n, steps):
+        t0 = time.perf_counter()
+        ctx = trt.native.Context(p, precision=0)
+        try:
+            for _ in range(steps):
+                ctx.tgen_step()
+            ctx.pull("Temp")                        # the request's result on the host
+        finally:
+            ctx.close()
+        lat.append(time.perf_counter() - t0)
+        upd += p.updates_per_sweep() * steps
+    wall = time.perf_counter() - t_all
+    out["fp64_one_at_a_time"] = {"requests": n, "wall_s": wall, "updates_per_s": upd / wall,
+                                 "service_p50_s": trt.serve.percentile(lat, 50),
+                                 "service_p99_s": trt.serve.percentile(lat, 99),
+                                 "note": "each request alone: context creation, 20 FP64 Tgen_QDf steps, result copy; "
+                                         "latency under arrivals at rate r would add the queueing of a single "
+                                         "server (this is its service time)"}
+    return out
 
 
 def measure_dropin_sweep(trt, prob, reps: int = 2):
@@ -244,13 +309,26 @@ def measure_restart(trt):
                     "from the page cache"}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(trt, budget_s: float = 12.0):
     """The FP64 CPU restatement (oracle) of TRANSPORT_SCB + COLLAPSE_INTENSITIES on the same
     256x256 S16 grid, OpenMP over groups as the reference, one group per thread."""
     import oracle
     oracle.build()
     cores = os.cpu_count() or 1
-    threads = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 32))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else cores
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = max(1, min(avail, int(omp) if omp else avail, 32))
     prob = trt.problem.cfg1()
     ng = threads
     prob = prob.with_(ng=ng, nu=prob.nu[:ng + 1])
@@ -270,8 +348,10 @@ def cpu_baseline(trt, budget_s: float = 12.0):
         "unit": "cell*angle*group updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{n} x (TRANSPORT_SCB + COLLAPSE_INTENSITIES) at 256x256 cells, S16, {ng} of the 32 groups "
-                   f"(one group per OpenMP thread as in the reference), FP64, gcc -O2 -fopenmp; {t_total:.1f} s"),
+        "sample": (f"{ng} of the 32 groups: {n} x (TRANSPORT_SCB + COLLAPSE_INTENSITIES) at 256x256 cells, S16, "
+                   f"FP64, one group per OpenMP thread as in the reference, gcc -O2 -fopenmp (the reference's "
+                   f"-fcheck=all omitted); {t_total:.1f} s"),
+        "cpu_model": cpu_model(), "nproc": cores, "cpus_available": avail, "omp_num_threads": omp,
     }
 
 
@@ -290,7 +370,6 @@ def main(argv=None):
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-array (PCIe-inclusive) sweep")
     ap.add_argument("--no-pod", action="store_true", help="skip the POD reconstruction measurement")
     ap.add_argument("--batch-gib", type=float, default=120.0, help="FP64 device budget of the batch mix")
-    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the load (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -336,7 +415,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(prec),
+            "traffic": traffic_from_profiles(head["kernel"]),
             "kernel": head["kernel"],
             "alg_bytes_per_launch": head["alg_bytes_per_sweep"] / head["launches_per_sweep"],
             "avg_launch_ms": head["launch_avg_ms"],
@@ -352,7 +431,7 @@ def main(argv=None):
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
     for k in res:
-        res[k]["traffic_bytes_per_launch"] = traffic_from_profiles({"fp32": 1, "fp64": 0, "fp32s": 2}[k])
+        out[k]["traffic_bytes_per_launch"] = traffic_from_profiles(res[k]["kernel"])
     if not args.no_batch and not args.only:
         probs = batch_mix(trt, args.batch_gib)
         print(f"[bench] batch of {len(probs)} problems", file=sys.stderr, flush=True)
@@ -368,12 +447,22 @@ def main(argv=None):
         if "fp64" in res:
             # the batched FP32 sweep set against the plain FP64 form (one 256x256 problem per launch)
             out["fp32_batched_over_plain_fp64"] = bt["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"]
-    if not args.no_serve and not args.only:
-        print(f"[bench] request loop, {args.serve_requests} requests at {args.serve_rate}/s", file=sys.stderr,
+    sat = None
+    if not args.no_batch and not args.only:
+        print("[bench] 64 draws of the batched distribution in waves under the HBM budget", file=sys.stderr,
               flush=True)
-        sv = trt.serve.run_load(args.serve_rate, args.serve_requests, steps=20, budget_gib=200.0, precision=1)
-        sv["note"] = ("requests drawn from the batched distribution, 20 Tgen_QDf steps each (one sweep per step), "
-                      "Poisson arrivals, HBM budget 200 GiB, FP32 sweep; latency = completion - arrival")
+        wv = measure_waves(trt)
+        out["batched_64_waves"] = wv
+        sat = wv["completed_per_s"]
+    if not args.no_serve and not args.only:
+        # five arrival rates up to saturation: fractions of the completion rate of the
+        # all-at-once run above (or fixed rates without it)
+        rates = [round(f * sat, 3) for f in (0.25, 0.5, 0.75, 1.0, 1.5)] if sat else [2.0, 4.0, 8.0, 12.0, 16.0]
+        sv = measure_serving(trt, rates, args.serve_requests)
+        sv["note"] = ("requests drawn from the batched distribution (the first draws, in order), 20 Tgen_QDf steps "
+                      "each (one sweep per step), Poisson arrivals, HBM budget 200 GiB, FP32 sweep, continuous "
+                      "batching; latency = completion - arrival; rates are fractions 0.25..1.5 of the completion "
+                      "rate of the all-at-once run")
         out["serving"] = sv
     if not args.no_steps and not args.only:
         print("[bench] full MLQD time steps (256x256, uniform material)", file=sys.stderr, flush=True)
