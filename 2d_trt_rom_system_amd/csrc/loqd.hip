@@ -1278,6 +1278,7 @@ __global__ void kbdt_kernel(size_t n, const double *T, const double *Tm, const d
 int egp_fv_newt(Ctx &c, int mgqd_it)
 {
     if (c.grey.nsys == 0) qd_alloc(c, c.grey, 1);
+    c.newt_kits.clear();
     const int nx = c.nx, ny = c.ny;
     const size_t nc = (size_t)nx * ny, nv = (size_t)(nx + 1) * ny, nh = (size_t)nx * (ny + 1);
     auto get = [&](const char *n, int d0, int d1) -> double * {
@@ -1344,6 +1345,7 @@ int egp_fv_newt(Ctx &c, int mgqd_it)
         TRT_CHECK(hipGetLastError());
         int kits = 0;
         qd_solve(c, c.grey, &kits, true);
+        c.newt_kits.push_back(kits);
         hipLaunchKernelGGL(qd_scatter, dim3(nb(c.grey.nrow)), dim3(256), 0, c.stream, c.grey.x, a.dEa, a.dEv, a.dEh, nc, nv, nh, 1);
         hipLaunchKernelGGL(newt_deltas, dim3(nb(nc)), dim3(256), 0, c.stream, a);
         // LINE_SEARCH on the six delta norms in the reference order (GL:604-636)
@@ -1477,6 +1479,7 @@ int step(Ctx &c, int its_out[4], int ho_form)
     for (auto n : {"HO_E_avg_RTold2", "HO_E_avg_RTold", "Temp_RTold2", "Temp_RTold"}) dfill(c, c.arr(n), 0.0, nc);
     int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0;
     bool rt_conv = false;
+    c.itlog.clear();
     const int rt_start = 2;                  // AL:302
     // ho_form 1 (the ROM runs, HO_Form 1 and 3, AL:573-631): in each RT iteration the
     // QD factors and C_g of the new time level come from the caller's reconstruction
@@ -1501,8 +1504,12 @@ int step(Ctx &c, int its_out[4], int ho_form)
         cp(c, "Temp_MGQDold", "Temp");
         int mg_its = 0;
         bool mg_conv = false;
+        c.itlog.emplace_back();
+        RtItLog &rlog = c.itlog.back();
         while (!mg_conv && mg_its < c.p.maxit_mloqd) {
             mg_its++;
+            rlog.mgl.emplace_back();
+            MgItLog &mlog = rlog.mgl.back();
             mloqd_fv(c, nullptr);            // AL:686-689
             collapse_mg_ef(c);               // AL:692-693
             if (c.p.no_grey) {
@@ -1513,7 +1520,9 @@ int step(Ctx &c, int its_out[4], int ho_form)
                 grey_coefs(c);               // AL:713-715
                 cp(c, "E_avg", "MGQD_E_avg"); cp(c, "E_edgV", "MGQD_E_edgV"); cp(c, "E_edgH", "MGQD_E_edgH");
                 cp(c, "Fx_edgV", "MGQD_Fx_edgV"); cp(c, "Fy_edgH", "MGQD_Fy_edgH");   // AL:717-721
-                newt_total += egp_fv_newt(c, mg_its);    // AL:724-728
+                mlog.newt = egp_fv_newt(c, mg_its);      // AL:724-728
+                mlog.gkits = c.newt_kits;
+                newt_total += mlog.newt;
             }
             material_update(c);              // AL:774
             int tc = 0, ec = 0;
@@ -1523,12 +1532,15 @@ int step(Ctx &c, int its_out[4], int ho_form)
             convergence(c, c.arr("MGQD_E_avg"), c.arr("MGQD_E_avg_MGQDold"), c.arr("MGQD_E_avg_MGQDold2"), (int)nc,
                         c.p.conv_type, c.p.conv_lo, mg_its, &en, &er, &ec);
             mg_conv = tc && ec;
+            mlog.tnorm = tn; mlog.trho = tr; mlog.enorm = en; mlog.erho = er;
             cp(c, "MGQD_E_avg_MGQDold2", "MGQD_E_avg_MGQDold");   // AL:797-800
             cp(c, "MGQD_E_avg_MGQDold", "MGQD_E_avg");
             cp(c, "Temp_MGQDold2", "Temp_MGQDold");
             cp(c, "Temp_MGQDold", "Temp");
         }
         mg_total += mg_its;
+        rlog.mg = mg_its;
+        rlog.kits = c.last_kits;
         int tc = 0, ec = 0;
         double tn, tr, en, er;
         convergence(c, c.arr("Temp"), c.arr("Temp_RTold"), c.arr("Temp_RTold2"), (int)nc, c.p.conv_type, c.p.conv_ho,
@@ -1536,6 +1548,7 @@ int step(Ctx &c, int its_out[4], int ho_form)
         convergence(c, c.arr("HO_E_avg"), c.arr("HO_E_avg_RTold"), c.arr("HO_E_avg_RTold2"), (int)nc, c.p.conv_type,
                     c.p.conv_ho, rt_its, &en, &er, &ec);
         rt_conv = tc && ec;
+        rlog.tnorm = tn; rlog.trho = tr; rlog.enorm = en; rlog.erho = er;
         cp(c, "HO_E_avg_RTold2", "HO_E_avg_RTold");   // AL:834-837
         cp(c, "HO_E_avg_RTold", "HO_E_avg");
         cp(c, "Temp_RTold2", "Temp_RTold");

@@ -77,6 +77,22 @@ struct QDSystems {
     std::vector<int> direct_next;
 };
 
+// Iteration history of a time step, as the reference writes it to its output file
+// (MainSrc/ALGORITHMS.f08:761-829): per MLOQD iteration the Newton count, the Krylov
+// count of each Newton solve and the MLOQD convergence norms; per RT iteration the MLOQD
+// count, the group solves' Krylov counts of its last MLOQD_FV and the RT norms.
+struct MgItLog {
+    int newt = 0;
+    std::vector<int> gkits;
+    double tnorm = 0, enorm = 0, trho = 0, erho = 0;
+};
+struct RtItLog {
+    int mg = 0;
+    std::vector<int> kits;
+    double tnorm = 0, enorm = 0, trho = 0, erho = 0;
+    std::vector<MgItLog> mgl;
+};
+
 struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
 struct Arena;     // one device allocation carved into contexts' blocks (state.hip)
 struct DirectWork;  // block-LU workspace of the direct QD fallback (qd_direct.hip)
@@ -129,6 +145,8 @@ struct Ctx {
     int verbose = 0;                       // TRT_VERBOSE=1: per-solve Krylov statistics on stderr
     double lin_rtol = 1e-13;
     std::vector<int> last_kits;
+    std::vector<int> newt_kits;        // Krylov count of each solve of the latest EGP_FV_NEWT
+    std::vector<RtItLog> itlog;        // iteration history of the latest time step
     DirectWork *direct = nullptr;
     NDPlan *nd = nullptr;
     int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback; 2: block LU

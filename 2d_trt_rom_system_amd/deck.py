@@ -58,6 +58,23 @@ def _int(s: str) -> int:
     return int(s)
 
 
+# [OUTPUT_OPTS] flag keys and the output-file option each sets (INPUTS.f08:1234-1413)
+OUT_FLAG_KEYS = {"I_out": "I_out", "HO_Eg_out": "HO_Eg_out", "HO_Fg_out": "HO_Fg_out", "HO_E_out": "HO_E_out",
+                 "HO_F_out": "HO_F_out", "Eg_out": "Eg_out", "Fg_out": "Fg_out", "MGQD_E_out": "MGQD_E_out",
+                 "MGQD_F_out": "MGQD_F_out", "fg_out": "QDfg_out", "E_out": "E_out", "F_out": "F_out",
+                 "D_out": "D_out", "old_parms_out": "old_parms_out", "its_out": "its_out", "conv_out": "conv_out",
+                 "kap_out": "kap_out", "Src_out": "Src_out"}
+
+
+def _flag(d, key: str, s: str) -> int:
+    """A 0/1 output flag; any other value warns and gives 0, as the reference does."""
+    v = _int(s)
+    if v not in (0, 1):
+        d.warnings.append(f"{key} must be 0 or 1; set to 0")
+        v = 0
+    return v
+
+
 def _is_comment(word: str) -> bool:
     return word[:1] in ("*", "!", "#")
 
@@ -133,6 +150,8 @@ class Deck:
     restart_outfile: str = "restart.out"
     restart_freq: int = 1
     init_out: int = 0
+    # output flags of [OUTPUT_OPTS] (INPUTS.f08:1174-1191; the deck key of QDfg_out is fg_out)
+    out_flags: dict = field(default_factory=lambda: {k: 1 for k in OUT_FLAG_KEYS.values()})
     custom_edges: Optional[List[float]] = None
     start_time: float = 0.0
     warnings: List[str] = field(default_factory=list)
@@ -175,6 +194,15 @@ class Deck:
                     mat[i, j] = 1 if (inside_x and inside_y) else 0
             return mat, np.array([1.0, 10.0])
         raise DeckError(f"Unrecognized Test type ({self.test})")
+
+    def out_opts(self) -> Dict:
+        """The run file's options (native.OutFile keywords) from the deck: run_type,
+        quadrature, energy structure, domain, time axis and the [OUTPUT_OPTS] flags, as
+        the reference passes them to OUTFILE_INIT (MainSrc/main.f08:96-103)."""
+        o = dict(kind=0, run_type=self.run_type, quadrature=self.quadrature, enrgy_strc=self.enrgy_strc,
+                 xlen=self.xlen, ylen=self.ylen, tlen=self.tlen, start_time=self.start_time, init_out=self.init_out)
+        o.update(self.out_flags)
+        return o
 
     def problem(self, nsteps: Optional[int] = None) -> P.Problem:
         """The Problem of the deck (reference code units: cm, sh, eV, erg/comp_unit)."""
@@ -375,8 +403,10 @@ def read_deck(text: str, restart_time=None) -> Deck:
             d.restart_outfile = a[0]
         elif key == "restart_freq":
             d.restart_freq = _int(a[0])
-        elif key == "Init_out":
-            d.init_out = _int(a[0])
+        elif key == "init_out":       # INPUTS.f08:1414 (the key is lower case)
+            d.init_out = _flag(d, key, a[0])
+        elif key in OUT_FLAG_KEYS:
+            d.out_flags[OUT_FLAG_KEYS[key]] = _flag(d, key, a[0])
 
     if d.enrgy_strc == "custom":
         k = locate_block(lines, "[ENERGY_STRUCTURE]")

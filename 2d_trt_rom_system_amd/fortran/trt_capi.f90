@@ -30,6 +30,24 @@ MODULE TRT_CAPI
   ! the context of the running problem, created by TRT_GPU_INIT
   TYPE(C_PTR), SAVE :: TRT_CTX = C_NULL_PTR
 
+  ! options of the output file (trt_outopts); flags in the order I_out, HO_Eg_out, HO_Fg_out, HO_E_out,
+  ! HO_F_out, Eg_out, Fg_out, MGQD_E_out, MGQD_F_out, fg_out, E_out, F_out, D_out, old_parms_out, its_out,
+  ! conv_out, kap_out, Src_out
+  TYPE, BIND(C) :: trt_outopts
+    INTEGER(C_INT) :: kind
+    TYPE(C_PTR) :: run_type, quadrature, enrgy_strc
+    REAL(C_DOUBLE) :: xlen, ylen, tlen, start_time
+    INTEGER(C_INT) :: n_t, init_out
+    INTEGER(C_INT) :: flags(18)
+    INTEGER(C_INT) :: nofill, format
+    TYPE(C_PTR) :: pod_type, dmd_type
+    INTEGER(C_INT) :: direc_diff, n_dsets
+    TYPE(C_PTR) :: dset_times, dsets, tpts
+  END TYPE trt_outopts
+
+  ! the output file of the running problem, created by TRT_GPU_OUTFILE_INIT
+  TYPE(C_PTR), SAVE :: TRT_OUT = C_NULL_PTR
+
   INTERFACE
     INTEGER(C_INT) FUNCTION trt_ctx_create(ctx, p) BIND(C, name="trt_ctx_create")
       IMPORT :: C_INT, C_PTR, trt_problem
@@ -163,6 +181,27 @@ MODULE TRT_CAPI
       TYPE(C_PTR), VALUE :: ctx
       CHARACTER(KIND=C_CHAR) :: path(*)
     END FUNCTION
+    ! the reference's NetCDF output file (OUTFILE_INIT + OUTFILE_VARDEFS, OUTPUTS.f08:13,839; TIMESTEP_OUTS :1429)
+    INTEGER(C_INT) FUNCTION trt_outfile_create(ctx, path, opts, f) BIND(C, name="trt_outfile_create")
+      IMPORT :: C_INT, C_PTR, C_CHAR, trt_outopts
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR) :: path(*)
+      TYPE(trt_outopts) :: opts
+      TYPE(C_PTR) :: f
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_timestep_outs(f, t, after_step) BIND(C, name="trt_timestep_outs")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: f
+      INTEGER(C_INT), VALUE :: t, after_step
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_outfile_close(f) BIND(C, name="trt_outfile_close")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: f
+    END FUNCTION
+    TYPE(C_PTR) FUNCTION trt_outfile_last_error(f) BIND(C, name="trt_outfile_last_error")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: f
+    END FUNCTION
   END INTERFACE
 
 CONTAINS
@@ -199,6 +238,56 @@ SUBROUTINE TRT_GPU_RESTART_IN(restart_infile)
   CHARACTER(*), INTENT(IN) :: restart_infile
   CALL TRT_CHECK(trt_restart_in(TRT_CTX, TRIM(restart_infile)//C_NULL_CHAR), 'RESTART_IN')
 END SUBROUTINE TRT_GPU_RESTART_IN
+
+!----------------------------------------------------------------------------------------------------------------------------------!
+! the output file of the running context: created where the reference calls OUTFILE_INIT (main.f08:96) and
+! OUTFILE_VARDEFS (ALGORITHMS.f08:251), written where it calls TIMESTEP_OUTS (ALGORITHMS.f08:269 for the
+! initial state with Init_out = 1, after_step = 0; :872 after each time step t, after_step = 1) and closed
+! where it calls NF_CLOSE_FILE (main.f08:121)
+!----------------------------------------------------------------------------------------------------------------------------------!
+SUBROUTINE TRT_GPU_OUTFILE_INIT(outfile, run_type, quadrature, enrgy_strc, xlen, ylen, tlen, start_time, init_out, flags)
+  CHARACTER(*), INTENT(IN) :: outfile, run_type, quadrature, enrgy_strc
+  REAL(C_DOUBLE), INTENT(IN) :: xlen, ylen, tlen, start_time
+  INTEGER, INTENT(IN) :: init_out, flags(18)
+  TYPE(trt_outopts) :: o
+  CHARACTER(KIND=C_CHAR, LEN=256), TARGET :: rt, qd, es
+  rt = TRIM(run_type)//C_NULL_CHAR
+  qd = TRIM(quadrature)//C_NULL_CHAR
+  es = TRIM(enrgy_strc)//C_NULL_CHAR
+  CALL TRT_REQUIRE_CTX('TRT_GPU_OUTFILE_INIT')
+  o%kind = 0
+  o%run_type = C_LOC(rt); o%quadrature = C_LOC(qd); o%enrgy_strc = C_LOC(es)
+  o%xlen = xlen; o%ylen = ylen; o%tlen = tlen; o%start_time = start_time
+  o%n_t = 0; o%init_out = init_out; o%flags = flags
+  o%nofill = 0; o%format = 0
+  o%pod_type = C_NULL_PTR; o%dmd_type = C_NULL_PTR; o%direc_diff = 0; o%n_dsets = 0
+  o%dset_times = C_NULL_PTR; o%dsets = C_NULL_PTR; o%tpts = C_NULL_PTR
+  CALL TRT_CHECK(trt_outfile_create(TRT_CTX, TRIM(outfile)//C_NULL_CHAR, o, TRT_OUT), 'OUTFILE_INIT')
+END SUBROUTINE TRT_GPU_OUTFILE_INIT
+
+SUBROUTINE TRT_GPU_TIMESTEP_OUTS(t, after_step)
+  INTEGER, INTENT(IN) :: t, after_step
+  TYPE(C_PTR) :: p
+  CHARACTER(KIND=C_CHAR), POINTER :: s(:)
+  INTEGER :: k
+  IF (trt_timestep_outs(TRT_OUT, INT(t, C_INT), INT(after_step, C_INT)) .NE. 0) THEN
+    p = trt_outfile_last_error(TRT_OUT)
+    CALL C_F_POINTER(p, s, [512])
+    WRITE(*,'(A)', ADVANCE='NO') ' TRT GPU: TIMESTEP_OUTS failed: '
+    DO k = 1, 512
+      IF (s(k) .EQ. C_NULL_CHAR) EXIT
+      WRITE(*,'(A)', ADVANCE='NO') s(k)
+    END DO
+    WRITE(*,*)
+    ERROR STOP 'TRT GPU call failed'
+  END IF
+END SUBROUTINE TRT_GPU_TIMESTEP_OUTS
+
+SUBROUTINE TRT_GPU_OUTFILE_CLOSE()
+  IF (.NOT. C_ASSOCIATED(TRT_OUT)) RETURN
+  CALL TRT_CHECK(trt_outfile_close(TRT_OUT), 'NF_CLOSE_FILE')
+  TRT_OUT = C_NULL_PTR
+END SUBROUTINE TRT_GPU_OUTFILE_CLOSE
 
 SUBROUTINE TRT_CHECK(rc, what)
   INTEGER(C_INT), INTENT(IN) :: rc

@@ -7,10 +7,13 @@
 ! state in host arrays as the reference keeps it.
 ! Mode 2 (device-resident): the same step on the GPU through trt_step, state kept on the device between steps.
 !
-! Usage: trt_mlqd <problem.bin> <results.bin> [<restart_infile> [<restart_outfile> <restart_freq>]]
+! Usage: trt_mlqd <problem.bin> <results.bin> [<restart_infile> [<restart_outfile> <restart_freq> [<outfile> [<flags>]]]]
 !   mode 2 only: the state is read from restart_infile before the time loop ('-' for none; RESTART_IN, ALGORITHMS.f08:234-237)
 !   and written to restart_outfile every restart_freq-th step (RESTART_OUT, ALGORITHMS.f08:889-894), in the reference's
-!   restart file format.
+!   restart file format ('-' and 0 for none); outfile: the reference's NetCDF output file, written after every step
+!   (TIMESTEP_OUTS, ALGORITHMS.f08:872) with the initial state first (Init_out = 1); flags: 18 digits 0/1 in the order
+!   I_out HO_Eg_out HO_Fg_out HO_E_out HO_F_out Eg_out Fg_out MGQD_E_out MGQD_F_out fg_out E_out F_out D_out old_parms_out
+!   its_out conv_out kap_out Src_out (default: all 1, the reference's defaults, INPUTS.f08:1174-1191).
 !   problem.bin (stream, native endianness): int32 nx ny nm ng nmat nsteps opacity_model maxit_rte maxit_mloqd maxit_gloqd
 !   conv_type use_line_search kapE_dT_flag bc_type(4) precision mode; float64 c dt theta comp_unit cv kap0 tini bcT(4)
 !   conv_ho conv_lo conv_gr1 conv_gr2 chi line_src; float64 Delx(nx) Dely(ny) Omega_x(nm) Omega_y(nm) quad_weight(nm)
@@ -34,8 +37,8 @@ PROGRAM trt_mlqd
   REAL*8:: c, Delt, Theta, Comp_Unit, cv, Kap0, Tini, bcT(4), conv_ho, conv_lo, conv_gr1, conv_gr2, chi, line_src
   REAL*8,ALLOCATABLE:: Delx(:), Dely(:), Omega_x(:), Omega_y(:), quad_weight(:), Nu_g(:), Kappa_Mult(:), A(:,:)
   INTEGER,ALLOCATABLE:: Mat(:,:)
-  CHARACTER(LEN=1024):: fin, fout, restart_infile, restart_outfile, freq_arg
-  INTEGER:: restart_freq
+  CHARACTER(LEN=1024):: fin, fout, restart_infile, restart_outfile, freq_arg, ncfile, flag_arg
+  INTEGER:: restart_freq, out_flags(18), k
   INTEGER:: t, u, v, i, j
   INTEGER(C_INT):: its4(4)
 
@@ -86,6 +89,16 @@ PROGRAM trt_mlqd
     CALL GET_COMMAND_ARGUMENT(5, freq_arg)
     READ(freq_arg, *) restart_freq
   END IF
+  IF (TRIM(restart_outfile) .EQ. '-') restart_outfile = ''
+  ncfile = ''
+  out_flags = 1
+  IF (COMMAND_ARGUMENT_COUNT() .GE. 6) CALL GET_COMMAND_ARGUMENT(6, ncfile)
+  IF (COMMAND_ARGUMENT_COUNT() .GE. 7) THEN
+    CALL GET_COMMAND_ARGUMENT(7, flag_arg)
+    DO k = 1, 18
+      out_flags(k) = MERGE(1, 0, flag_arg(k:k) .EQ. '1')
+    END DO
+  END IF
   OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
   READ(u) N_x, N_y, N_m, N_g, N_mat, N_steps, opacity_model, Maxit_RTE, Maxit_MLOQD, Maxit_GLOQD, Conv_Type, &
     line_flag, dT_flag, BC_Type, precision, mode
@@ -114,8 +127,14 @@ PROGRAM trt_mlqd
     !  device-resident: one trt_step per time step                     !
     !------------------------------------------------------------------!
     IF (LEN_TRIM(restart_infile) .GT. 0) CALL TRT_GPU_RESTART_IN(restart_infile)
+    IF (LEN_TRIM(ncfile) .GT. 0) THEN
+      CALL TRT_GPU_OUTFILE_INIT(ncfile, 'mlqd', 'custom', 'custom', SUM(Delx), SUM(Dely), trt_time(TRT_CTX) + N_steps*Delt, &
+        trt_time(TRT_CTX), 1, out_flags)
+      CALL TRT_GPU_TIMESTEP_OUTS(1, 0)
+    END IF
     DO t = 1, N_steps
       CALL TRT_CHECK(trt_step(TRT_CTX, its4), 'trt_step')
+      IF (LEN_TRIM(ncfile) .GT. 0) CALL TRT_GPU_TIMESTEP_OUTS(t + 1, 1)
       CALL TRT_PULL_ARRAY('Temp', Temp, SIZE(Temp))
       CALL TRT_PULL_ARRAY('E_avg', E_avg, SIZE(E_avg))
       CALL TRT_PULL_ARRAY('HO_E_avg', HO_E_avg, SIZE(HO_E_avg))
@@ -129,6 +148,7 @@ PROGRAM trt_mlqd
       END IF
     END DO
     CLOSE(v)
+    CALL TRT_GPU_OUTFILE_CLOSE()
     CALL TRT_GPU_FINALIZE()
     STOP
   END IF

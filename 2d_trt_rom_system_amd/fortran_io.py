@@ -50,19 +50,25 @@ def read_results(path: str, prob):
 
 
 def run(prob, workdir: str, mode: int = DROPIN, precision: int = 0, timeout: float = 600.0,
-        restart_in: str | None = None, restart_out: str | None = None, restart_freq: int = 0):
+        restart_in: str | None = None, restart_out: str | None = None, restart_freq: int = 0,
+        outfile: str | None = None, out_flags: str | None = None):
     """Run the Fortran host program on prob; returns (per-step results, stdout).  In mode
     RESIDENT the state can be read from a restart file before the time loop (restart_in)
-    and written to one every restart_freq-th step (restart_out)."""
+    and written to one every restart_freq-th step (restart_out), and the reference's NetCDF
+    output file written to outfile (out_flags: 18 digits 0/1, the OUTPUT_OPTS flags)."""
     os.makedirs(workdir, exist_ok=True)
     pin = os.path.join(workdir, "problem.bin")
     pout = os.path.join(workdir, "results.bin")
     write_problem(pin, prob, mode, precision)
     args = [EXE, pin, pout]
-    if restart_in or restart_freq > 0:
+    if restart_in or restart_freq > 0 or outfile:
         args.append(restart_in or "-")
-    if restart_freq > 0:
-        args += [restart_out, str(int(restart_freq))]
+    if restart_freq > 0 or outfile:
+        args += [restart_out or "-", str(int(restart_freq))]
+    if outfile:
+        args.append(outfile)
+        if out_flags:
+            args.append(out_flags)
     r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"trt_mlqd failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")

@@ -54,7 +54,9 @@ EXPORTS = [
     "trt_restart_out", "trt_restart_in", "trt_restart_time_in", "trt_frecords_write", "trt_frecords_read",
     "trt_frecords_last_error", "trt_step_given_qdf", "trt_pod_create", "trt_pod_reconstruct", "trt_pod_destroy",
     "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
-    "trt_convergence_noalloc", "trt_meb_solve",
+    "trt_convergence_noalloc", "trt_meb_solve", "trt_outfile_create", "trt_timestep_outs", "trt_tgen_outs",
+    "trt_outfile_put", "trt_outfile_close", "trt_outfile_last_error", "trt_nc_open", "trt_nc_dim", "trt_nc_get",
+    "trt_nc_last_error", "trt_nc_close",
 ]
 
 _lib = None
@@ -163,6 +165,19 @@ def load(path: str = LIBPATH):
     L.trt_frecords_write.argtypes = [C.c_char_p, C.c_int, C.POINTER(_dp), C.POINTER(C.c_size_t), C.c_int]
     L.trt_frecords_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(_dp), C.POINTER(C.c_size_t)]
     L.trt_frecords_last_error.restype = C.c_char_p
+    L.trt_outfile_create.argtypes = [vp, C.c_char_p, C.POINTER(OutOpts), C.POINTER(vp)]
+    L.trt_timestep_outs.argtypes = [vp, C.c_int, C.c_int]
+    L.trt_tgen_outs.argtypes = [vp, C.c_int]
+    L.trt_outfile_put.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _dp]
+    L.trt_outfile_close.argtypes = [vp]
+    L.trt_outfile_last_error.argtypes = [vp]
+    L.trt_outfile_last_error.restype = C.c_char_p
+    L.trt_nc_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.trt_nc_dim.argtypes = [vp, C.c_char_p, C.POINTER(C.c_size_t)]
+    L.trt_nc_get.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _dp]
+    L.trt_nc_last_error.argtypes = [vp]
+    L.trt_nc_last_error.restype = C.c_char_p
+    L.trt_nc_close.argtypes = [vp]
     _lib = L
     return L
 
@@ -220,6 +235,21 @@ def calib_copy(nbytes: int, reps: int = 10) -> float:
 def device_bytes(prob, precision: int = 0, tile=(0, 0), sweep_mode: int = 0) -> int:
     s, keep = make_problem_struct(prob, precision, tile, sweep_mode=sweep_mode)
     return int(load().trt_ctx_bytes(C.byref(s)))
+
+
+OUT_FLAGS = ("I_out", "HO_Eg_out", "HO_Fg_out", "HO_E_out", "HO_F_out", "Eg_out", "Fg_out", "MGQD_E_out",
+             "MGQD_F_out", "QDfg_out", "E_out", "F_out", "D_out", "old_parms_out", "its_out", "conv_out", "kap_out",
+             "Src_out")
+
+
+class OutOpts(C.Structure):
+    """Mirror of trt_outopts (include/trt_capi.h)."""
+    _fields_ = ([("kind", C.c_int), ("run_type", C.c_char_p), ("quadrature", C.c_char_p), ("enrgy_strc", C.c_char_p),
+                 ("xlen", C.c_double), ("ylen", C.c_double), ("tlen", C.c_double), ("start_time", C.c_double),
+                 ("n_t", C.c_int), ("init_out", C.c_int)]
+                + [(f, C.c_int) for f in OUT_FLAGS]
+                + [("nofill", C.c_int), ("format", C.c_int), ("pod_type", C.c_char_p), ("dmd_type", C.c_char_p), ("direc_diff", C.c_int),
+                   ("n_dsets", C.c_int), ("dset_times", _dp), ("dsets", C.POINTER(C.c_char_p)), ("tpts", _dp)])
 
 
 class Context:
@@ -578,3 +608,105 @@ def autotune_batch(probs, precision: int = 1, sweeps: int = 3):
         raise TrtError(f"trt_autotune_batch failed ({rc}): {L.trt_autotune_last_error().decode()}")
     table = {(sh, sg): all_ms[i * len(segs) + q] for i, sh in enumerate(shapes) for q, sg in enumerate(segs)}
     return (tx.value, ty.value), seg.value, best.value, table
+
+
+class OutFile:
+    """The reference's NetCDF output file of a context (trt_outfile_*): OUTFILE_INIT +
+    OUTFILE_VARDEFS (kind 0) or OUTFILE_INIT_Tgen (kind 1), MainSrc/OUTPUTS.f08.  Keyword
+    arguments set trt_outopts fields by name (the OUTPUT_OPTS flags, run_type, tlen, ...)."""
+
+    def __init__(self, ctx: "Context", path: str, **opts):
+        self.L = load()
+        self.ctx = ctx
+        o = OutOpts()
+        self._keep = []
+        for k, v in opts.items():
+            if k in ("run_type", "quadrature", "enrgy_strc", "pod_type", "dmd_type"):
+                b = str(v).encode()
+                self._keep.append(b)
+                setattr(o, k, b)
+            elif k in ("dset_times", "tpts"):
+                a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+                self._keep.append(a)
+                setattr(o, k, a.ctypes.data_as(_dp))
+            elif k == "dsets":
+                arr = (C.c_char_p * len(v))(*[str(x).encode() for x in v])
+                self._keep.append(arr)
+                o.dsets = arr
+            else:
+                setattr(o, k, v)
+        h = C.c_void_p()
+        rc = self.L.trt_outfile_create(ctx.h, os.fsencode(path), C.byref(o), C.byref(h))
+        if rc != 0:
+            raise TrtError(f"trt_outfile_create failed ({rc}): {self.L.trt_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise TrtError(f"{what} failed ({rc}): {self.L.trt_outfile_last_error(self.h).decode()}")
+
+    def timestep_outs(self, t: int, after_step: bool = True) -> None:
+        self._check(self.L.trt_timestep_outs(self.h, int(t), int(bool(after_step))), "timestep_outs")
+
+    def tgen_outs(self, t: int) -> None:
+        self._check(self.L.trt_tgen_outs(self.h, int(t)), "tgen_outs")
+
+    def put(self, var: str, start, count, vals) -> None:
+        """nf90_put_var: Fortran-order start (1-based) and count."""
+        a = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).ravel(order="F"))
+        n = len(start)
+        st, ct = (C.c_size_t * n)(*start), (C.c_size_t * n)(*count)
+        self._check(self.L.trt_outfile_put(self.h, var.encode(), n, st, ct, a.ctypes.data_as(_dp)), f"put {var}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            if self.L.trt_outfile_close(h) != 0:
+                raise TrtError(f"trt_outfile_close failed: {self.L.trt_last_error(None).decode()}")
+
+    def __del__(self):
+        try:
+            self.close()
+        except TrtError:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class NcReader:
+    """A classic-format NetCDF file read by the library (trt_nc_*): NF_INQ_DIM and
+    NF_INQ_VAR_nD of MainSrc/NCDF_IO.f08.  Host only."""
+
+    def __init__(self, path: str):
+        self.L = load()
+        h = C.c_void_p()
+        if self.L.trt_nc_open(os.fsencode(path), C.byref(h)) != 0:
+            raise TrtError(f"trt_nc_open failed: {self.L.trt_last_error(None).decode()}")
+        self.h = h
+
+    def dim(self, name: str) -> int:
+        n = C.c_size_t()
+        if self.L.trt_nc_dim(self.h, name.encode(), C.byref(n)) != 0:
+            raise TrtError(self.L.trt_nc_last_error(self.h).decode())
+        return int(n.value)
+
+    def get(self, var: str, start, count) -> np.ndarray:
+        """Fortran-order start (1-based) and count; the values as a Fortran-order array."""
+        nd = len(count)
+        out = np.empty(int(np.prod(count)) if nd else 1, dtype=np.float64)
+        st, ct = (C.c_size_t * max(nd, 1))(*start), (C.c_size_t * max(nd, 1))(*count)
+        if self.L.trt_nc_get(self.h, var.encode(), nd, st, ct, out.ctypes.data_as(_dp)) != 0:
+            raise TrtError(self.L.trt_nc_last_error(self.h).decode())
+        return out.reshape(tuple(count), order="F") if nd else out[0]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.trt_nc_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

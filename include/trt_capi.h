@@ -225,6 +225,67 @@ int trt_frecords_write(const char *path, int nrec, const double *const *data, co
 int trt_frecords_read(const char *path, int nrec, double *const *data, const size_t *count);
 const char *trt_frecords_last_error(void);
 
+/* ---- Output files -----------------------------------------------------------
+ * The reference's NetCDF output file, written from the context's device state.
+ * trt_outfile_create ≙ OUTFILE_INIT + OUTFILE_VARDEFS (MainSrc/OUTPUTS.f08:13,839; kind 0,
+ * the run file of run_type mlqd, mg_pod, mg_dmd) or OUTFILE_INIT_Tgen (:412; kind 1, the
+ * Tgen_QDf file); trt_timestep_outs ≙ TIMESTEP_OUTS (:1429) at time index t (1-based) --
+ * with after_step = 1 also the per-step writes of TRT_MLQD_ALGORITHM after a time step
+ * (MainSrc/ALGORITHMS.f08:859-869: E_xWave_Speed, Temp_xWave_Speed, and the iteration
+ * counts and convergence norms of the step, ALGORITHMS.f08:688-829); 0 for the initial
+ * state (Init_out, :268-270); trt_tgen_outs ≙ the writes of Tgen_QDf's time loop
+ * (:1036-1095) after trt_tgen_step; trt_outfile_put ≙ the driver's own nf90_put_var calls
+ * (PODgsum, rrank_*, ..., :449-506), Fortran order, 1-based starts; trt_outfile_close ≙
+ * NF_CLOSE_FILE (MainSrc/NCDF_IO.f08:120).  Dimension, variable and attribute names,
+ * types, units and shapes are the reference's (netCDF readers see the Fortran dimensions
+ * reversed, e.g. Temperature(N_t,N_y,N_x)).  The container is the NetCDF classic 64-bit
+ * offset format (CDF-5, 64-bit data, when a variable exceeds 4 GiB) instead of
+ * NETCDF4/HDF5 (NCDF_IO.f08:71); unwritten values hold the default fill values.  The
+ * reference's three unlimited iteration dimensions (RT_Its-dim, MGQD_Its-dim,
+ * GQD_Its-dim) become fixed ones as long as the longest iteration sequence the run wrote;
+ * they and the iteration variables are added at close.  Res_Calc residual variables are
+ * not written (the GPU path has no Res_Calc).  The file keeps a pointer to the context:
+ * close it first.  Errors: trt_outfile_last_error (trt_last_error(NULL) after a failed
+ * create or close). */
+typedef struct trt_outfile trt_outfile;
+typedef struct {
+    int kind;                          /* 0: run file; 1: Tgen_QDf file */
+    const char *run_type, *quadrature, *enrgy_strc;   /* attribute text (trailing blanks dropped) */
+    double xlen, ylen;                 /* domain (0: sums of Delx, Dely) */
+    double tlen, start_time;
+    int n_t;                           /* time points; 0: NINT((tlen - start_time)/dt), +1 with
+                                          init_out (MainSrc/INPUTS.f08:120-123) */
+    int init_out;
+    int I_out, HO_Eg_out, HO_Fg_out, HO_E_out, HO_F_out, Eg_out, Fg_out, MGQD_E_out, MGQD_F_out,
+        QDfg_out, E_out, F_out, D_out, old_parms_out, its_out, conv_out, kap_out, Src_out;
+    int nofill;                        /* 1: no fill values (nf90_set_fill NOFILL) */
+    int format;                        /* 0: CDF-2, or CDF-5 when a variable exceeds 4 GiB; 5: CDF-5 */
+    /* ROM runs (run_type mg_pod / mg_dmd, OUTPUTS.f08:98-136) */
+    const char *pod_type, *dmd_type;
+    int direc_diff, n_dsets;
+    const double *dset_times;          /* n_dsets values */
+    const char *const *dsets;          /* n_dsets database names */
+    const double *tpts;                /* n_t time points (kind 1: the temperature file's);
+                                          NULL: from start_time and dt (INPUTS.f08:144-151) */
+} trt_outopts;
+int trt_outfile_create(trt_ctx *ctx, const char *path, const trt_outopts *opts, trt_outfile **out);
+int trt_timestep_outs(trt_outfile *f, int t, int after_step);
+int trt_tgen_outs(trt_outfile *f, int t);
+int trt_outfile_put(trt_outfile *f, const char *var, int ndims, const size_t *start, const size_t *count,
+                    const double *vals);
+int trt_outfile_close(trt_outfile *f);
+const char *trt_outfile_last_error(const trt_outfile *f);
+/* Reading a classic-format file (CDF-1, CDF-2, CDF-5): the reads of NF_INQ_DIM and
+ * NF_INQ_VAR_nD (MainSrc/NCDF_IO.f08:339-460), e.g. Tgen_QDf's temperature input
+ * (ALGORITHMS.f08:1038).  Fortran order, 1-based starts, values converted to FP64.
+ * No device, no context. */
+typedef struct trt_ncread trt_ncread;
+int trt_nc_open(const char *path, trt_ncread **out);
+int trt_nc_dim(trt_ncread *r, const char *name, size_t *len);
+int trt_nc_get(trt_ncread *r, const char *var, int ndims, const size_t *start, const size_t *count, double *out);
+const char *trt_nc_last_error(const trt_ncread *r);
+int trt_nc_close(trt_ncread *r);
+
 /* Sweep timing: runs nsweeps sweeps on the context's stream and returns the
  * total elapsed milliseconds measured with HIP events on that stream. */
 int trt_sweep_timed(trt_ctx *ctx, int nsweeps, double *ms);

@@ -51,6 +51,22 @@ def test_reference_deck():
     assert p.nu[0] == 0.0 and p.nu[1] == 707.5 and p.nu[-1] == 1e10
 
 
+def test_output_flags():
+    """[OUTPUT_OPTS] flags (INPUTS.f08:1174-1421): the reference deck's values (its I_out 0,
+    every other flag 1, fg_out setting QDfg_out), the defaults (all 1, init_out 0), the
+    lower-case init_out key, and the fallback to 0 with a warning."""
+    d = D.read_deck_file(GOLDEN)
+    want = {k: 1 for k in D.OUT_FLAG_KEYS.values()}
+    want["I_out"] = 0
+    assert d.out_flags == want and d.init_out == 0
+    o = d.out_opts()
+    assert o["run_type"] == "mg_dmd" and o["QDfg_out"] == 1 and o["I_out"] == 0 and o["tlen"] == 0.2
+    e = D.read_deck(deck(extra="[OUTPUT_OPTS]\n  init_out 1\n  fg_out 0\n  kap_out 7\n  Init_out 0\n"))
+    assert e.init_out == 1 and e.out_flags["QDfg_out"] == 0 and e.out_flags["kap_out"] == 0
+    assert e.out_flags["Src_out"] == 1
+    assert any("kap_out" in w for w in e.warnings)
+
+
 def test_cv_from_radiation_constant():
     """C_v = 0.5917 a_R T_left^3 (INPUTS.f08:121) in code units, against a_R from CODATA
     (7.5657e-15 erg cm^-3 K^-4, 11604.518 K/eV): agreement to the reference constants'
