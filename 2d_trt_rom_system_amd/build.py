@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrt_hip.so")
-SOURCES = ["capi.hip", "state.hip", "sweep.hip", "sweep_pk.hip", "physics.hip", "loqd.hip", "qd_direct.hip", "qd_nd.hip", "serve.hip", "tune.hip", "restart.hip", "rom.hip", "noalloc.hip", "ncfile.hip", "outputs.hip"]
+SOURCES = ["capi.hip", "state.hip", "sweep.hip", "sweep_pk.hip", "physics.hip", "loqd.hip", "qd_direct.hip", "qd_nd.hip", "serve.hip", "tune.hip", "restart.hip", "rom.hip", "noalloc.hip", "ncfile.hip", "outputs.hip", "romgrid.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -105,6 +105,7 @@ void ctx_destroy(trt_ctx *ctx)
     Ctx &c = ctx->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     trt::rom_release_all(c);
+    trt::romgrid_release_all(c);
     trt::pk_destroy(c);
     trt::direct_destroy(c);
     trt::nd_destroy(c);
@@ -471,9 +472,13 @@ int trt_batch_sweep_timed(trt_batch *b, int nsweeps, double *ms)
 int trt_batch_tgen_step(trt_batch *b, double *sweep_ms)
 {
     return bguarded(b, [&](trt::PkBatch &pb) {
-        for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_pre(*c);
-        trt::pk_batch_sweep(pb, 3);
-        for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_post(*c);
+        if (trt::pk_batch_step_graphs(pb)) {
+            trt::pk_batch_tgen_step_graph(pb);
+        } else {
+            for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_pre(*c);
+            trt::pk_batch_sweep(pb, 3);
+            for (Ctx *c : trt::pk_batch_ctxs(pb)) trt::tgen_post(*c);
+        }
         sync_members(pb);
         const double t = trt::pk_batch_elapsed_ms(pb);
         trt::pk_batch_check(pb);
@@ -483,7 +488,10 @@ int trt_batch_tgen_step(trt_batch *b, double *sweep_ms)
 
 int trt_batch_use_graph(trt_batch *b, int on)
 {
-    return bguarded(b, [&](trt::PkBatch &pb) { trt::pk_batch_use_graph(pb, on != 0); });
+    return bguarded(b, [&](trt::PkBatch &pb) {
+        if (on < 0 || on > 2) throw Error("graph mode must be 0, 1 or 2");
+        trt::pk_batch_use_graph(pb, on);
+    });
 }
 
 int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems)

@@ -1082,6 +1082,19 @@ struct PkBatch {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // whole Tgen_QDf steps as graphs, one per I_crn_old buffer signature of the members
+    // (the handover swaps each member's two buffers every step, so a batch stepping
+    // together alternates between two graphs)
+    bool step_graphs_on = false;
+    struct StepGraph {
+        std::vector<int> sig;       // each member's I_crn_old buffer in the graph's step
+        PkProb *table = nullptr;    // device descriptor table of that signature
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<StepGraph> step_graphs;
+    hipEvent_t fork = nullptr, fork2 = nullptr;
+    std::vector<hipEvent_t> joined;
 };
 
 static void *batch_alloc(PkBatch &b, size_t bytes)
@@ -1101,6 +1114,13 @@ void pk_batch_destroy(PkBatch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto &g : b->step_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    for (hipEvent_t e : b->joined) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->fork, b->fork2})
+        if (e) (void)hipEventDestroy(e);
     for (size_t k = 0; k < b->owned.size(); ++k) {
         if (b->pooled) cache_release(b->owned[k], b->owned_bytes[k]);
         else (void)hipFree(b->owned[k]);
@@ -1224,12 +1244,13 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
     return b;
 }
 
-static void batch_enqueue(PkBatch &b)
+static void batch_enqueue(PkBatch &b, const PkProb *table = nullptr)
 {
+    if (!table) table = b.probs;
     hipLaunchKernelGGL(pk_reset, dim3(1), dim3(64), 0, b.stream, b.queue);
-    boundary_moments(b.precision, b.probs, (int)b.ctxs.size(), b.nbt, b.stream);
+    boundary_moments(b.precision, table, (int)b.ctxs.size(), b.nbt, b.stream);
     PkArgs a{};
-    a.probs = b.probs;
+    a.probs = table;
     a.items = b.items;
     a.nitems = b.nitems;
     a.queue = b.queue;
@@ -1291,7 +1312,91 @@ void pk_batch_sweep(PkBatch &b, int mark)
     }
 }
 
-void pk_batch_use_graph(PkBatch &b, bool on) { b.use_graph = on; }
+void pk_batch_use_graph(PkBatch &b, int mode)
+{
+    b.use_graph = mode == 1;
+    b.step_graphs_on = mode == 2;
+}
+
+bool pk_batch_step_graphs(const PkBatch &b) { return b.step_graphs_on; }
+
+// One Tgen_QDf step of every member (MATERIAL_UPDATE, I_crn_old = I_crn, the batched
+// TRANSPORT_SCB, COLLAPSE_INTENSITIES + fg_Calc, Cg_Calc) as one graph launch on the batch
+// stream.  The graph forks to each member's stream for its MATERIAL_UPDATE, joins for the
+// persistent sweep, and forks again for the moments, QD factors and C_g; it is captured
+// on first use for each signature of the members' I_crn_old buffers and replayed after.
+// The captured work is kernels only (no memset or copy nodes).  Events t0 and t1 bracket
+// the whole step.
+void pk_batch_tgen_step_graph(PkBatch &b)
+{
+    const int n = (int)b.ctxs.size();
+    std::vector<int> sig(n);
+    for (int i = 0; i < n; ++i) sig[i] = b.ctxs[i]->icur;   // the buffer the handover makes I_crn_old
+    PkBatch::StepGraph *sg = nullptr;
+    for (auto &g : b.step_graphs)
+        if (g.sig == sig) sg = &g;
+    // the step starts after the work already enqueued on every member's stream
+    for (int i = 0; i < n; ++i) {
+        TRT_CHECK(hipEventRecord(b.ready[i], b.ctxs[i]->stream));
+        TRT_CHECK(hipStreamWaitEvent(b.stream, b.ready[i], 0));
+    }
+    if (!sg) {
+        if (!b.fork) {
+            TRT_CHECK(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
+            TRT_CHECK(hipEventCreateWithFlags(&b.fork2, hipEventDisableTiming));
+            b.joined.resize(n, nullptr);
+            for (auto &e : b.joined) TRT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        PkBatch::StepGraph g;
+        g.sig = sig;
+        std::vector<PkProb> host(n);
+        for (int i = 0; i < n; ++i) {
+            host[i] = b.ctxs[i]->pk->host[sig[i]];
+            host[i].seg = b.seg_of[i];
+            host[i].nseg = (host[i].NTY + b.seg_of[i] - 1) / b.seg_of[i];
+        }
+        g.table = static_cast<PkProb *>(batch_alloc(b, (size_t)n * sizeof(PkProb)));
+        TRT_CHECK(hipMemcpyAsync(g.table, host.data(), (size_t)n * sizeof(PkProb), hipMemcpyHostToDevice, b.stream));
+        TRT_CHECK(hipStreamSynchronize(b.stream));
+        TRT_CHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+        TRT_CHECK(hipEventRecord(b.fork, b.stream));
+        for (int i = 0; i < n; ++i) {
+            Ctx &c = *b.ctxs[i];
+            TRT_CHECK(hipStreamWaitEvent(c.stream, b.fork, 0));
+            material_update(c);
+            TRT_CHECK(hipEventRecord(b.joined[i], c.stream));
+            TRT_CHECK(hipStreamWaitEvent(b.stream, b.joined[i], 0));
+        }
+        batch_enqueue(b, g.table);
+        TRT_CHECK(hipEventRecord(b.fork2, b.stream));
+        for (int i = 0; i < n; ++i) {
+            Ctx &c = *b.ctxs[i];
+            TRT_CHECK(hipStreamWaitEvent(c.stream, b.fork2, 0));
+            tgen_post(c);
+            TRT_CHECK(hipEventRecord(b.joined[i], c.stream));
+            TRT_CHECK(hipStreamWaitEvent(b.stream, b.joined[i], 0));
+        }
+        TRT_CHECK(hipStreamEndCapture(b.stream, &g.graph));
+        TRT_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        b.step_graphs.push_back(g);
+        sg = &b.step_graphs.back();
+    }
+    for (Ctx *c : b.ctxs) {        // the host side of the step (tgen_pre without its kernel)
+        c->time += c->p.dt;
+        c->iold = c->icur;
+    }
+    TRT_CHECK(hipEventRecord(b.t0, b.stream));
+    TRT_CHECK(hipGraphLaunch(sg->exec, b.stream));
+    TRT_CHECK(hipEventRecord(b.t1, b.stream));
+    TRT_CHECK(hipGetLastError());
+    TRT_CHECK(hipEventRecord(b.done, b.stream));
+    for (int i = 0; i < n; ++i) {
+        Ctx &c = *b.ctxs[i];
+        TRT_CHECK(hipStreamWaitEvent(c.stream, b.done, 0));
+        c.icur = 1 - c.iold;
+        c.sweeps += 1;
+    }
+}
 
 // waits for the batch stream; throws if a dependency wait of the last sweeps timed out
 void pk_batch_check(PkBatch &b)

@@ -154,6 +154,7 @@ struct Ctx {
     // the context frees any still open when it is destroyed
     std::vector<trt_pod *> pods;
     std::vector<trt_dmd *> dmds;
+    std::vector<trt_romgrid *> romgrids;   // data grids of the ROM runs (csrc/romgrid.hip)
 
     double *arr(const char *name);
     DevArray &reg(const std::string &name, int d0, int d1 = 1, int d2 = 1, int d3 = 1);
@@ -219,6 +220,8 @@ void restart_in(Ctx &c, const char *path);
 void bcg_split_if(Ctx &c, const std::string &name);
 // frees the POD/DMD bases still registered on the context (csrc/rom.hip)
 void rom_release_all(Ctx &c);
+// frees the ROM data grids still registered on the context (csrc/romgrid.hip)
+void romgrid_release_all(Ctx &c);
 void sweep_setup(Ctx &c);
 // persistent sweep (sweep_pk.hip)
 bool pk_supported(const Ctx &c);
@@ -234,7 +237,10 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled = false, in
 int pk_batch_seg(const PkBatch &b);
 void pk_batch_destroy(PkBatch *b);
 void pk_batch_sweep(PkBatch &b, int mark);
-void pk_batch_use_graph(PkBatch &b, bool on);
+// mode 0: eager; 1: the batched sweep as a graph; 2: whole Tgen_QDf steps as graphs
+void pk_batch_use_graph(PkBatch &b, int mode);
+bool pk_batch_step_graphs(const PkBatch &b);
+void pk_batch_tgen_step_graph(PkBatch &b);
 void pk_batch_check(PkBatch &b);
 double pk_batch_elapsed_ms(PkBatch &b);
 void pk_batch_info(const PkBatch &b, long long *updates, int *grid, int *nitems);

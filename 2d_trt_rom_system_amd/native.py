@@ -56,7 +56,8 @@ EXPORTS = [
     "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
     "trt_convergence_noalloc", "trt_meb_solve", "trt_outfile_create", "trt_timestep_outs", "trt_tgen_outs",
     "trt_outfile_put", "trt_outfile_close", "trt_outfile_last_error", "trt_nc_open", "trt_nc_dim", "trt_nc_get",
-    "trt_nc_last_error", "trt_nc_close",
+    "trt_nc_last_error", "trt_nc_close", "trt_romgrid_create", "trt_romgrid_tmap", "trt_romgrid_map",
+    "trt_romgrid_maps", "trt_romgrid_destroy",
 ]
 
 _lib = None
@@ -178,6 +179,12 @@ def load(path: str = LIBPATH):
     L.trt_nc_last_error.argtypes = [vp]
     L.trt_nc_last_error.restype = C.c_char_p
     L.trt_nc_close.argtypes = [vp]
+    L.trt_romgrid_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                     C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.trt_romgrid_tmap.argtypes = [vp, C.c_int, ip, ip, _dp, _dp, _dp]
+    L.trt_romgrid_map.argtypes = [vp, C.c_int]
+    L.trt_romgrid_maps.argtypes = [vp, C.c_int, ip, ip, _dp, _dp]
+    L.trt_romgrid_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -403,7 +410,7 @@ class Batch:
             raise TrtError(f"trt_batch_create failed ({rc}): {self.L.trt_batch_last_error(None).decode()}")
         self.h = h
         if graph:
-            self._check(self.L.trt_batch_use_graph(self.h, 1), "batch_use_graph")
+            self._check(self.L.trt_batch_use_graph(self.h, int(graph)), "batch_use_graph")
 
     def close(self):
         if getattr(self, "h", None):
@@ -707,6 +714,58 @@ class NcReader:
         if getattr(self, "h", None):
             self.L.trt_nc_close(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class RomGrid:
+    """The ROM runs' data grid on a context (trt_romgrid_*): GENERATE_GRIDS_XYT and the
+    map of data-grid fields (dat_*) onto the simulation grid's *_next arrays
+    (MainSrc/GRID_FUNCTIONS.f08, POD_RECONSTRUCT_fg / POD_RECONSTRUCT_BCg)."""
+
+    LAYOUT_POINTS = ("cells", "V faces", "H faces", "left", "bottom", "right", "top")
+
+    def __init__(self, ctx: "Context", delx_d, dely_d, xlen_d: float, ylen_d: float, dnt: int, delt_d: float,
+                 start_time_d: float, nt: int, start_time: float, xlen: float, ylen: float):
+        self.L = load()
+        self.ctx = ctx
+        dx = np.ascontiguousarray(np.asarray(delx_d, dtype=np.float64))
+        dy = np.ascontiguousarray(np.asarray(dely_d, dtype=np.float64))
+        h = C.c_void_p()
+        ctx._check(self.L.trt_romgrid_create(ctx.h, C.byref(h), len(dx), len(dy), dx.ctypes.data_as(_dp),
+                                             dy.ctypes.data_as(_dp), float(xlen_d), float(ylen_d), int(dnt),
+                                             float(delt_d), float(start_time_d), int(nt), float(start_time),
+                                             float(xlen), float(ylen)), "romgrid_create")
+        self.h = h
+        self.dnx, self.dny = len(dx), len(dy)
+        ctx._bases.add(self)
+
+    def tmap(self, t: int):
+        """(t1, t2, x, x1, x2) of simulation time t (1-based) for PodBasis.reconstruct."""
+        t1, t2 = C.c_int(), C.c_int()
+        x, x1, x2 = C.c_double(), C.c_double(), C.c_double()
+        self.ctx._check(self.L.trt_romgrid_tmap(self.h, int(t), C.byref(t1), C.byref(t2), C.byref(x), C.byref(x1),
+                                                C.byref(x2)), "romgrid_tmap")
+        return t1.value, t2.value, x.value, x1.value, x2.value
+
+    def map(self, with_bcg: bool = True) -> None:
+        self.ctx._check(self.L.trt_romgrid_map(self.h, int(bool(with_bcg))), "romgrid_map")
+
+    def maps(self, layout: int, nsim: int, ndat: int):
+        gm = np.empty(4 * nsim, dtype=np.int32)
+        vm = np.empty(4 * nsim, dtype=np.int32)
+        sg = np.empty(2 * nsim)
+        dg = np.empty(2 * ndat)
+        ip_ = C.POINTER(C.c_int)
+        self.L.trt_romgrid_maps(self.h, int(layout), gm.ctypes.data_as(ip_), vm.ctypes.data_as(ip_),
+                                sg.ctypes.data_as(_dp), dg.ctypes.data_as(_dp))
+        return gm, vm, sg, dg
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.L.trt_romgrid_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         self.close()

@@ -164,7 +164,8 @@ int trt_step_given_qdf(trt_ctx *ctx, int its_out[4]);
  * rank(g) columns of size/N_g values, or rank(1) columns of the whole size for gsum 1),
  * V (right vectors: per group nt x rank(g), t-major), rank (N_g values, or 1 for gsum 1)
  * -- for reconstruction into the context array `name` (normally a *_next array of
- * trt_step_given_qdf).  trt_pod_reconstruct writes the field at data time t1 (1-based),
+ * trt_step_given_qdf, or a dat_* array of a trt_romgrid for data on another grid).
+ * trt_pod_reconstruct writes the field at data time t1 (1-based),
  * or, when t2 != t1, interpolated linearly to time x between data times x1 (t1) and x2
  * (t2).  The basis is device memory of its own; destroy it before its context.  Errors:
  * trt_last_error(ctx). */
@@ -193,6 +194,37 @@ int trt_dmd_create(trt_ctx *ctx, trt_dmd **out, const char *name, int gsum, cons
 int trt_dmd_reconstruct(trt_dmd *dmd, double time);
 int trt_dmd_destroy(trt_dmd *dmd);
 double trt_time(const trt_ctx *ctx);
+
+/* The ROM runs' data grid: GENERATE_GRIDS_XYT (MainSrc/GRID_FUNCTIONS.f08:661-817) for the
+ * context's simulation grid and a data grid of dnx x dny cells (delx_d, dely_d; xlen_d,
+ * ylen_d) with dnt data times (delt_d after start_time_d), and nt simulation times (dt
+ * after start_time); xlen, ylen are the simulation domain's sizes as the reference passes
+ * them.  The data grid holds N_g groups, as the simulation.  Creation registers the
+ * data-grid arrays dat_fg_avg_xx, dat_fg_avg_yy (dnx,dny,N_g), dat_fg_edgV_xx,
+ * dat_fg_edgV_xy (dnx+1,dny,N_g), dat_fg_edgH_yy, dat_fg_edgH_xy (dnx,dny+1,N_g) and dat_BCg
+ * (2(dnx+dny),N_g) on the context -- POD and DMD bases reconstruct into them -- and computes
+ * the point maps (MAP_GRIDS, :222-327) on the GPU; a simulation point outside the data
+ * domain is an error.  trt_romgrid_tmap gives, for simulation time t (1-based), the data
+ * times the reconstruction uses and the interpolation times (TMap, :754-817, as
+ * POD_RECONSTRUCT_fg passes them to POD_RECONSTRUCT and LINEAR_INTERPOLATION,
+ * POD_ROUTINES.f08:66-73): call trt_pod_reconstruct(b, t1, t2, x, x1, x2) with them.
+ * trt_romgrid_map appends the boundary values the reference appends (EXTND_BT_BND,
+ * EXTND_LR_BND, AVG_APND_BND with COLLECT_LR_BND / COLLECT_BT_BND / COLLECT_CRN_BND,
+ * GRID_FUNCTIONS.f08:406-659, combined as POD_RECONSTRUCT_fg does, POD_ROUTINES.f08:81-232)
+ * and maps the data-grid fields onto the simulation grid (FMAP, :14-43) into fg_*_next and,
+ * with with_bcg = 1, Cg_*_next from dat_BCg (POD_RECONSTRUCT_BCg, :253-353; needs
+ * N_x = N_y and dnx = dny, as the reference's side maps do).  trt_romgrid_maps copies
+ * layout's maps and point grids to the host (layouts 0 cells, 1 V faces, 2 H faces, 3..6
+ * left, bottom, right, top sides; 4 ints per simulation point).  The context frees a data
+ * grid still open when it is destroyed.  Errors: trt_last_error(ctx). */
+typedef struct trt_romgrid trt_romgrid;
+int trt_romgrid_create(trt_ctx *ctx, trt_romgrid **out, int dnx, int dny, const double *delx_d, const double *dely_d,
+                       double xlen_d, double ylen_d, int dnt, double delt_d, double start_time_d, int nt,
+                       double start_time, double xlen, double ylen);
+int trt_romgrid_tmap(const trt_romgrid *r, int t, int *t1, int *t2, double *x, double *x1, double *x2);
+int trt_romgrid_map(trt_romgrid *r, int with_bcg);
+int trt_romgrid_maps(const trt_romgrid *r, int layout, int *gmap, int *vmap, double *sim_grid, double *dat_grid);
+int trt_romgrid_destroy(trt_romgrid *r);
 
 /* Checkpoint and resume.  trt_restart_out ≙ RESTART_OUT, MainSrc/OUTPUTS.f08:1624-1674
  * (called every restart_freq-th step, ALGORITHMS.f08:889-894); trt_restart_in ≙
@@ -329,9 +361,14 @@ const char *trt_batch_last_error(const trt_batch *b);
  * member); ms receives their time on the batch stream */
 int trt_batch_sweep_timed(trt_batch *b, int nsweeps, double *ms);
 /* one Tgen_QDf step of every member, the sweeps in one launch; sweep_ms (may
- * be NULL) receives the batched sweep's time */
+ * be NULL) receives the batched sweep's time (with whole-step graphs: the step's) */
 int trt_batch_tgen_step(trt_batch *b, double *sweep_ms);
-/* on = 1: the batched sweep is captured once in a HIP graph and replayed */
+/* on = 1: the batched sweep is captured once in a HIP graph and replayed; on = 2: the
+ * whole batched Tgen_QDf step -- every member's MATERIAL_UPDATE on its stream, the
+ * persistent sweep, every member's COLLAPSE_INTENSITIES + fg_Calc + Cg_Calc -- is one graph
+ * launch, captured once per combination of the members' I_crn/I_crn_old buffers (two
+ * for a batch stepping together) and replayed; 0 = eager.  Results are bitwise the same
+ * in every mode. */
 int trt_batch_use_graph(trt_batch *b, int on);
 int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems);
 

@@ -149,6 +149,39 @@ def test_batch_tgen_equals_single(trt, graph):
         c.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_whole_step_graphs_equal_eager_steps(trt, precision):
+    """Batched Tgen_QDf steps replayed from whole-step graphs (every member's
+    MATERIAL_UPDATE, the persistent sweep, every member's moments, QD factors and C_g in
+    one graph launch; one graph per I_crn / I_crn_old parity) equal eager batched steps
+    bitwise at every step, with host work (a new prescribed Temp) between the replays."""
+    probs = _probs(trt)
+    tile = (4, 2) if precision else (2, 4)
+    eager = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]
+    graph = [trt.native.Context(p, precision=precision, tile=tile) for p in probs]
+    temps = [np.random.default_rng(40 + k).uniform(200.0, 1200.0, (p.nx, p.ny)) for k, p in enumerate(probs)]
+    for k in range(len(probs)):
+        eager[k].push("Temp", temps[k])
+        graph[k].push("Temp", temps[k])
+    be = trt.native.Batch(eager)
+    bg = trt.native.Batch(graph, graph=2)
+    for s in range(5):
+        be.tgen_step()
+        bg.tgen_step()
+        for e, g in zip(eager, graph):
+            assert e.time == g.time
+            for n in TGEN_OUT + ["I_crn", "Temp"]:
+                assert np.array_equal(e.pull(n), g.pull(n)), (s, n)
+        for k, (e, g) in enumerate(zip(eager, graph)):
+            t = temps[k] * (1.0 + 0.05 * (s + 1))
+            e.push("Temp", t)
+            g.push("Temp", t)
+    be.close()
+    bg.close()
+    for c in eager + graph:
+        c.close()
+
+
 def test_batch_rejects_mixed_tiles(trt):
     probs = _probs(trt)[:2]
     c1 = trt.native.Context(probs[0], precision=1, tile=(4, 2))
