@@ -758,6 +758,49 @@ int trt_nc_get(trt_ncread *r, const char *var, int ndims, const size_t *start, c
     return 0;
 }
 
+int trt_nc_att(trt_ncread *r, const char *var, const char *name, double *vals, size_t cap, char *text, size_t tcap,
+               size_t *n)
+{
+    if (!r || !name || !n) return -1;
+    const nc::Header &h = r->r.header();
+    const std::vector<nc::Att> *list = &h.gatts;
+    if (var) {
+        const int id = h.find_var(var);
+        if (id < 0) {
+            r->err = std::string("no variable ") + var;
+            return -2;
+        }
+        list = &h.vars[id].atts;
+    }
+    for (const nc::Att &a : *list) {
+        if (a.name != name) continue;
+        *n = a.n;
+        if (a.type == nc::NC_CHAR) {
+            if (text && tcap) {
+                const size_t k = std::min(a.n, tcap - 1);
+                memcpy(text, a.raw.data(), k);
+                text[k] = '\0';
+            }
+            return 1;
+        }
+        for (size_t i = 0; i < a.n && vals && i < cap; ++i) {
+            const unsigned char *p = a.raw.data() + i * nc::type_size(a.type);
+            double x;
+            switch (a.type) {
+            case nc::NC_BYTE: x = (double)*(const signed char *)p; break;
+            case nc::NC_SHORT: { int16_t y; memcpy(&y, p, 2); x = y; break; }
+            case nc::NC_INT: { int32_t y; memcpy(&y, p, 4); x = y; break; }
+            case nc::NC_FLOAT: { float y; memcpy(&y, p, 4); x = y; break; }
+            default: memcpy(&x, p, 8);
+            }
+            vals[i] = x;
+        }
+        return 0;
+    }
+    r->err = std::string("no attribute ") + name;
+    return -2;
+}
+
 const char *trt_nc_last_error(const trt_ncread *r) { return r ? r->err.c_str() : trt_last_error(nullptr); }
 
 int trt_nc_close(trt_ncread *r)

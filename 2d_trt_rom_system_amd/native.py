@@ -56,7 +56,7 @@ EXPORTS = [
     "trt_pod_timed", "trt_pod_basis_bytes", "trt_dmd_create", "trt_dmd_reconstruct", "trt_dmd_destroy",
     "trt_convergence_noalloc", "trt_meb_solve", "trt_outfile_create", "trt_timestep_outs", "trt_tgen_outs",
     "trt_outfile_put", "trt_outfile_close", "trt_outfile_last_error", "trt_nc_open", "trt_nc_dim", "trt_nc_get",
-    "trt_nc_last_error", "trt_nc_close", "trt_romgrid_create", "trt_romgrid_tmap", "trt_romgrid_map",
+    "trt_nc_last_error", "trt_nc_close", "trt_nc_att", "trt_romgrid_create", "trt_romgrid_tmap", "trt_romgrid_map",
     "trt_romgrid_maps", "trt_romgrid_destroy",
 ]
 
@@ -179,6 +179,8 @@ def load(path: str = LIBPATH):
     L.trt_nc_last_error.argtypes = [vp]
     L.trt_nc_last_error.restype = C.c_char_p
     L.trt_nc_close.argtypes = [vp]
+    L.trt_nc_att.argtypes = [vp, C.c_char_p, C.c_char_p, _dp, C.c_size_t, C.c_char_p, C.c_size_t,
+                             C.POINTER(C.c_size_t)]
     L.trt_romgrid_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                      C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double]
     L.trt_romgrid_tmap.argtypes = [vp, C.c_int, ip, ip, _dp, _dp, _dp]
@@ -709,6 +711,18 @@ class NcReader:
         if self.L.trt_nc_get(self.h, var.encode(), nd, st, ct, out.ctypes.data_as(_dp)) != 0:
             raise TrtError(self.L.trt_nc_last_error(self.h).decode())
         return out.reshape(tuple(count), order="F") if nd else out[0]
+
+    def att(self, name: str, var: str | None = None):
+        """An attribute (nf90_get_att): text as str, numbers as an FP64 array."""
+        n = C.c_size_t()
+        self.L.trt_nc_att(self.h, var.encode() if var else None, name.encode(), None, 0, None, 0, C.byref(n))
+        vals = np.empty(max(int(n.value), 1))
+        txt = C.create_string_buffer(int(n.value) + 1)
+        rc = self.L.trt_nc_att(self.h, var.encode() if var else None, name.encode(), vals.ctypes.data_as(_dp),
+                               vals.size, txt, len(txt), C.byref(n))
+        if rc < 0:
+            raise TrtError(self.L.trt_nc_last_error(self.h).decode())
+        return txt.value.decode() if rc == 1 else vals[:int(n.value)]
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -315,6 +315,11 @@ typedef struct trt_ncread trt_ncread;
 int trt_nc_open(const char *path, trt_ncread **out);
 int trt_nc_dim(trt_ncread *r, const char *name, size_t *len);
 int trt_nc_get(trt_ncread *r, const char *var, int ndims, const size_t *start, const size_t *count, double *out);
+/* attribute name of variable var (NULL: a global attribute), as nf90_get_att reads it:
+ * numbers (up to cap, as FP64) into vals and 0 returned, or text (NUL-terminated, at most
+ * tcap - 1 characters) into text and 1 returned; *n receives the attribute's length */
+int trt_nc_att(trt_ncread *r, const char *var, const char *name, double *vals, size_t cap, char *text, size_t tcap,
+               size_t *n);
 const char *trt_nc_last_error(const trt_ncread *r);
 int trt_nc_close(trt_ncread *r);
 

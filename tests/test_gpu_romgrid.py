@@ -170,3 +170,41 @@ def test_simulation_grid_outside_the_data_grid_is_an_error(trt):
         with pytest.raises(native.TrtError, match="outside the data grid"):
             native.RomGrid(ctx, np.full(6, 0.1), np.full(6, 0.1), 0.6, 0.6, 2, prob.dt, 0.0, 2, 0.0, prob.xlen,
                            prob.ylen)
+
+
+def test_database_to_simulation_grid(trt, tmp_path):
+    """A POD database file (classic netCDF, the decomposer's layout) read by rom_db, its bases
+    reconstructed on the GPU into the data grid at the data times TMap picks, and mapped onto
+    the simulation grid: equal to the same chain in numpy (oracle.rom + oracle.romgrid)."""
+    import importlib
+    from test_rom_db import _write_database
+    native = trt.native
+    rom_db = importlib.import_module("2d_trt_rom_system_amd.rom_db")
+    dn, nt, ng = 6, 4, 2
+    prob = trt.problem.small(nx=9, ny=9, order=4, ng=ng)
+    rng = np.random.default_rng(21)
+    snaps = {n: 0.3 + 0.01 * rng.standard_normal((rom_db.field_length(n, dn, dn) * ng, nt)) for n in rom_db.FIELDS}
+    path = str(tmp_path / "db.nc")
+    _write_database(path, snaps, dn, dn, ng, nt, 0, delt=0.7 * prob.dt, t0=0.0)
+    db = rom_db.read_pod_database(path, eps=1e-10)
+    with native.Context(prob) as ctx:
+        rg = native.RomGrid(ctx, db.delx, db.dely, db.xlen, db.ylen, db.nt, db.delt, db.start_time, 3, 0.0,
+                            prob.xlen, prob.ylen)
+        bases = db.create_bases(ctx)
+        G = R.Grids(db.delx, db.dely, prob.dx, prob.dy, db.xlen, db.ylen, prob.xlen, prob.ylen)
+        for t in (1, 2, 3):
+            t1, t2, x, x1, x2 = rg.tmap(t)
+            dat_ref = {}
+            for name, b in bases.items():
+                b.reconstruct(t1, t2, x, x1, x2)
+                C, S, U, V, rank = db.bases[name]
+                y1 = rom.pod_reconstruct(C, S, U, V, rank, len(C), ng, nt, t1, 0)
+                y2 = rom.pod_reconstruct(C, S, U, V, rank, len(C), ng, nt, t2, 0)
+                dat_ref[name] = y1 if t1 == t2 else y1 + (x - x1) * (y2 - y1) / (x2 - x1)
+            rg.map(with_bcg=True)
+            ref = R.map_fields(G, dat_ref, ng)
+            for k in FIELDS + ("Cg_L", "Cg_B", "Cg_R", "Cg_T"):
+                assert _rel(ctx.pull(k + "_next"), ref[k]) < 1e-13, (t, k)
+        for b in bases.values():
+            b.close()
+        rg.close()
