@@ -41,8 +41,15 @@ void validate(const trt_problem *p)
     if (p->nmat < 1) throw Error("need at least one material");
     if (!p->dx || !p->dy || !p->ox || !p->oy || !p->w || !p->nu || !p->mat || !p->kmult)
         throw Error("null problem vector");
+    // BC_Type 2 passes the reference's input check (INPUTS.f08:1024-1027) but no routine
+    // gives it a boundary condition: RT_BC_UPDATE, Cg_Calc and MGQD_INIT branch on 0 and 1
     for (int k = 0; k < 4; ++k)
-        if (p->bc_type[k] != 0) throw Error("only incoming-Planckian boundaries (BC_Type 0) are supported");
+        if (p->bc_type[k] != 0 && p->bc_type[k] != 1)
+            throw Error("BC_Type must be 0 (incoming Planckian) or 1 (reflective); the reference defines no other");
+    if (p->sweep_mode == 1)
+        for (int k = 0; k < 4; ++k)
+            if (p->bc_type[k] == 1)
+                throw Error("reflective boundaries (BC_Type 1) run on the tile engine (sweep_mode 0, 2, 3 or 4)");
     if (p->precision < 0 || p->precision > 2)
         throw Error("precision must be 0 (FP64), 1 (FP32 sweep) or 2 (FP32 storage, FP64 arithmetic)");
     if (p->precision == 2 && p->sweep_mode == 1)
@@ -243,6 +250,11 @@ int trt_transport_scb(trt_ctx *ctx)
         c.sweep_ms += ms;
         trt::pk_check(c);
     });
+}
+
+int trt_rt_bc_update(trt_ctx *ctx)
+{
+    return guarded(ctx, [&](Ctx &c) { trt::rt_bc_update(c); TRT_CHECK(hipStreamSynchronize(c.stream)); });
 }
 
 int trt_mloqd_fv(trt_ctx *ctx, int *kits)

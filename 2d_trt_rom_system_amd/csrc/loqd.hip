@@ -1492,6 +1492,11 @@ int step(Ctx &c, int its_out[4], int ho_form)
             transport_sweep(c);              // AL:541-542
             sweeps++;
             collapse_with_fg(c);             // AL:551-553 and fg_Calc AL:568-569
+            if (c.refl) {                    // AL:556-565 (the moments above already hold the
+                rt_bc_update(c);             // sweep's inflow; the update feeds the next sweep)
+                mgqd_in_calc(c);
+                if (!c.p.no_grey) gqd_in_calc(c);
+            }
             cg_calc(c);                      // AL:572
         } else if (ho_form) {
             for (auto n : {"fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy", "Cg_L",

@@ -330,6 +330,9 @@ void restart_in(Ctx &c, const char *path)
         else TRT_CHECK(hipMemcpyAsync(c.arr(name), vals[k].data(), vals[k].size() * 8, hipMemcpyHostToDevice, c.stream));
     }
     rebuild_sc(c);   // the sweep's per-(cell, group) (KapE + 1/(c dt), RT_Src) pairs
+    // the reference restarts on a freshly initialised state: the inflow of reflective
+    // sides is RT_INIT's Planckian (RESTART_IN reads no boundary intensities, AL:234-236)
+    reflect_reset(c);
     TRT_CHECK(hipStreamSynchronize(c.stream));
     c.time = time;
 }

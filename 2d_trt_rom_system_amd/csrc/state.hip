@@ -496,6 +496,9 @@ void init_state(Ctx &c)
     hipLaunchKernelGGL(bg_init_kernel, dim3((ng + 255) / 256), dim3(256), 0, c.stream, bg, c.nu, c.p.tini, ng, cu);
     c.h_bcin.resize(4 * (size_t)ng);
     TRT_CHECK(hipMemcpyAsync(c.h_bcin.data(), c.bcin, 4 * ng * 8, hipMemcpyDeviceToHost, c.stream));
+    // RT_INIT sets the inflow of every side, reflective ones included, to Bg_planck(bcT)
+    // (RT_BC_UPDATE with BC_Type (0,0,0,0), INITIALIZERS.f08:106)
+    reflect_reset(c);
 
     // RT_INIT, IN:81-107: all intensities at bg(Tini); boundary values are the bcin array
     const size_t nrec = na * 4 * (size_t)ng * c.sg.K * 4;

@@ -306,6 +306,10 @@ size_t sweep_bytes(const trt_problem &p, SweepGeom *out)
     size_t b = 2 * nc * 4 * NL * 4 * ti;                                   // I_crn, I_crn_old
     b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * sg.NTY * sg.TY * NL * ti;   // XV (whole tiles)
     b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * sg.NTX * sg.TX * NL * ti;   // XH
+    bool refl = false;
+    for (int k = 0; k < 4; ++k) refl = refl || p.bc_type[k] == 1;
+    if (refl)                                                              // boundary slices
+        b += 2 * 4 * ((size_t)sg.NTY * 2 * sg.TY + (size_t)sg.NTX * 2 * sg.TX) * NL * ti;
     b += (size_t)p.nm * 6 * 8;                                             // moment weight table
     b += nc * p.ng * 2 * 8;                                                // sc
     if (p.precision) b += nc * p.ng * 2 * 4;                               // FP32 copy of sc
@@ -333,6 +337,15 @@ void sweep_setup(Ctx &c)
     // covers the per-row layout of sweep_mode 1
     c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * sg.NTY * sg.TY * NL * ti));
     c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * sg.NTX * sg.TX * NL * ti));
+    c.refl = 0;
+    for (int k = 0; k < 4; ++k)
+        if (c.p.bc_type[k] == 1) c.refl |= 1 << k;
+    if (c.refl && c.sweep_mode == 1)
+        throw Error("reflective boundaries (BC_Type 1) run on the tile engine (sweep_mode 0, 2, 3 or 4)");
+    if (c.refl) {
+        c.BV = c.dalloc(2 * 4 * (size_t)sg.NTY * NL * 2 * sg.TY * ti);
+        c.BH = c.dalloc(2 * 4 * (size_t)sg.NTX * NL * 2 * sg.TX * ti);
+    }
     c.sc = static_cast<double *>(c.dalloc(nc * c.ng * 2 * 8));
     if (c.precision) c.scf = static_cast<float *>(c.dalloc(nc * c.ng * 2 * 4));
     c.Pavg = static_cast<double *>(c.dalloc(nc * 4 * c.ng * 4 * 8));
@@ -465,10 +478,18 @@ void push_icrn(Ctx &c, int which, const double *host)
 // Reference-shaped views derived from I_crn (pull only): I_avg(N_x,N_y,N_m,N_g),
 // I_edgV(N_x+1,N_y,..), I_edgH(N_x,N_y+1,..), Ic_edgV(N_x+1,2N_y,..), Ic_edgH(2N_x,N_y+1,..).
 // Interior and outgoing faces hold the upwind corner values (TRANSPORT_SOLVES.f08:86-93),
-// incoming boundary faces the boundary intensity (RT_BC_UPDATE, UPDATES.f08:26-102).
+// incoming boundary faces the boundary intensity (RT_BC_UPDATE, UPDATES.f08:26-102): the
+// Planckian of the side, or on a reflective side the inflow slice (after RT_BC_UPDATE, the
+// mirror direction's outflow of the last sweep).
 // ---------------------------------------------------------------------------
+struct InflowSlices {
+    const void *BV, *BH;
+    int refl, TX, TY, NTX, NTY;
+};
+
 template <typename TI>
-__global__ void derived_kernel(const TI *I, const double *bcin, double *out, int kind, int nx, int ny, int nm, int ng)
+__global__ void derived_kernel(const TI *I, const double *bcin, InflowSlices bs, double *out, int kind, int nx, int ny,
+                               int nm, int ng)
 {
     const int K = nm / 4;
     const size_t NL = (size_t)ng * K;
@@ -497,9 +518,22 @@ __global__ void derived_kernel(const TI *I, const double *bcin, double *out, int
         } else if (kind == 1 || kind == 3) {          // vertical faces
             const int f = a;
             const int j = kind == 1 ? b : b / 2, top = kind == 3 ? (b & 1) : -1;
-            if (px && f == 0) v = bcin[0 * ng + g];
-            else if (!px && f == nx) v = bcin[2 * ng + g];
-            else {
+            if ((px && f == 0) || (!px && f == nx)) {
+                const int side = px ? 0 : 2;
+                if (bs.refl & (1 << side)) {
+                    const int jf = py ? j : ny - 1 - j, tile = jf / bs.TY, r = jf - tile * bs.TY;
+                    const TI *sl = static_cast<const TI *>(bs.BV) + (size_t)((px ? 0 : 1) * 4 + q) * bs.NTY * NL * 2 * bs.TY;
+                    const int lane = g * K + k;
+                    const double sc = 1.0 / iscale<TI>();
+                    // frame-bottom and frame-top corners; the frame is flipped in y when !py
+                    const double fb = (double)sl[rec_elem<TI>(tile, 2 * r, 2 * bs.TY, (int)NL, lane)] * sc;
+                    const double ft = (double)sl[rec_elem<TI>(tile, 2 * r + 1, 2 * bs.TY, (int)NL, lane)] * sc;
+                    const double lo = py ? fb : ft, hi = py ? ft : fb;
+                    v = top < 0 ? (lo + hi) * 0.5 : (top ? hi : lo);
+                } else {
+                    v = bcin[side * ng + g];
+                }
+            } else {
                 const TI *p = px ? rec(f - 1, j) : rec(f, j);
                 const double sc = 1.0 / iscale<TI>();
                 const double lo = (px ? (double)p[1] : (double)p[0]) * sc, hi = (px ? (double)p[2] : (double)p[3]) * sc;
@@ -508,9 +542,22 @@ __global__ void derived_kernel(const TI *I, const double *bcin, double *out, int
         } else {                                      // horizontal faces
             const int f = b;
             const int i = kind == 2 ? a : a / 2, right = kind == 4 ? (a & 1) : -1;
-            if (py && f == 0) v = bcin[1 * ng + g];
-            else if (!py && f == ny) v = bcin[3 * ng + g];
-            else {
+            if ((py && f == 0) || (!py && f == ny)) {
+                const int side = py ? 1 : 3;
+                if (bs.refl & (1 << side)) {
+                    const int xf = px ? i : nx - 1 - i, tile = xf / bs.TX, ii = xf - tile * bs.TX;
+                    const TI *sl = static_cast<const TI *>(bs.BH) + (size_t)((py ? 0 : 1) * 4 + q) * bs.NTX * NL * 2 * bs.TX;
+                    const int lane = g * K + k;
+                    const double sc = 1.0 / iscale<TI>();
+                    // frame-left and frame-right corners; the frame is flipped in x when !px
+                    const double fl = (double)sl[rec_elem<TI>(tile, 2 * ii, 2 * bs.TX, (int)NL, lane)] * sc;
+                    const double fr = (double)sl[rec_elem<TI>(tile, 2 * ii + 1, 2 * bs.TX, (int)NL, lane)] * sc;
+                    const double lft = px ? fl : fr, rgt = px ? fr : fl;
+                    v = right < 0 ? (lft + rgt) * 0.5 : (right ? rgt : lft);
+                } else {
+                    v = bcin[side * ng + g];
+                }
+            } else {
                 const TI *p = py ? rec(i, f - 1) : rec(i, f);
                 const double sc = 1.0 / iscale<TI>();
                 const double lft = (py ? (double)p[3] : (double)p[0]) * sc, rgt = (py ? (double)p[2] : (double)p[1]) * sc;
@@ -540,12 +587,13 @@ void pull_derived(Ctx &c, const char *name, double *host)
     if (kind < 0) throw Error(std::string("not a derived intensity view: ") + name);
     const size_t n = derived_size(c, name);
     double *tmp = static_cast<double *>(c.dalloc(n * 8));
+    const InflowSlices bs{c.BV, c.BH, c.refl, c.sg.TX, c.sg.TY, c.sg.NTX, c.sg.NTY};
     if (c.precision)
         hipLaunchKernelGGL(derived_kernel<float>, dim3(1024), dim3(256), 0, c.stream,
-                           static_cast<const float *>(c.Icrn[c.icur]), c.bcin, tmp, kind, c.nx, c.ny, c.nm, c.ng);
+                           static_cast<const float *>(c.Icrn[c.icur]), c.bcin, bs, tmp, kind, c.nx, c.ny, c.nm, c.ng);
     else
         hipLaunchKernelGGL(derived_kernel<double>, dim3(1024), dim3(256), 0, c.stream,
-                           static_cast<const double *>(c.Icrn[c.icur]), c.bcin, tmp, kind, c.nx, c.ny, c.nm, c.ng);
+                           static_cast<const double *>(c.Icrn[c.icur]), c.bcin, bs, tmp, kind, c.nx, c.ny, c.nm, c.ng);
     TRT_CHECK(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, c.stream));
     TRT_CHECK(hipStreamSynchronize(c.stream));
     c.dfree_last();

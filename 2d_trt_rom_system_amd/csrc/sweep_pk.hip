@@ -67,6 +67,9 @@ struct PkProb {
     void *XV, *XH;
     double *Pavg, *PV, *PH;
     const double *ctab;    // [m][6]: w, w ox^2, w oy^2, w ox oy, w ox, w oy, over the stored intensities' factor
+    int tx, ty;            // tile shape
+    int refl;              // reflective sides: bit 0 left, 1 bottom, 2 right, 3 top
+    void *BV, *BH;         // their boundary slices: [left, right][q][NTY][NL][2 TY], [bottom, top][q][NTX][NL][2 TX]
 };
 
 struct PkArgs {
@@ -315,6 +318,9 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         if (txl > 0 && act)
             xload_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl - 1) * P.NTY * NL * NV, vslice, (unsigned)tyl,
                               (unsigned)NL, (unsigned)L, pv);
+        else if (act && (P.refl & (px ? 1 : 4)))   // reflective left (right) side: the inflow slice
+            xload_rec<TI, NV>(static_cast<TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * NL * NV, vslice,
+                              (unsigned)tyl, (unsigned)NL, (unsigned)L, pv);
 #pragma unroll
         for (int r = 0; r < TY; ++r) { vbr[r] = pv[2 * r]; vtr[r] = pv[2 * r + 1]; }
         TI ph[NH];
@@ -323,6 +329,9 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         if (tyl > 0 && act && !hin_regs)
             xload_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl - 1) * P.NTX * NL * NH, hslice, (unsigned)txl,
                               (unsigned)NL, (unsigned)L, ph);
+        else if (tyl == 0 && act && (P.refl & (py ? 2 : 8)))   // reflective bottom (top) side
+            xload_rec<TI, NH>(static_cast<TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * NL * NH, hslice,
+                              (unsigned)txl, (unsigned)NL, (unsigned)L, ph);
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             hl[ii] = ph[2 * ii];
@@ -400,19 +409,25 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         vbr[r] = (TI)vb;                 // outgoing V faces of the row
         vtr[r] = (TI)vt;
     }
-    if (txl < P.NTX - 1 && act) {
+    // outgoing faces: to the next tile, or on a reflective side of the domain to its
+    // outflow slice (RT_BC_UPDATE copies it to the mirror quadrant's inflow after the sweep)
+    const bool v_next = txl < P.NTX - 1, v_bnd = !v_next && (P.refl & (px ? 4 : 1));
+    if ((v_next || v_bnd) && act) {
         TI pv[NV];
 #pragma unroll
         for (int r = 0; r < TY; ++r) { pv[2 * r] = vbr[r]; pv[2 * r + 1] = vtr[r]; }
-        xstore_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl) * P.NTY * NL * NV, vslice, (unsigned)tyl, (unsigned)NL,
-                           (unsigned)L, pv);
+        TI *dst = v_next ? XV + (size_t)(q * (P.NTX - 1) + txl) * P.NTY * NL * NV
+                     : static_cast<TI *>(P.BV) + (size_t)((px ? 1 : 0) * 4 + q) * P.NTY * NL * NV;
+        xstore_rec<TI, NV>(dst, vslice, (unsigned)tyl, (unsigned)NL, (unsigned)L, pv);
     }
-    if (tyl < P.NTY - 1 && act && hout_xh) {
+    const bool h_next = tyl < P.NTY - 1 && hout_xh, h_bnd = tyl == P.NTY - 1 && (P.refl & (py ? 8 : 2));
+    if ((h_next || h_bnd) && act) {
         TI ph[NH];
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) { ph[2 * ii] = (TI)hlc[ii]; ph[2 * ii + 1] = (TI)hrc[ii]; }
-        xstore_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl) * P.NTX * NL * NH, hslice, (unsigned)txl, (unsigned)NL,
-                           (unsigned)L, ph);
+        TI *dst = h_next ? XH + (size_t)(q * (P.NTY - 1) + tyl) * P.NTX * NL * NH
+                     : static_cast<TI *>(P.BH) + (size_t)((py ? 1 : 0) * 4 + q) * P.NTX * NL * NH;
+        xstore_rec<TI, NH>(dst, hslice, (unsigned)txl, (unsigned)NL, (unsigned)L, ph);
     }
 #pragma unroll
     for (int ii = 0; ii < TX; ++ii) {
@@ -724,7 +739,17 @@ __global__ void pk_reset(int *queue)
 }
 
 // Incoming boundary faces: their moments are those of the boundary intensities
-// (RT_BC_UPDATE, MainSrc/UPDATES.f08:12-105), the same for every face of a side.
+// (RT_BC_UPDATE, MainSrc/UPDATES.f08:12-105): Bg_planck(bcT), the same for every face
+// of a side, or on a reflective side the face averages of the inflow slice's corner
+// values (I_edgV = (Ic_bottom + Ic_top) / 2 of the mirror direction, TRANSPORT_SOLVES.f08:93).
+template <typename TI>
+__device__ __forceinline__ double inflow_face(const TI *slice, int tile, int t, int N, int NL, int lane)
+{
+    const double a = (double)slice[rec_elem<TI>(tile, t, N, NL, lane)];
+    const double b = (double)slice[rec_elem<TI>(tile, t + 1, N, NL, lane)];
+    return (double)(TI)((a + b) * 0.5) / iscale<TI>();
+}
+
 template <typename TI>
 __global__ void pk_boundary_moments(const PkProb *probs)
 {
@@ -739,10 +764,16 @@ __global__ void pk_boundary_moments(const PkProb *probs)
         const bool px = (q == 0 || q == 3), py = (q == 0 || q == 1);
         if (f < ny) {
             const int j = f;
-            const double b = (double)(TI)(P.bcin[(px ? 0 : 2) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
+            const double b0 = (double)(TI)(P.bcin[(px ? 0 : 2) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
+            const bool rf = P.refl & (px ? 1 : 4);
+            const TI *bs = rf ? static_cast<const TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * ng * K * 2 * P.ty
+                              : nullptr;
+            const int jf = py ? j : ny - 1 - j, tile = jf / P.ty;
             double s[4] = {0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;
+                const double b = rf ? inflow_face(bs, tile, 2 * (jf - tile * P.ty), 2 * P.ty, ng * K, g * K + k) + P.eps
+                                    : b0;
                 const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
                 s[0] += wm * ox * ox * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * b;
             }
@@ -751,10 +782,16 @@ __global__ void pk_boundary_moments(const PkProb *probs)
             o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3];
         } else {
             const int i = f - ny;
-            const double b = (double)(TI)(P.bcin[(py ? 1 : 3) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
+            const double b0 = (double)(TI)(P.bcin[(py ? 1 : 3) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
+            const bool rf = P.refl & (py ? 2 : 8);
+            const TI *bs = rf ? static_cast<const TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * ng * K * 2 * P.tx
+                              : nullptr;
+            const int xf = px ? i : nx - 1 - i, tile = xf / P.tx;
             double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;
+                const double b = rf ? inflow_face(bs, tile, 2 * (xf - tile * P.tx), 2 * P.tx, ng * K, g * K + k) + P.eps
+                                    : b0;
                 const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
                 s[0] += wm * oy * oy * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * oy * b;
                 s[4] += wm * b;
@@ -763,6 +800,59 @@ __global__ void pk_boundary_moments(const PkProb *probs)
             double *o = P.PH + ((fh * 4 + q) * ng + g) * 5;
             o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3]; o[4] = s[4];
         }
+    }
+}
+
+// RT_BC_UPDATE with BC_Type 1 (UPDATES.f08:34-44, 53-63, 72-82, 91-101): on each
+// reflective side the incoming quadrants take the last sweep's outflow of the quadrant
+// mirrored across the side, direction by direction (the same index within the quadrant),
+// face by face and corner by corner.  A quadrant and its mirror across a side share the
+// frame along that side (the same tile rows for a V side, the same tile columns for an H
+// side, the same corner order in a record), so the update copies whole slices.  In the
+// reference's 1-based quadrant numbering: left q1 <- q2, q4 <- q3; bottom q1 <- q4,
+// q2 <- q3; right q2 <- q1, q3 <- q4; top q4 <- q1, q3 <- q2.
+__device__ __forceinline__ void mirror_pair(int side, int h, int &dst, int &src)
+{
+    switch (side) {
+    case 0: dst = h ? 3 : 0; src = h ? 2 : 1; break;
+    case 1: dst = h ? 1 : 0; src = h ? 2 : 3; break;
+    case 2: dst = h ? 2 : 1; src = h ? 3 : 0; break;
+    default: dst = h ? 2 : 3; src = h ? 1 : 0; break;
+    }
+}
+
+// blockIdx.y = 2 side + pair; nv, nh: values per V and per H slice
+template <typename TI>
+__global__ void reflect_kernel(TI *BV, TI *BH, size_t nv, size_t nh, int refl)
+{
+    const int side = blockIdx.y >> 1, h = blockIdx.y & 1;
+    if (!(refl & (1 << side))) return;
+    int dst, src;
+    mirror_pair(side, h, dst, src);
+    const bool vert = side == 0 || side == 2;
+    const size_t n = vert ? nv : nh;
+    TI *base = vert ? BV + (size_t)(side == 0 ? 0 : 4) * nv : BH + (size_t)(side == 1 ? 0 : 4) * nh;
+    const TI *from = base + (size_t)src * n;
+    TI *to = base + (size_t)dst * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        to[e] = from[e];
+}
+
+// Every quadrant's slice of each reflective side (blockIdx.y) at Bg_planck(bcT of the
+// side) of the lane's group, as the tile engine rounds the Planckian inflow.
+template <typename TI>
+__global__ void reflect_reset_kernel(TI *BV, TI *BH, const double *bcin, size_t nv, size_t nh, int NL, int K,
+                                     int ng, int refl)
+{
+    const int side = blockIdx.y;
+    if (!(refl & (1 << side))) return;
+    const bool vert = side == 0 || side == 2;
+    const size_t n = 4 * (vert ? nv : nh);
+    TI *base = vert ? BV + (size_t)(side == 0 ? 0 : 4) * nv : BH + (size_t)(side == 1 ? 0 : 4) * nh;
+    constexpr int E = 16 / (int)sizeof(TI);
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int g = (int)((e / E) % NL) / K;
+        base[e] = (TI)(bcin[side * ng + g] * iscale<TI>());
     }
 }
 
@@ -934,6 +1024,8 @@ static void pk_setup(Ctx &c)
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
     P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
+    P.tx = sg.TX; P.ty = sg.TY;
+    P.refl = c.refl; P.BV = c.BV; P.BH = c.BH;
     pk->lds = (size_t)P.xsw * 8 + stb;
     // The default form (sweep_mode 0) takes column segments of 4 or 2 tiles when their
     // wavefront (items of one diagonal) still offers twice the resident workgroups, as
@@ -987,6 +1079,43 @@ static void boundary_moments(int precision, const PkProb *tab, int nprob, int nb
 static void boundary_moments(Ctx &c, const PkProb *tab)
 {
     boundary_moments(c.precision, tab, 1, 4 * c.ng * (c.nx + c.ny), c.stream);
+}
+
+// values per boundary slice of a V side and of an H side
+static void slice_sizes(const Ctx &c, size_t &nv, size_t &nh)
+{
+    const size_t NL = (size_t)c.ng * c.sg.K;
+    nv = (size_t)c.sg.NTY * NL * 2 * c.sg.TY;
+    nh = (size_t)c.sg.NTX * NL * 2 * c.sg.TX;
+}
+
+void rt_bc_update(Ctx &c)
+{
+    if (!c.refl) return;
+    size_t nv, nh;
+    slice_sizes(c, nv, nh);
+    const dim3 grid((unsigned)std::min<size_t>(1024, (std::max(nv, nh) + 255) / 256), 8);
+    if (c.precision)
+        hipLaunchKernelGGL(reflect_kernel<float>, grid, dim3(256), 0, c.stream, static_cast<float *>(c.BV),
+                           static_cast<float *>(c.BH), nv, nh, c.refl);
+    else
+        hipLaunchKernelGGL(reflect_kernel<double>, grid, dim3(256), 0, c.stream, static_cast<double *>(c.BV),
+                           static_cast<double *>(c.BH), nv, nh, c.refl);
+}
+
+void reflect_reset(Ctx &c)
+{
+    if (!c.refl) return;
+    size_t nv, nh;
+    slice_sizes(c, nv, nh);
+    const dim3 grid((unsigned)std::min<size_t>(1024, (4 * std::max(nv, nh) + 255) / 256), 4);
+    const int NL = c.ng * c.sg.K;
+    if (c.precision)
+        hipLaunchKernelGGL(reflect_reset_kernel<float>, grid, dim3(256), 0, c.stream, static_cast<float *>(c.BV),
+                           static_cast<float *>(c.BH), c.bcin, nv, nh, NL, c.sg.K, c.ng, c.refl);
+    else
+        hipLaunchKernelGGL(reflect_reset_kernel<double>, grid, dim3(256), 0, c.stream, static_cast<double *>(c.BV),
+                           static_cast<double *>(c.BH), c.bcin, nv, nh, NL, c.sg.K, c.ng, c.refl);
 }
 
 // Enqueues one sweep: work-list reset, boundary-face moments, persistent kernel.

@@ -30,6 +30,16 @@ constexpr double kIScale32 = 1099511627776.0;   // 2^40
 template <typename TI>
 __host__ __device__ constexpr double iscale() { return sizeof(TI) == 4 ? kIScale32 : 1.0; }
 
+// Element index of value t of the record of (tile, lane) in a face-exchange or boundary
+// slice whose records hold N values of TI: records are stored in 16-byte chunks,
+// chunk-major ([tile][chunk][lane][16 bytes]).
+template <typename TI>
+__host__ __device__ inline size_t rec_elem(int tile, int t, int N, int NL, int lane)
+{
+    constexpr int E = 16 / (int)sizeof(TI);
+    return ((size_t)(tile * (N / E) + t / E) * NL + lane) * E + t % E;
+}
+
 struct Error {
     std::string msg;
     explicit Error(std::string m) : msg(std::move(m)) {}
@@ -129,6 +139,13 @@ struct Ctx {
     int icur = 0;                          // buffer holding the latest I_crn
     int iold = 0;                          // buffer holding I_crn_old
     void *XV = nullptr, *XH = nullptr;     // tile face exchange
+    // reflective sides (BC_Type 1): bit 0 left, 1 bottom, 2 right, 3 top; the boundary
+    // slices hold the corner intensities of the domain faces, in the exchange record
+    // layout, per side and quadrant: BV [left, right][q][NTY tiles][NL lanes][2 TY],
+    // BH [bottom, top][q][NTX tiles][NL lanes][2 TX].  A side's incoming quadrants
+    // read their inflow there; its outgoing quadrants write their outflow there.
+    int refl = 0;
+    void *BV = nullptr, *BH = nullptr;
     double *sc = nullptr;                  // [cell][g][2] = (KapE + 1/(c dt), RT_Src)
     float *scf = nullptr;                  // FP32 copy of sc for the FP32 sweep
     double *Pavg = nullptr, *PV = nullptr, *PH = nullptr;   // per-quadrant partial moments
@@ -230,6 +247,12 @@ void pk_check(Ctx &c);
 void diag_sweep(Ctx &c, std::vector<hipEvent_t> *ev = nullptr);
 void pk_destroy(Ctx &c);
 int pk_grid(const Ctx &c);
+// RT_BC_UPDATE (UPDATES.f08:12-105) on the boundary slices of the reflective sides: each
+// incoming quadrant takes the outflow of its mirror quadrant from the last sweep
+void rt_bc_update(Ctx &c);
+// the inflow of every reflective side back to Bg_planck(bcT) (RT_INIT's RT_BC_UPDATE with
+// BC_Type 0, INITIALIZERS.f08:106)
+void reflect_reset(Ctx &c);
 // batched persistent sweep over many contexts (one precision and tile shape)
 struct PkBatch;
 // seg: column-segment length of the work items (0 = chosen from the parallelism)

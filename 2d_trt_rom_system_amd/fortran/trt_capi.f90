@@ -89,6 +89,10 @@ MODULE TRT_CAPI
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION
+    INTEGER(C_INT) FUNCTION trt_rt_bc_update(ctx) BIND(C, name="trt_rt_bc_update")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
     INTEGER(C_INT) FUNCTION trt_collapse_intensities(ctx) BIND(C, name="trt_collapse_intensities")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx

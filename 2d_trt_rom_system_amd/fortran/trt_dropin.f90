@@ -228,7 +228,7 @@ SUBROUTINE Cg_Calc(Cg_L,Cg_B,Cg_R,Cg_T,I_edgV,I_edgH,Omega_x,Omega_y,quad_weight
   CALL TRT_PULL_ARRAY('Cg_T', Cg_T, SIZE(Cg_T))
 END SUBROUTINE Cg_Calc
 
-! MGQD_In_Calc, MainSrc/MLOQD_SOLVES.f08:493-614 (incoming-Planckian boundaries)
+! MGQD_In_Calc, MainSrc/MLOQD_SOLVES.f08:493-614 (zero on reflective sides)
 SUBROUTINE MGQD_In_Calc(Eg_in_L,Eg_in_B,Eg_in_R,Eg_in_T,Fg_in_L,Fg_in_B,Fg_in_R,Fg_in_T,I_edgV,I_edgH,Omega_x,Omega_y,&
   quad_weight,c,Comp_Unit,BC_Type,Open_Threads)
   REAL*8,INTENT(OUT):: Eg_in_L(:,:), Eg_in_B(:,:), Eg_in_R(:,:), Eg_in_T(:,:)
@@ -561,6 +561,22 @@ SUBROUTINE MATERIAL_UPDATE(RT_Src,MGQD_Src,KapE,KapB,KapR,Bg,Temp,Comp_Unit,Nu_g
   END DO
   DEALLOCATE(src)
 END SUBROUTINE MATERIAL_UPDATE
+
+! RT_BC_UPDATE, MainSrc/UPDATES.f08:12-105 (BC_Type, bcT_*, Nu_g and Comp_Unit are those given to TRT_GPU_INIT): on the
+! reflective sides the incoming directions take the mirror directions' outgoing intensities of the last TRANSPORT_SCB; the
+! four intensity arrays are returned with the updated incoming faces.
+SUBROUTINE RT_BC_UPDATE(BC_Type,bcT_left,bcT_lower,bcT_right,bcT_upper,Comp_Unit,Nu_g,I_edgV,I_edgH,Ic_edgV,Ic_edgH)
+  REAL*8,INTENT(IN):: bcT_left, bcT_lower, bcT_right, bcT_upper
+  REAL*8,INTENT(IN):: Comp_Unit, Nu_g(:)
+  INTEGER,INTENT(IN):: BC_Type(:)
+  REAL*8,INTENT(OUT):: I_edgV(:,:,:,:), I_edgH(:,:,:,:), Ic_edgV(:,:,:,:), Ic_edgH(:,:,:,:)
+
+  CALL TRT_CHECK(trt_rt_bc_update(TRT_CTX), 'RT_BC_UPDATE')
+  CALL TRT_PULL_ARRAY('I_edgV', I_edgV, SIZE(I_edgV))
+  CALL TRT_PULL_ARRAY('I_edgH', I_edgH, SIZE(I_edgH))
+  CALL TRT_PULL_ARRAY('Ic_edgV', Ic_edgV, SIZE(Ic_edgV))
+  CALL TRT_PULL_ARRAY('Ic_edgH', Ic_edgH, SIZE(Ic_edgH))
+END SUBROUTINE RT_BC_UPDATE
 
 ! MEB_SOLVE, MainSrc/UPDATES.f08:189-231 (Bg, Delt, cV, Kap0, Comp_Unit are the context's; Eg_HO is
 ! pushed as the context's Eg_avg, the array the MLOQD loop passes, ALGORITHMS.f08:769)

@@ -57,7 +57,7 @@ EXPORTS = [
     "trt_convergence_noalloc", "trt_meb_solve", "trt_outfile_create", "trt_timestep_outs", "trt_tgen_outs",
     "trt_outfile_put", "trt_outfile_close", "trt_outfile_last_error", "trt_nc_open", "trt_nc_dim", "trt_nc_get",
     "trt_nc_last_error", "trt_nc_close", "trt_nc_att", "trt_romgrid_create", "trt_romgrid_tmap", "trt_romgrid_map",
-    "trt_romgrid_maps", "trt_romgrid_destroy",
+    "trt_romgrid_maps", "trt_romgrid_destroy", "trt_rt_bc_update",
 ]
 
 _lib = None
@@ -103,7 +103,7 @@ def load(path: str = LIBPATH):
     L.trt_convergence_noalloc.argtypes = [vp, vp, vp, C.c_size_t, C.c_int, C.c_double, C.c_int, vp, vp]
     for f in ("trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
               "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_collapse_mg_ef", "trt_cbar_calc",
-              "trt_grey_coefs", "trt_gqd_in_calc"):
+              "trt_grey_coefs", "trt_gqd_in_calc", "trt_rt_bc_update"):
         getattr(L, f).argtypes = [vp]
     L.trt_mloqd_fv.argtypes = [vp, C.POINTER(C.c_int)]
     L.trt_egp_fv_newt.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]

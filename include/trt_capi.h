@@ -37,7 +37,7 @@ typedef struct {
     const int *mat;                 /* Mat(N_x,N_y), 0-based material ids           */
     const double *kmult;            /* Kappa_Mult(nmat)                             */
     double c, dt, theta, comp_unit, cv, kap0, tini;
-    int bc_type[4];                 /* left, bottom, right, top; only 0 (incoming Planckian) */
+    int bc_type[4];                 /* left, bottom, right, top: 0 incoming Planckian, 1 reflective */
     double bcT[4];
     int opacity_model;              /* 0 = kapB_calc group Planck mean, 1 = group-midpoint formula */
     int maxit_rte, maxit_mloqd, maxit_gloqd, conv_type;
@@ -101,6 +101,13 @@ int trt_material_update(trt_ctx *ctx);
 /* TRANSPORT_SCB, MainSrc/TRANSPORT_SOLVES.f08:14-327, with the angular moments
  * of COLLAPSE_INTENSITIES (:381-511) accumulated in the same pass. */
 int trt_transport_scb(trt_ctx *ctx);
+/* RT_BC_UPDATE, MainSrc/UPDATES.f08:12-105: on each reflective side (BC_Type 1) the
+ * incoming directions take the outgoing intensities of their mirror directions from the
+ * last sweep (the next sweep's inflow, the incoming faces of the I_edgV/I_edgH views);
+ * nothing on a context without reflective sides.  trt_step calls it after every sweep
+ * (ALGORITHMS.f08:556-565); the Tgen_QDf steps do not, as the reference's Tgen_QDf
+ * does not (its reflective sides keep their initial Planckian inflow). */
+int trt_rt_bc_update(trt_ctx *ctx);
 /* COLLAPSE_INTENSITIES, MainSrc/TRANSPORT_SOLVES.f08:381-511 (reduces the partial
  * moments written by trt_transport_scb). */
 int trt_collapse_intensities(trt_ctx *ctx);

@@ -81,8 +81,9 @@ def lib(path: str | None = None):
         L.orc_iarray.restype = C.POINTER(C.c_int)
         L.orc_iarray.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int * 4)]
         for f in ("orc_material_update", "orc_transport_scb", "orc_collapse_intensities", "orc_fg_calc",
-                  "orc_cg_calc", "orc_mgqd_in_calc", "orc_gqd_in_calc", "orc_old_mgqd_coefs", "orc_meb_solve",
-                  "orc_old_grey_coefs", "orc_mloqd_fv", "orc_collapse_mg_ef", "orc_cbar_calc", "orc_grey_coefs"):
+                  "orc_cg_calc", "orc_mgqd_in_calc", "orc_gqd_in_calc", "orc_rt_bc_update", "orc_old_mgqd_coefs",
+                  "orc_meb_solve", "orc_old_grey_coefs", "orc_mloqd_fv", "orc_collapse_mg_ef", "orc_cbar_calc",
+                  "orc_grey_coefs"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = None
         L.orc_egp_fv_newt.argtypes = [C.c_void_p, C.c_int]

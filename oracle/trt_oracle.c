@@ -37,13 +37,14 @@
 #include <stdint.h>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
 
 /* ORC_MUTANT: deliberate mistakes compiled in only by the pin tests
- * (tests/test_oracle_grey_pins.py, through oracle.build_mutant) to show that the
- * pins catch them.  0 -- the oracle itself -- in every other build. */
+ * (tests/test_oracle_grey_pins.py and tests/test_oracle_reflective.py, through
+ * oracle.build_mutant) to show that the pins catch them.  0 -- the oracle itself --
+ * in every other build. */
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
-#endif
 #endif
 
 static const double PI = 3.14159265358979323846264338327950288;
@@ -633,6 +634,57 @@ void orc_gqd_in_calc(orc_state *s)
             cc += s->Fg_in_B[i + g * nx]; d += s->Fg_in_T[i + g * nx];
         }
         s->E_in_B[i] = a; s->E_in_T[i] = b; s->F_in_B[i] = cc; s->F_in_T[i] = d;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* RT_BC_UPDATE, UPDATES.f08:12-105: the incoming boundary intensities (I_edgV,   */
+/* Ic_edgV on the left and right sides, I_edgH, Ic_edgH on the bottom and top).    */
+/* BC_Type 0 sets them to Bg_planck(bcT of the side); BC_Type 1 (reflective)       */
+/* copies, face by face and corner by corner, the outgoing intensity of the        */
+/* direction with the same index in the mirrored quadrant (quadrants 1..4 are the  */
+/* blocks of N_m/4 directions): left q1 <- q2 and q4 <- q3 (:34-44), bottom        */
+/* q1 <- q4 and q2 <- q3 (:53-63), right q2 <- q1 and q3 <- q4 (:72-82), top        */
+/* q3 <- q2 and q4 <- q1 (:91-101).  Other types leave the side alone.             */
+/* ------------------------------------------------------------------------ */
+void orc_rt_bc_update(orc_state *s)
+{
+    const int nx = s->nx, ny = s->ny, nm = s->nm, ng = s->ng, K = nm / 4;
+    const int nx2 = 2 * nx, ny2 = 2 * ny;
+    const double cu = s->p.comp_unit;
+    /* 0-based quadrants: incoming ones of each side (left, bottom, right, top), their sources */
+    static const int qin[4][2] = {{0, 3}, {0, 1}, {1, 2}, {2, 3}};
+    static const int qsrc[4][2] = {{1, 2}, {3, 2}, {ORC_MUTANT == 9 ? 3 : 0, ORC_MUTANT == 9 ? 0 : 3}, {1, 0}};
+    for (int side = 0; side < 4; ++side) {
+        const int type = s->p.bc_type[side];
+        if (type != 0 && type != 1) continue;
+        for (int g = 0; g < ng; ++g) {
+            const double b = orc_Bg_planck(s->p.bcT[side], s->nu[g], s->nu[g + 1], cu);
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < K; ++k) {
+                    const int m = qin[side][h] * K + k, ms = qsrc[side][h] * K + k;
+                    if (side == 0 || side == 2) {
+                        const int f = side == 0 ? 0 : nx;
+                        for (int j = 0; j < ny; ++j) {
+                            s->I_edgV[IX4(f, j, m, g, nx + 1, ny, nm)] =
+                                type ? s->I_edgV[IX4(f, j, ms, g, nx + 1, ny, nm)] : b;
+                            for (int c2 = 0; c2 < 2; ++c2)
+                                s->Ic_edgV[IX4(f, 2 * j + c2, m, g, nx + 1, ny2, nm)] =
+                                    type ? s->Ic_edgV[IX4(f, 2 * j + (ORC_MUTANT == 10 ? 1 - c2 : c2), ms, g, nx + 1,
+                                                          ny2, nm)] : b;
+                        }
+                    } else {
+                        const int f = side == 1 ? 0 : ny;
+                        for (int i = 0; i < nx; ++i) {
+                            s->I_edgH[IX4(i, f, m, g, nx, ny + 1, nm)] =
+                                type ? s->I_edgH[IX4(i, f, ms, g, nx, ny + 1, nm)] : b;
+                            for (int c2 = 0; c2 < 2; ++c2)
+                                s->Ic_edgH[IX4(2 * i + c2, f, m, g, nx2, ny + 1, nm)] =
+                                    type ? s->Ic_edgH[IX4(2 * i + c2, f, ms, g, nx2, ny + 1, nm)] : b;
+                        }
+                    }
+                }
+        }
     }
 }
 
@@ -1671,6 +1723,11 @@ int orc_step_ex(orc_state *s, int it_out[4], int ho_form)
             orc_transport_scb(s);                /* AL:541-542 */
             sweeps++;
             orc_collapse_intensities(s);         /* AL:551-553 */
+            if (s->p.bc_type[0] > 0 || s->p.bc_type[1] > 0 || s->p.bc_type[2] > 0 || s->p.bc_type[3] > 0) {
+                orc_rt_bc_update(s);             /* AL:556-565 */
+                orc_mgqd_in_calc(s);
+                if (!s->p.no_grey) orc_gqd_in_calc(s);
+            }
             orc_fg_calc(s);                      /* AL:568-569 */
             orc_cg_calc(s);                      /* AL:572 */
         }

@@ -69,6 +69,8 @@ void orc_fg_calc(orc_state *s);
 void orc_cg_calc(orc_state *s);
 void orc_mgqd_in_calc(orc_state *s);
 void orc_gqd_in_calc(orc_state *s);
+/* RT_BC_UPDATE, UPDATES.f08:12-105 (BC_Type 0: Planckian inflow, 1: reflective) */
+void orc_rt_bc_update(orc_state *s);
 void orc_old_mgqd_coefs(orc_state *s);
 void orc_old_grey_coefs(orc_state *s);
 void orc_mloqd_fv(orc_state *s);
