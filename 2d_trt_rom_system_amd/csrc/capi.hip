@@ -59,8 +59,9 @@ void validate(const trt_problem *p)
     if (p->no_grey != 0 && p->no_grey != 1) throw Error("no_grey must be 0 or 1");
     if (p->use_safety_search && !(p->line_src > 1.0))
         throw Error("the safety search needs a line-search factor line_src above 1");
-    if (p->qd_solver < 0 || p->qd_solver > 2)
-        throw Error("qd_solver must be 0 (nested dissection), 1 (BiCGSTAB) or 2 (block LU)");
+    if (p->qd_solver < 0 || p->qd_solver > 3)
+        throw Error("qd_solver must be 0 (nested dissection, factors reused within a step), 1 (BiCGSTAB), 2 (block "
+                    "LU) or 3 (nested dissection factorizing every solve)");
     if (p->dt <= 0 || p->c <= 0 || p->theta <= 0) throw Error("c, dt and theta must be positive");
     for (size_t k = 0; k < (size_t)p->nx * p->ny; ++k)
         if (p->mat[k] < 0 || p->mat[k] >= p->nmat) throw Error("material id out of range");
@@ -379,6 +380,15 @@ int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms)
     if (!ctx) return -1;
     if (sweeps) *sweeps = ctx->c.sweeps;
     if (ms) *ms = ctx->c.sweep_ms;
+    return 0;
+}
+
+int trt_qd_stats(const trt_ctx *ctx, long long *factorized, long long *reused, long long *refine_steps)
+{
+    if (!ctx) return -1;
+    if (factorized) *factorized = ctx->c.nd_factor;
+    if (reused) *reused = ctx->c.nd_reused;
+    if (refine_steps) *refine_steps = ctx->c.nd_reuse_steps;
     return 0;
 }
 

@@ -473,11 +473,22 @@ static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vec
 {
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<int> bad;
+    int steps = -1;
     if (c.qd_solver == 2) {
         bad = which;       // block LU chosen for every system
     } else {
         try {
-            qd_nd(c, s, which);
+            // qd_solver 0: first the factors of these systems' last factorization in this
+            // time step, by iterative refinement to the same backward error; a fresh
+            // factorization when they no longer contract
+            steps = c.qd_solver == 0 ? qd_nd_reuse(c, s, which, 10) : -1;
+            if (steps >= 0) {
+                c.nd_reused += 1;
+                c.nd_reuse_steps += steps;
+            } else {
+                qd_nd(c, s, which);
+                c.nd_factor += 1;
+            }
             const DirectRes d = direct_residuals(c, s, a, which);
             for (size_t q = 0; q < which.size(); ++q)
                 if (!(d.bkwd[q] <= 1e-11)) bad.push_back(which[q]);
@@ -494,8 +505,10 @@ static void direct_and_check(Ctx &c, QDSystems &s, KrylovArgs &a, const std::vec
         for (size_t q = 1; q < which.size(); ++q)
             if (!(d.bkwd[q] <= d.bkwd[w])) w = q;
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        fprintf(stderr, "[qd_solve] direct: %zu of %d systems of %zu rows in %.2f ms, worst backward error %.3e "
-                "(system %d, |r|/|b| = %.3e)\n", which.size(), s.nsys, s.nrow, ms, d.bkwd[w], which[w], d.rel[w]);
+        fprintf(stderr, "[qd_solve] direct: %zu of %d systems of %zu rows in %.2f ms (%s %d), worst backward error "
+                "%.3e (system %d, |r|/|b| = %.3e)\n", which.size(), s.nsys, s.nrow, ms,
+                steps >= 0 ? "reused factors, refinement steps" : "factorized, qd_solver", steps >= 0 ? steps : c.qd_solver,
+                d.bkwd[w], which[w], d.rel[w]);
     }
     for (size_t q = 0; q < which.size(); ++q)
         if (!(d.bkwd[q] <= 1e-9)) {
@@ -1480,6 +1493,10 @@ int step(Ctx &c, int its_out[4], int ho_form)
     int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0;
     bool rt_conv = false;
     c.itlog.clear();
+    // factors are reused within a time step only: a step depends on the state at its
+    // start alone, so a run resumed from a restart file continues bitwise
+    qd_nd_forget(c.mg);
+    qd_nd_forget(c.grey);
     const int rt_start = 2;                  // AL:302
     // ho_form 1 (the ROM runs, HO_Form 1 and 3, AL:573-631): in each RT iteration the
     // QD factors and C_g of the new time level come from the caller's reconstruction
@@ -1561,6 +1578,10 @@ int step(Ctx &c, int its_out[4], int ho_form)
     }
     TRT_CHECK(hipStreamSynchronize(c.stream));
     if (its_out) { its_out[0] = rt_its; its_out[1] = mg_total; its_out[2] = newt_total; its_out[3] = sweeps; }
+    if (c.verbose)
+        fprintf(stderr, "[step] RT %d, MLOQD %d, Newton %d; direct solves so far: %lld factorizations, %lld on reused "
+                "factors (%lld refinement steps)\n", rt_its, mg_total, newt_total, c.nd_factor, c.nd_reused,
+                c.nd_reuse_steps);
     return rt_its;
 }
 

@@ -259,6 +259,47 @@ __global__ void nd_pull_vec(const NDLevDev *lev, int L, int ns, const int *kids)
     }
 }
 
+// The solve phase on fronts whose pivot block holds its explicit inverse G = A_II^-1
+// (the factor-reuse mode, levels of fronts too large for nd_front_small), one block
+// per (node, system) of a level.  Forward:
+// z_I = G f_I in place of f_I, then f_B -= F_BI z_I.  Backward, with x_B gathered
+// into f_B: x_I = z_I - (A_II^-1 F_IB) x_B in place of z_I.  Matrix columns are read
+// across the threads (column-major fronts): every load instruction is coalesced.
+__global__ __launch_bounds__(256) void nd_fwd_inv(const double *F, double *f, int nIp, int nBp, int nF)
+{
+    extern __shared__ double sh[];            // f_I, then z
+    const double *Fb = F + (size_t)blockIdx.x * nF * nF;
+    double *fb = f + (size_t)blockIdx.x * nF;
+    for (int i = threadIdx.x; i < nIp; i += blockDim.x) sh[i] = fb[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < nIp; ++j) acc += Fb[i + (size_t)j * nF] * sh[j];
+        sh[nIp + i] = acc;
+        fb[i] = acc;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nBp; q += blockDim.x) {
+        double acc = fb[nIp + q];
+        for (int j = 0; j < nIp; ++j) acc -= Fb[nIp + q + (size_t)j * nF] * sh[nIp + j];
+        fb[nIp + q] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void nd_bwd_inv(const double *F, double *f, int nIp, int nBp, int nF)
+{
+    extern __shared__ double sh[];            // x_B
+    const double *Fb = F + (size_t)blockIdx.x * nF * nF;
+    double *fb = f + (size_t)blockIdx.x * nF;
+    for (int q = threadIdx.x; q < nBp; q += blockDim.x) sh[q] = fb[nIp + q];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
+        double acc = fb[i];
+        for (int q = 0; q < nBp; ++q) acc -= Fb[i + (size_t)(nIp + q) * nF] * sh[q];
+        fb[i] = acc;
+    }
+}
+
 __global__ void nd_add(double *x, const double *D, const int *sysmap, int ns, size_t nrow)
 {
     const size_t tot = (size_t)ns * nrow;
@@ -364,7 +405,7 @@ __device__ __forceinline__ void nd_small_back(NDSmall &w)
 }
 
 __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int nF, int nIp, size_t sF, size_t sf,
-                                                     rocblas_int *ipiv, rocblas_int *info, int nb)
+                                                     rocblas_int *ipiv, rocblas_int *info, int *perm, int nb)
 {
     __shared__ double prow[ND_SF + 1];
     __shared__ int s_orig[ND_SI];
@@ -402,6 +443,8 @@ __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int n
             if (j >= nIp && j < nF) A[(size_t)j * nF + w.t] = w.r[j];
         v[w.t] = w.rf;
     }
+    // the front row chosen at each pivot step (the row permutation, for nd_fwd_lu_small)
+    if (w.t < nIp) perm[fr * nIp + w.t] = s_orig[w.t];
     // getrf's pivot sequence: at step k rows k and ipiv[k]-1 of the current order swap
     if (w.t == 0) {
         rocblas_int *pv = ipiv + fr * nIp;
@@ -419,6 +462,35 @@ __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int n
         }
         info[fr] = w.zero_at;
     }
+}
+
+// The forward solve phase on a level of small fronts with the LU factors nd_front_small
+// left (one wavefront per front, lane t holding entry t): z_I = A_II^-1 f_I by the row
+// permutation, the unit lower and the upper triangular solves, then f_B -= F_BI z_I.
+__global__ __launch_bounds__(64) void nd_fwd_lu_small(const double *F, double *f, const int *perm, int nIp, int nBp,
+                                                      int nF)
+{
+    const size_t fr = blockIdx.x;
+    const double *A = F + fr * nF * nF;
+    double *v = f + fr * nF;
+    const int t = threadIdx.x;
+    double y = t < nIp ? v[perm[fr * nIp + t]] : 0.0;
+    for (int k = 0; k < nIp; ++k) {               // unit lower: column k below the diagonal
+        const double yk = __shfl(y, k);
+        if (t > k && t < nIp) y = fma(-A[t + (size_t)k * nF], yk, y);
+    }
+    for (int k = nIp - 1; k >= 0; --k) {          // upper: x_k = y_k / U_kk, then column k above it
+        if (t == k) y = y / A[(size_t)k + (size_t)k * nF];
+        const double yk = __shfl(y, k);
+        if (t < k) y = fma(-A[t + (size_t)k * nF], yk, y);
+    }
+    if (t < nIp) v[t] = y;
+    double acc = t < nBp ? v[nIp + t] : 0.0;
+    for (int j = 0; j < nIp; ++j) {
+        const double zj = __shfl(y, j);
+        if (t < nBp) acc = fma(-A[nIp + t + (size_t)j * nF], zj, acc);
+    }
+    if (t < nBp) v[nIp + t] = acc;
 }
 
 __global__ void nd_info_max(const rocblas_int *info, size_t n, int *out)
@@ -449,6 +521,7 @@ struct NDPlan {
     size_t fsz = 0, vsz = 0, psz = 0;   // per-system doubles of F, f and pivots over all levels
     double *F = nullptr, *f = nullptr;
     rocblas_int *ipiv = nullptr, *info = nullptr;
+    int *perm = nullptr;            // small fronts: the front row of each pivot position
     int *sysmap = nullptr, *imax = nullptr;
     int *ident = nullptr;           // 0, 1, ..., cap-1
     double *R = nullptr, *D = nullptr;   // refinement residual and correction [cap][nrow]
@@ -457,15 +530,12 @@ struct NDPlan {
     size_t ninfo = 0;
     rocblas_handle h = nullptr;
     double flops = 0.0;             // per system
+    // the factors the last numeric phase left in F, ipiv and W, of the systems
+    // fact_which (one chunk): a preconditioner for later solves of those systems
+    bool fact_ok = false;
+    std::vector<int> fact_which;
+    bool ginv = false;              // the pivot blocks of the large fronts hold A_II^-1 instead of their LU factors
 };
-
-void nd_destroy(Ctx &c)
-{
-    if (!c.nd) return;
-    if (c.nd->h) rocblas_destroy_handle(c.nd->h);
-    delete c.nd;     // device buffers belong to the context (dalloc)
-    c.nd = nullptr;
-}
 
 // Symbolic phase: the dissection tree, fronts, assembly and extend-add maps.
 static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
@@ -642,10 +712,11 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
     P->fsz = fo; P->vsz = vo; P->psz = po;
     // systems per chunk: fronts of all levels within 8 GiB
     const double budget = 8.0 * (1u << 30);
-    P->cap = (int)std::max(1.0, std::min<double>(std::max(s.nsys, c.ng), std::floor(budget / (8.0 * (double)fo))));
+    P->cap = (int)std::max(1.0, std::min<double>(s.nsys, std::floor(budget / (8.0 * (double)fo))));
     P->F = static_cast<double *>(c.dalloc((size_t)P->cap * fo * sizeof(double)));
     P->f = static_cast<double *>(c.dalloc((size_t)P->cap * vo * sizeof(double)));
     P->ipiv = static_cast<rocblas_int *>(c.dalloc((size_t)P->cap * po * sizeof(rocblas_int)));
+    P->perm = static_cast<int *>(c.dalloc((size_t)P->cap * po * sizeof(int)));
     size_t ni = 0;
     for (const NDLevel &L : P->lev) ni += (size_t)L.nn;
     P->ninfo = ni * P->cap;
@@ -668,30 +739,160 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
     return P;
 }
 
-// Solve the listed systems of s; their solutions replace s.x.
-void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
+// fronts that nd_front_small factors (one wavefront each)
+static bool nd_small(const NDLevel &L) { return L.nF <= ND_SF && L.nIp <= ND_SI; }
+
+static QDView nd_view(const QDSystems &s)
 {
-    if (which.empty()) return;
-    if (c.nd && (c.nd->nx != s.nx || c.nd->ny != s.ny || c.nd->nrow != s.nrow)) nd_destroy(c);
-    if (!c.nd) c.nd = nd_plan(c, s);
-    NDPlan &P = *c.nd;
-    ND_RB(rocblas_set_stream(P.h, c.stream));
     QDView V{};
     V.nx = s.nx; V.ny = s.ny; V.nb = std::max(s.nx, s.ny);
     V.nc = (size_t)s.nx * s.ny; V.nv = (size_t)(s.nx + 1) * s.ny; V.nh = (size_t)s.nx * (s.ny + 1);
     V.nrow = s.nrow; V.coef = s.coef; V.bnd = s.bnd; V.cs = nullptr;
+    return V;
+}
+
+// The level table of a chunk of ns systems (front and right-hand-side blocks of each
+// level in P.F, P.f), uploaded for the kernels that walk the tree.
+static std::vector<NDLevDev> nd_levels(Ctx &c, NDPlan &P, int ns)
+{
+    const int H = (int)P.lev.size() - 1;
+    std::vector<NDLevDev> tab(H + 1);
+    for (int l = 0; l <= H; ++l) {
+        const NDLevel &L = P.lev[l];
+        tab[l] = NDLevDev{L.nn, L.nIp, L.nBp, L.nF, P.F + L.foff * ns, P.f + L.voff * ns, L.up};
+    }
+    TRT_CHECK(hipMemcpyAsync(P.levdev, tab.data(), tab.size() * sizeof(NDLevDev), hipMemcpyHostToDevice, c.stream));
+    return tab;
+}
+
+// Normwise backward error of every system of the chunk at x, in the rows scaled by P.W,
+// with the residual b - A x left in P.R; true when all are at most tol (and finite).
+static bool nd_bkwd(Ctx &c, NDPlan &P, const QDView &V, const double *x, int ns, double tol, double *worst)
+{
+    hipLaunchKernelGGL(nd_residual_bkwd, dim3(ND_RB_BLOCKS, ns), dim3(256), 0, c.stream, V, P.sysmap, x, P.W, P.R,
+                       P.part);
+    hipLaunchKernelGGL(nd_bkwd_fin, dim3((ns + 63) / 64), dim3(64), 0, c.stream, P.part, ND_RB_BLOCKS, ns, P.bkwd);
+    std::vector<double> hb(ns);
+    TRT_CHECK(hipMemcpyAsync(hb.data(), P.bkwd, ns * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
+    bool ok = true;
+    *worst = 0.0;
+    for (double b : hb) {
+        if (!(b <= tol)) ok = false;
+        *worst = std::isfinite(b) ? std::fmax(*worst, b) : INFINITY;
+    }
+    return ok;
+}
+
+// One solve with the stored factors: D = (W A_f)^-1 W R for the chunk's systems, by
+// elimination through the fronts bottom-up and back substitution top-down, then x += D.
+static void nd_apply(Ctx &c, NDPlan &P, const std::vector<NDLevDev> &tab, int ns, size_t nrow, double *x)
+{
+    const int H = (int)P.lev.size() - 1;
+    const double one = 1.0, mone = -1.0;
+    hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
+    if (P.ginv) {
+        for (int l = 0; l <= H; ++l) {
+            const NDLevel &L = P.lev[l];
+            const int nb = L.nn * ns, bs = L.nF <= 64 ? 64 : 256;
+            hipLaunchKernelGGL(nd_rhs, dim3(nb), dim3(128), 0, c.stream, P.R, P.W, nrow, ns, L.nIp, L.nF, L.rows,
+                               tab[l].f);
+            if (l > 0) hipLaunchKernelGGL(nd_pull_vec, dim3(nb), dim3(128), 0, c.stream, P.levdev, l, ns, L.kids);
+            if (nd_small(L))
+                hipLaunchKernelGGL(nd_fwd_lu_small, dim3(nb), dim3(64), 0, c.stream, tab[l].F, tab[l].f,
+                                   P.perm + L.poff * ns, L.nIp, L.nBp, L.nF);
+            else
+                hipLaunchKernelGGL(nd_fwd_inv, dim3(nb), dim3(bs), 2 * L.nIp * sizeof(double), c.stream, tab[l].F,
+                                   tab[l].f, L.nIp, L.nBp, L.nF);
+        }
+        for (int l = H; l >= 0; --l) {
+            const NDLevel &L = P.lev[l];
+            const int nb = L.nn * ns, bs = L.nF <= 64 ? 64 : 256;
+            if (L.nBp > 0) {
+                hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nBp,
+                                   L.nF, L.rows, tab[l].f);
+                hipLaunchKernelGGL(nd_bwd_inv, dim3(nb), dim3(bs), std::max(L.nBp, 1) * sizeof(double), c.stream,
+                                   tab[l].F, tab[l].f, L.nIp, L.nBp, L.nF);
+            }
+            hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nF,
+                               L.rows, tab[l].f);
+        }
+        hipLaunchKernelGGL(nd_add, dim3(nblocks((size_t)ns * nrow)), dim3(256), 0, c.stream, x, P.D, P.sysmap, ns,
+                           nrow);
+        TRT_CHECK(hipGetLastError());
+        return;
+    }
+    for (int l = 0; l <= H; ++l) {
+        const NDLevel &L = P.lev[l];
+        double *F = tab[l].F, *f = tab[l].f;
+        const int nb = L.nn * ns;
+        hipLaunchKernelGGL(nd_rhs, dim3(nb), dim3(128), 0, c.stream, P.R, P.W, nrow, ns, L.nIp, L.nF, L.rows, f);
+        if (l > 0) hipLaunchKernelGGL(nd_pull_vec, dim3(nb), dim3(128), 0, c.stream, P.levdev, l, ns, L.kids);
+        const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
+        rocblas_int *piv = P.ipiv + L.poff * ns;
+        ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, 1, F, L.nF, sF, piv, sP, f, L.nF,
+                                               sf, nb));
+        if (L.nBp > 0)
+            ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nBp, 1, L.nIp,
+                                                &mone, F + L.nIp, L.nF, sF, f, L.nF, sf, &one, f + L.nIp, L.nF, sf,
+                                                nb));
+    }
+    for (int l = H; l >= 0; --l) {
+        const NDLevel &L = P.lev[l];
+        double *F = tab[l].F, *f = tab[l].f;
+        const int nb = L.nn * ns;
+        if (L.nBp > 0) {
+            hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nBp,
+                               L.nF, L.rows, f);
+            const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF;
+            ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nIp, 1, L.nBp,
+                                                &mone, F + (size_t)L.nIp * L.nF, L.nF, sF, f + L.nIp, L.nF, sf, &one,
+                                                f, L.nF, sf, nb));
+        }
+        hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nF, L.rows,
+                           f);
+    }
+    hipLaunchKernelGGL(nd_add, dim3(nblocks((size_t)ns * nrow)), dim3(256), 0, c.stream, x, P.D, P.sysmap, ns, nrow);
+    TRT_CHECK(hipGetLastError());
+}
+
+void nd_destroy(Ctx &c)
+{
+    for (QDSystems *s : {&c.mg, &c.grey}) {
+        if (!s->nd) continue;
+        if (s->nd->h) rocblas_destroy_handle(s->nd->h);
+        delete s->nd;     // device buffers belong to the context (dalloc)
+        s->nd = nullptr;
+    }
+}
+
+static NDPlan &nd_plan_of(Ctx &c, QDSystems &s)
+{
+    if (s.nd && (s.nd->nx != s.nx || s.nd->ny != s.ny || s.nd->nrow != s.nrow)) {
+        if (s.nd->h) rocblas_destroy_handle(s.nd->h);
+        delete s.nd;
+        s.nd = nullptr;
+    }
+    if (!s.nd) s.nd = nd_plan(c, s);
+    ND_RB(rocblas_set_stream(s.nd->h, c.stream));
+    return *s.nd;
+}
+
+// Solve the listed systems of s; their solutions replace s.x.  When they fit one
+// chunk their factors stay in the plan, for qd_nd_reuse.
+void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
+{
+    if (which.empty()) return;
+    NDPlan &P = nd_plan_of(c, s);
+    P.fact_ok = false;
+    P.ginv = false;
+    const QDView V = nd_view(s);
     const int H = (int)P.lev.size() - 1;
     const double one = 1.0, mone = -1.0;
     for (size_t c0 = 0; c0 < which.size(); c0 += P.cap) {
         const int ns = (int)std::min<size_t>(P.cap, which.size() - c0);
         TRT_CHECK(hipMemcpyAsync(P.sysmap, which.data() + c0, ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
-        // level table for this chunk size
-        std::vector<NDLevDev> tab(H + 1);
-        for (int l = 0; l <= H; ++l) {
-            const NDLevel &L = P.lev[l];
-            tab[l] = NDLevDev{L.nn, L.nIp, L.nBp, L.nF, P.F + L.foff * ns, P.f + L.voff * ns, L.up};
-        }
-        TRT_CHECK(hipMemcpyAsync(P.levdev, tab.data(), tab.size() * sizeof(NDLevDev), hipMemcpyHostToDevice, c.stream));
+        const std::vector<NDLevDev> tab = nd_levels(c, P, ns);
         hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.fsz * ns)), dim3(256), 0, c.stream, P.F, P.fsz * ns);
         hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
         hipLaunchKernelGGL(nd_rowscale, dim3(nblocks((size_t)ns * s.nrow)), dim3(256), 0, c.stream, V, P.sysmap, ns,
@@ -707,10 +908,10 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
             TRT_CHECK(hipGetLastError());
             const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
             rocblas_int *piv = P.ipiv + L.poff * ns;
-            if (L.nF <= ND_SF && L.nIp <= ND_SI) {
+            if (nd_small(L)) {
                 // small fronts (the leaves and the levels above them): one kernel, one wavefront each
                 hipLaunchKernelGGL(nd_front_small, dim3(nb), dim3(64), 0, c.stream, F, f, L.nF, L.nIp, (size_t)sF,
-                                   (size_t)sf, piv, P.info + ioff, nb);
+                                   (size_t)sf, piv, P.info + ioff, P.perm + L.poff * ns, nb);
                 TRT_CHECK(hipGetLastError());
                 ioff += (size_t)nb;
                 continue;
@@ -753,66 +954,77 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
         TRT_CHECK(hipMemcpyAsync(&imax, P.imax, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         TRT_CHECK(hipStreamSynchronize(c.stream));
         if (imax > 0) throw Error("nested dissection: a front's pivot block is singular");
+        // factor-reuse mode: the pivot blocks' LU factors become their inverses, so that
+        // later solves with these factors are matrix-vector products (bandwidth-bound)
+        // instead of batched triangular solves
+        if (c.qd_solver == 0) {
+            for (int l = 0; l <= H; ++l) {
+                const NDLevel &L = P.lev[l];
+                if (nd_small(L)) continue;       // small fronts keep their LU (nd_fwd_lu_small)
+                const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sP = L.nIp;
+                ND_RB(rocsolver_dgetri_strided_batched(P.h, L.nIp, tab[l].F, L.nF, sF, P.ipiv + L.poff * ns, sP,
+                                                       P.info, L.nn * ns));
+            }
+            P.ginv = true;
+        }
         // iterative refinement with the stored factors while the worst backward
         // error of the chunk is above 1e-14 (at most three steps): pivoting inside
         // each front only can leave it above rounding
-        std::vector<double> hb(ns);
         for (int it = 0;; ++it) {
-            const size_t nv = (size_t)ns * s.nrow;
-            hipLaunchKernelGGL(nd_residual_bkwd, dim3(ND_RB_BLOCKS, ns), dim3(256), 0, c.stream, V, P.sysmap, s.x,
-                               P.W, P.R, P.part);
-            hipLaunchKernelGGL(nd_bkwd_fin, dim3((ns + 63) / 64), dim3(64), 0, c.stream, P.part, ND_RB_BLOCKS, ns,
-                               P.bkwd);
-            TRT_CHECK(hipMemcpyAsync(hb.data(), P.bkwd, ns * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-            TRT_CHECK(hipStreamSynchronize(c.stream));
-            bool above = false;    // a system above 1e-14, or not finite
             double worst = 0.0;
-            for (double b : hb) {
-                if (!(b <= 1e-14)) above = true;
-                worst = std::fmax(worst, b);
-            }
+            const bool ok = nd_bkwd(c, P, V, s.x, ns, 1e-14, &worst);
             if (c.verbose > 2)
                 fprintf(stderr, "[qd_nd] chunk of %d: backward error %.3e%s after %d refinements\n", ns, worst,
-                        above ? " (or not finite)" : "", it);
-            if (!above || it == 3) break;
-            hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
-            for (int l = 0; l <= H; ++l) {
-                const NDLevel &L = P.lev[l];
-                double *F = tab[l].F, *f = tab[l].f;
-                const int nb = L.nn * ns;
-                hipLaunchKernelGGL(nd_rhs, dim3(nb), dim3(128), 0, c.stream, P.R, P.W, s.nrow, ns, L.nIp, L.nF, L.rows,
-                                   f);
-                if (l > 0) hipLaunchKernelGGL(nd_pull_vec, dim3(nb), dim3(128), 0, c.stream, P.levdev, l, ns, L.kids);
-                const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
-                rocblas_int *piv = P.ipiv + L.poff * ns;
-                ND_RB(rocsolver_dgetrs_strided_batched(P.h, rocblas_operation_none, L.nIp, 1, F, L.nF, sF, piv, sP, f,
-                                                       L.nF, sf, nb));
-                if (L.nBp > 0)
-                    ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nBp, 1,
-                                                        L.nIp, &mone, F + L.nIp, L.nF, sF, f, L.nF, sf, &one,
-                                                        f + L.nIp, L.nF, sf, nb));
-            }
-            for (int l = H; l >= 0; --l) {
-                const NDLevel &L = P.lev[l];
-                double *F = tab[l].F, *f = tab[l].f;
-                const int nb = L.nn * ns;
-                if (L.nBp > 0) {
-                    hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, P.D, s.nrow, P.ident, ns, L.nIp,
-                                       L.nBp, L.nF, L.rows, f);
-                    const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF;
-                    ND_RB(rocblas_dgemm_strided_batched(P.h, rocblas_operation_none, rocblas_operation_none, L.nIp, 1,
-                                                        L.nBp, &mone, F + (size_t)L.nIp * L.nF, L.nF, sF, f + L.nIp,
-                                                        L.nF, sf, &one, f, L.nF, sf, nb));
-                }
-                hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, P.D, s.nrow, P.ident, ns, L.nIp,
-                                   L.nF, L.rows, f);
-            }
-            hipLaunchKernelGGL(nd_add, dim3(nblocks(nv)), dim3(256), 0, c.stream, s.x, P.D, P.sysmap, ns, s.nrow);
-            TRT_CHECK(hipGetLastError());
+                        ok ? "" : " (or not finite)", it);
+            if (ok || it == 3) break;
+            nd_apply(c, P, tab, ns, s.nrow, s.x);
+        }
+        if (c0 == 0 && (size_t)ns == which.size()) {
+            P.fact_ok = true;
+            P.fact_which = which;
         }
     }
     if (c.verbose > 1)
         fprintf(stderr, "[qd_nd] %zu systems, %zu levels, %.2e flop per system\n", which.size(), P.lev.size(), P.flops);
+}
+
+// The listed systems solved with the factors that the last numeric phase of s left in
+// its plan, for the same systems, whose matrices have changed since (the Newton and
+// MLOQD iterations of a time step change them a little): iterative refinement from the
+// initial guess in s.x, x += (W A_f)^-1 W (b - A x), until every normwise backward
+// error is at most 1e-14, the acceptance of qd_nd.  Returns the refinement steps taken,
+// or -1 -- s.x then undefined, the caller solves afresh -- when the plan holds no
+// factors of these systems, or when the observed contraction (the last step's ratio of
+// worst backward errors) is above 1/2 or predicts more than max_steps steps in all:
+// refinement stops as soon as it would cost more than it saves.
+void qd_nd_forget(QDSystems &s)
+{
+    if (s.nd) s.nd->fact_ok = false;
+}
+
+int qd_nd_reuse(Ctx &c, QDSystems &s, const std::vector<int> &which, int max_steps)
+{
+    if (which.empty() || !s.nd || !s.nd->fact_ok || s.nd->fact_which != which) return -1;
+    NDPlan &P = nd_plan_of(c, s);
+    const int ns = (int)which.size();
+    const QDView V = nd_view(s);
+    TRT_CHECK(hipMemcpyAsync(P.sysmap, which.data(), ns * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    const std::vector<NDLevDev> tab = nd_levels(c, P, ns);
+    double prev = INFINITY;
+    for (int it = 0;; ++it) {
+        double worst = 0.0;
+        const bool ok = nd_bkwd(c, P, V, s.x, ns, 1e-14, &worst);
+        if (c.verbose > 2)
+            fprintf(stderr, "[qd_nd reuse] %d systems: backward error %.3e after %d steps\n", ns, worst, it);
+        if (ok) return it;
+        if (it == max_steps || !(worst <= 0.5 * prev)) return -1;
+        if (it > 0) {
+            const double need = it + std::log(1e-14 / worst) / std::log(worst / prev);
+            if (!(need <= max_steps)) return -1;
+        }
+        prev = worst;
+        nd_apply(c, P, tab, ns, s.nrow, s.x);
+    }
 }
 
 }  // namespace trt

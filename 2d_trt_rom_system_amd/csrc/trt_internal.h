@@ -66,6 +66,8 @@ struct SweepGeom {
 };
 
 // Batched reduced-QD systems (QD_FV) stored as coefficient arrays on device.
+struct NDPlan;      // nested-dissection plan and fronts of the direct QD solver (qd_nd.hip)
+
 struct QDSystems {
     int nsys = 0, nx = 0, ny = 0;
     size_t nrow = 0;              // unknowns per system: 3 nx ny + nx + ny
@@ -82,6 +84,7 @@ struct QDSystems {
     double *scal = nullptr;       // per-system scalars
     int *flags = nullptr;         // per-system convergence flags (device)
     int red_blocks = 0;
+    NDPlan *nd = nullptr;         // nested-dissection plan, fronts and factors of these systems
     // per system: how many coming solves go straight to the direct solver (set
     // when BiCGSTAB failed on it; the systems of successive iterations are close)
     std::vector<int> direct_next;
@@ -106,7 +109,6 @@ struct RtItLog {
 struct PkState;   // persistent-sweep work list and flags (sweep_pk.hip)
 struct Arena;     // one device allocation carved into contexts' blocks (state.hip)
 struct DirectWork;  // block-LU workspace of the direct QD fallback (qd_direct.hip)
-struct NDPlan;      // nested-dissection plan and fronts of the direct QD solver (qd_nd.hip)
 
 struct Ctx {
     trt_problem p{};
@@ -165,8 +167,9 @@ struct Ctx {
     std::vector<int> newt_kits;        // Krylov count of each solve of the latest EGP_FV_NEWT
     std::vector<RtItLog> itlog;        // iteration history of the latest time step
     DirectWork *direct = nullptr;
-    NDPlan *nd = nullptr;
-    int qd_solver = 0;            // 0: nested dissection; 1: BiCGSTAB with the direct fallback; 2: block LU
+    int qd_solver = 0;            // 0: nested dissection, factors reused within a time step; 1: BiCGSTAB with
+                                  // the direct fallback; 2: block LU; 3: nested dissection, factorizing every solve
+    long long nd_factor = 0, nd_reused = 0, nd_reuse_steps = 0;   // direct-solve statistics
     // POD and DMD bases created on this context: they use its stream and arrays, so
     // the context frees any still open when it is destroyed
     std::vector<trt_pod *> pods;
@@ -311,6 +314,11 @@ void qd_direct(Ctx &c, QDSystems &s, const std::vector<int> &which);
 void direct_destroy(Ctx &c);
 // nested-dissection direct solve of the listed systems of s (solutions replace s.x)
 void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which);
+// the listed systems by iterative refinement with the factors of their last qd_nd:
+// refinement steps taken, or -1 (factors absent or no longer contracting; s.x undefined)
+int qd_nd_reuse(Ctx &c, QDSystems &s, const std::vector<int> &which, int max_steps);
+// no reuse of the factors s holds now (their next solve factorizes afresh)
+void qd_nd_forget(QDSystems &s);
 void nd_destroy(Ctx &c);
 // device copy helper
 void dcopy(Ctx &c, double *dst, const double *src, size_t n);

@@ -44,7 +44,7 @@ EXPORTS = [
     "trt_material_update", "trt_transport_scb", "trt_collapse_intensities", "trt_fg_calc", "trt_cg_calc",
     "trt_mgqd_in_calc", "trt_old_mgqd_coefs", "trt_old_grey_coefs", "trt_mloqd_fv", "trt_collapse_mg_ef",
     "trt_cbar_calc", "trt_grey_coefs", "trt_gqd_in_calc", "trt_egp_fv_newt", "trt_convergence", "trt_step",
-    "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats",
+    "trt_time", "trt_sweep_timed", "trt_sweep_kernel_timed", "trt_calib_copy", "trt_sweep_stats", "trt_qd_stats",
     "trt_sweep_config", "trt_ctx_device_bytes", "trt_tgen_step", "trt_batch_create", "trt_batch_destroy",
     "trt_batch_last_error", "trt_batch_sweep_timed", "trt_batch_tgen_step", "trt_batch_use_graph", "trt_batch_info",
     "trt_batch_create_seg", "trt_batch_segment_length", "trt_autotune_batch", "trt_autotune_segments",
@@ -125,6 +125,7 @@ def load(path: str = LIBPATH):
     L.trt_time.restype = C.c_double
     L.trt_sweep_timed.argtypes = [vp, C.c_int, _dp]
     L.trt_sweep_stats.argtypes = [vp, C.POINTER(C.c_longlong), _dp]
+    L.trt_qd_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     ip = C.POINTER(C.c_int)
     L.trt_sweep_config.argtypes = [vp, ip, ip, ip, ip]
     L.trt_sweep_kernel_timed.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
@@ -387,6 +388,12 @@ class Context:
         n, ms = C.c_longlong(), C.c_double()
         self.L.trt_sweep_stats(self.h, C.byref(n), C.byref(ms))
         return n.value, ms.value
+
+    def qd_stats(self):
+        """(factorized, reused, refinement steps) of the direct QD solves so far."""
+        v = [C.c_longlong() for _ in range(3)]
+        self._check(self.L.trt_qd_stats(self.h, *(C.byref(x) for x in v)), "qd_stats")
+        return tuple(x.value for x in v)
 
     def tgen_step(self) -> None:
         """One Tgen_QDf step (Temp prescribed, one sweep, QD factors)."""

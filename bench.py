@@ -134,23 +134,28 @@ def measure_batch(trt, probs, precision: int, steps: int):
             "autotune_ms_per_sweep": {f"{t[0]}x{t[1]}/seg{sg}": round(v, 3) for (t, sg), v in table.items()}}
 
 
-def measure_steps(trt, precision: int, steps: int = 10):
+def measure_steps(trt, precision: int, steps: int = 10, qd_solver: int = 0):
     """Full MLQD time steps (the drop-in's whole iteration: sweeps, QD factors, MLOQD and
     GLOQD solves, temperature update) of the 256x256, S16, 32-group problem with uniform
-    material: cfg1's lognormal(0, 1.5) map stops the reference algorithm in its first step."""
+    material: cfg1's lognormal(0, 1.5) map stops the reference algorithm in its first step.
+    qd_solver 0 reuses the nested-dissection factors within a step, 3 factorizes every solve."""
     prob = trt.problem.mapped(nx=256, order=16, ng=32, sigma=0.0, nsteps=steps + 1)
-    ctx = trt.native.Context(prob, precision=precision)
+    ctx = trt.native.Context(prob, precision=precision, qd_solver=qd_solver)
     try:
         ctx.step()                                  # warm-up (plans, handles, first factorisations)
+        q0 = ctx.qd_stats()
         its = []
         t0 = time.perf_counter()
         for _ in range(steps):
             its.append(list(ctx.step()))
         dt = (time.perf_counter() - t0) / steps
+        q1 = ctx.qd_stats()
     finally:
         ctx.close()
     sweeps = sum(i[3] for i in its)
-    return {"s_per_step": dt, "steps": steps, "iterations_per_step": its[-1],
+    return {"s_per_step": dt, "steps": steps, "iterations_per_step": its[-1], "qd_solver": qd_solver,
+            "direct_solves_per_step": {"factorized": (q1[0] - q0[0]) / steps, "reused_factors": (q1[1] - q0[1]) / steps,
+                                       "refinement_steps": (q1[2] - q0[2]) / steps},
             "end_to_end_updates_per_s": prob.updates_per_sweep() * sweeps / (dt * steps),
             "note": "iterations of the step: RT, MLOQD, grey Newton, sweeps; end-to-end updates/s = sweep updates "
                     "of the timed steps over their whole wall time (low-order solves included)"}
@@ -468,7 +473,8 @@ def main(argv=None):
         print("[bench] full MLQD time steps (256x256, uniform material)", file=sys.stderr, flush=True)
         out["full_step"] = {"problem": "256x256 cells, S16, 32 groups 1e-3..1e2 keV, uniform material, "
                                        "dt 2e-3 ns, QD systems by nested dissection",
-                            "fp64": measure_steps(trt, 0), "fp32": measure_steps(trt, 1)}
+                            "fp64": measure_steps(trt, 0), "fp32": measure_steps(trt, 1),
+                            "fp64_factorizing_every_solve": measure_steps(trt, 0, qd_solver=3)}
     if not args.no_dropin and not args.only:
         print("[bench] drop-in TRANSPORT_SCB with host arrays (PCIe-inclusive)", file=sys.stderr, flush=True)
         out["dropin_sweep"] = measure_dropin_sweep(trt, prob)

@@ -59,10 +59,14 @@ typedef struct {
                                        sweep; 3 = one launch per tile diagonal with per-row
                                        moment reduction; 4 = one persistent launch over column
                                        segments of tiles                                  */
-    int qd_solver;                  /* reduced QD systems: 0 = nested-dissection direct solve;
-                                       1 = batched BiCGSTAB with the direct solve as fallback;
-                                       2 = block LU over grid rows (the slow direct solve that
-                                       backs up the other two) */
+    int qd_solver;                  /* reduced QD systems: 0 = nested-dissection direct solve whose
+                                       factors are reused within a time step (iterative refinement
+                                       with the last factors of the same systems to the direct
+                                       solve's backward error 1e-14; a new factorization when they no
+                                       longer contract); 1 = batched BiCGSTAB with the direct solve as
+                                       fallback; 2 = block LU over grid rows (the slow direct solve
+                                       that backs up the others); 3 = nested dissection factorizing
+                                       every solve */
     int grey_emission;              /* grey emission of EGP_FV_NEWT: 0 = the reference's analytic
                                        Q_bar = 15 kap0 c aR T / pi^4 (GLOQD_SOLVES.f08:439-440,766);
                                        1 = sum_g 4 pi kappa_g(T) B_g(T) with the cell's opacity
@@ -342,6 +346,9 @@ int trt_sweep_kernel_timed(trt_ctx *ctx, int nsweeps, double *wall_ms, double *k
 int trt_calib_copy(size_t nbytes, int reps, double *ms);
 /* Accumulated sweep statistics since creation: sweeps run and total sweep ms. */
 int trt_sweep_stats(const trt_ctx *ctx, long long *sweeps, double *ms);
+/* Accumulated direct QD-solve statistics since creation (qd_solver 0 and 3): solves that
+ * factorized, solves on reused factors, and the refinement steps those took. */
+int trt_qd_stats(const trt_ctx *ctx, long long *factorized, long long *reused, long long *refine_steps);
 /* The sweep configuration in use: sweep_mode (1, 2, 3 or 4), tile shape, and the
  * persistent launch's grid (workgroups; 0 before the first persistent sweep). */
 int trt_sweep_config(const trt_ctx *ctx, int *mode, int *tile_x, int *tile_y, int *grid);

@@ -167,6 +167,7 @@ def _l2(a, b):
     ("cfg0", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 0),
     ("cfg0", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 1),
     ("map20", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 0),
+    ("map20", dict(use_line_search=0, conv_ho=1e-11, conv_lo=1e-12, conv_gr1=1e-13), 1e-8, 3),
 ])
 def test_time_steps_match(trt, oracle_mod, name, kw, tol, qd_solver):
     prob = {"small": lambda: trt.problem.small(nx=12, ny=10, order=4, ng=3), "cfg0": trt.problem.cfg0,
@@ -186,3 +187,25 @@ def test_time_steps_match(trt, oracle_mod, name, kw, tol, qd_solver):
             assert err < 10 * tol, (s, n, err)
     assert ctx.time == pytest.approx(o.time)
     ctx.close()
+
+
+def test_reused_factors_equal_fresh_factorizations(trt):
+    """qd_solver 0 reuses the nested-dissection factors within a time step (iterative
+    refinement to the direct solve's backward error 1e-14), qd_solver 3 factorizes every
+    solve: three steps of a 24x24 map problem agree to within the iteration tolerances (the
+    two forms differ by rounding, which can move a convergence test by an iteration), and
+    most solves reused."""
+    prob = trt.problem.mapped(nx=24, order=4, ng=6, sigma=1.0)
+    prob.use_line_search, prob.conv_ho, prob.conv_lo, prob.conv_gr1 = 0, 1e-11, 1e-12, 1e-13
+    a = trt.native.Context(prob, qd_solver=0)
+    b = trt.native.Context(prob, qd_solver=3)
+    for s in range(3):
+        a.step()
+        b.step()
+        for n in ("Temp", "E_avg", "HO_E_avg", "MGQD_E_avg", "fg_avg_xx", "Eg_avg"):
+            assert rel_err(a.pull(n), b.pull(n)) < 1e-9, (s, n)
+    fact, reused, steps = a.qd_stats()
+    assert reused > fact > 0 and steps >= reused
+    assert b.qd_stats()[1] == 0
+    a.close()
+    b.close()
