@@ -508,6 +508,16 @@ int trt_batch_tgen_step(trt_batch *b, double *sweep_ms)
     });
 }
 
+int trt_batch_step(trt_batch *b, int *its)
+{
+    return bguarded(b, [&](trt::PkBatch &pb) {
+        std::vector<int> v;
+        trt::batch_step(pb, its ? &v : nullptr);
+        sync_members(pb);
+        if (its) std::copy(v.begin(), v.end(), its);
+    });
+}
+
 int trt_batch_use_graph(trt_batch *b, int on)
 {
     return bguarded(b, [&](trt::PkBatch &pb) {

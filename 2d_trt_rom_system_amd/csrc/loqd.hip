@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <thread>
 
 namespace trt {
 
@@ -1471,10 +1473,14 @@ static void cp(Ctx &c, const char *dst, const char *src)
     dcopy(c, d.p, c.arr(src), d.size());
 }
 
-int step(Ctx &c, int its_out[4], int ho_form)
+// The time step of TRT_MLQD_ALGORITHM in phases, so that a batch can run the sweeps of
+// many problems' RT iterations as one launch: step_begin (the step's prologue), then
+// step_rt_iteration until step_done, then step_end.
+void step_begin(Ctx &c, StepState &st, int ho_form)
 {
-    const int nx = c.nx, ny = c.ny;
-    const size_t nc = (size_t)nx * ny;
+    const size_t nc = (size_t)c.nx * c.ny;
+    st = StepState{};
+    st.ho_form = ho_form;
     c.time += c.p.dt;
     // AL:396-414 (I_crn_old = I_crn is a buffer handover)
     cp(c, "Temp_Old", "Temp");
@@ -1490,99 +1496,181 @@ int step(Ctx &c, int its_out[4], int ho_form)
     old_mgqd_coefs(c);                       // AL:417-419
     old_grey_coefs(c);                       // AL:420
     for (auto n : {"HO_E_avg_RTold2", "HO_E_avg_RTold", "Temp_RTold2", "Temp_RTold"}) dfill(c, c.arr(n), 0.0, nc);
-    int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0;
-    bool rt_conv = false;
     c.itlog.clear();
     // factors are reused within a time step only: a step depends on the state at its
     // start alone, so a run resumed from a restart file continues bitwise
     qd_nd_forget(c.mg);
     qd_nd_forget(c.grey);
-    const int rt_start = 2;                  // AL:302
     // ho_form 1 (the ROM runs, HO_Form 1 and 3, AL:573-631): in each RT iteration the
     // QD factors and C_g of the new time level come from the caller's reconstruction
     // (pushed to the *_next arrays) where POD/DMD_RECONSTRUCT_fg would write them; one RT
     // iteration, as INPUT sets maxit_RTE = 1 for 'fg' ROM runs (IP:94-104)
-    const int maxit_rte = ho_form ? 1 : c.p.maxit_rte;
-    while (!rt_conv && rt_its < maxit_rte) {
-        rt_its++;
-        if (!ho_form && rt_its >= rt_start) {
-            transport_sweep(c);              // AL:541-542
-            sweeps++;
-            collapse_with_fg(c);             // AL:551-553 and fg_Calc AL:568-569
-            if (c.refl) {                    // AL:556-565 (the moments above already hold the
-                rt_bc_update(c);             // sweep's inflow; the update feeds the next sweep)
-                mgqd_in_calc(c);
-                if (!c.p.no_grey) gqd_in_calc(c);
-            }
-            cg_calc(c);                      // AL:572
-        } else if (ho_form) {
-            for (auto n : {"fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy", "Cg_L",
-                           "Cg_B", "Cg_R", "Cg_T"})
-                cp(c, n, (std::string(n) + "_next").c_str());   // AL:582-593
+    st.maxit_rte = ho_form ? 1 : c.p.maxit_rte;
+}
+
+bool step_done(const StepState &st) { return st.rt_conv || st.rt_its >= st.maxit_rte; }
+
+// the next RT iteration starts with a sweep (from the second on, AL:302)
+bool step_sweeps_next(const StepState &st) { return !step_done(st) && !st.ho_form && st.rt_its + 1 >= 2; }
+
+// One RT iteration (AL:527-837).  swept: its sweep has been enqueued already (by a
+// batched sweep that the context's stream waits for).
+void step_rt_iteration(Ctx &c, StepState &st, bool swept)
+{
+    const size_t nc = (size_t)c.nx * c.ny;
+    const bool sweeps = step_sweeps_next(st);
+    st.rt_its++;
+    if (sweeps) {
+        if (!swept) transport_sweep(c);      // AL:541-542
+        st.sweeps++;
+        collapse_with_fg(c);                 // AL:551-553 and fg_Calc AL:568-569
+        if (c.refl) {                        // AL:556-565 (the moments above already hold the
+            rt_bc_update(c);                 // sweep's inflow; the update feeds the next sweep)
+            mgqd_in_calc(c);
+            if (!c.p.no_grey) gqd_in_calc(c);
         }
-        dfill(c, c.arr("MGQD_E_avg_MGQDold2"), 0.0, nc);   // AL:676-679
-        cp(c, "MGQD_E_avg_MGQDold", "HO_E_avg");
-        dfill(c, c.arr("Temp_MGQDold2"), 0.0, nc);
-        cp(c, "Temp_MGQDold", "Temp");
-        int mg_its = 0;
-        bool mg_conv = false;
-        c.itlog.emplace_back();
-        RtItLog &rlog = c.itlog.back();
-        while (!mg_conv && mg_its < c.p.maxit_mloqd) {
-            mg_its++;
-            rlog.mgl.emplace_back();
-            MgItLog &mlog = rlog.mgl.back();
-            mloqd_fv(c, nullptr);            // AL:686-689
-            collapse_mg_ef(c);               // AL:692-693
-            if (c.p.no_grey) {
-                meb_solve(c, "Eg_avg");      // AL:767-770
-            } else {
-                cbar_calc(c);                // AL:709
-                cp(c, "KapE_bar_MGQDold", "KapE_Bar");   // AL:712
-                grey_coefs(c);               // AL:713-715
-                cp(c, "E_avg", "MGQD_E_avg"); cp(c, "E_edgV", "MGQD_E_edgV"); cp(c, "E_edgH", "MGQD_E_edgH");
-                cp(c, "Fx_edgV", "MGQD_Fx_edgV"); cp(c, "Fy_edgH", "MGQD_Fy_edgH");   // AL:717-721
-                mlog.newt = egp_fv_newt(c, mg_its);      // AL:724-728
-                mlog.gkits = c.newt_kits;
-                newt_total += mlog.newt;
-            }
-            material_update(c);              // AL:774
-            int tc = 0, ec = 0;
-            double tn, tr, en, er;
-            convergence(c, c.arr("Temp"), c.arr("Temp_MGQDold"), c.arr("Temp_MGQDold2"), (int)nc, c.p.conv_type,
-                        c.p.conv_lo, mg_its, &tn, &tr, &tc);
-            convergence(c, c.arr("MGQD_E_avg"), c.arr("MGQD_E_avg_MGQDold"), c.arr("MGQD_E_avg_MGQDold2"), (int)nc,
-                        c.p.conv_type, c.p.conv_lo, mg_its, &en, &er, &ec);
-            mg_conv = tc && ec;
-            mlog.tnorm = tn; mlog.trho = tr; mlog.enorm = en; mlog.erho = er;
-            cp(c, "MGQD_E_avg_MGQDold2", "MGQD_E_avg_MGQDold");   // AL:797-800
-            cp(c, "MGQD_E_avg_MGQDold", "MGQD_E_avg");
-            cp(c, "Temp_MGQDold2", "Temp_MGQDold");
-            cp(c, "Temp_MGQDold", "Temp");
+        cg_calc(c);                          // AL:572
+    } else if (st.ho_form) {
+        for (auto n : {"fg_avg_xx", "fg_avg_yy", "fg_edgV_xx", "fg_edgV_xy", "fg_edgH_yy", "fg_edgH_xy", "Cg_L", "Cg_B",
+                       "Cg_R", "Cg_T"})
+            cp(c, n, (std::string(n) + "_next").c_str());   // AL:582-593
+    }
+    dfill(c, c.arr("MGQD_E_avg_MGQDold2"), 0.0, nc);   // AL:676-679
+    cp(c, "MGQD_E_avg_MGQDold", "HO_E_avg");
+    dfill(c, c.arr("Temp_MGQDold2"), 0.0, nc);
+    cp(c, "Temp_MGQDold", "Temp");
+    int mg_its = 0;
+    bool mg_conv = false;
+    c.itlog.emplace_back();
+    RtItLog &rlog = c.itlog.back();
+    while (!mg_conv && mg_its < c.p.maxit_mloqd) {
+        mg_its++;
+        rlog.mgl.emplace_back();
+        MgItLog &mlog = rlog.mgl.back();
+        mloqd_fv(c, nullptr);                // AL:686-689
+        collapse_mg_ef(c);                   // AL:692-693
+        if (c.p.no_grey) {
+            meb_solve(c, "Eg_avg");          // AL:767-770
+        } else {
+            cbar_calc(c);                    // AL:709
+            cp(c, "KapE_bar_MGQDold", "KapE_Bar");   // AL:712
+            grey_coefs(c);                   // AL:713-715
+            cp(c, "E_avg", "MGQD_E_avg"); cp(c, "E_edgV", "MGQD_E_edgV"); cp(c, "E_edgH", "MGQD_E_edgH");
+            cp(c, "Fx_edgV", "MGQD_Fx_edgV"); cp(c, "Fy_edgH", "MGQD_Fy_edgH");   // AL:717-721
+            mlog.newt = egp_fv_newt(c, mg_its);      // AL:724-728
+            mlog.gkits = c.newt_kits;
+            st.newt_total += mlog.newt;
         }
-        mg_total += mg_its;
-        rlog.mg = mg_its;
-        rlog.kits = c.last_kits;
+        material_update(c);                  // AL:774
         int tc = 0, ec = 0;
         double tn, tr, en, er;
-        convergence(c, c.arr("Temp"), c.arr("Temp_RTold"), c.arr("Temp_RTold2"), (int)nc, c.p.conv_type, c.p.conv_ho,
-                    rt_its, &tn, &tr, &tc);
-        convergence(c, c.arr("HO_E_avg"), c.arr("HO_E_avg_RTold"), c.arr("HO_E_avg_RTold2"), (int)nc, c.p.conv_type,
-                    c.p.conv_ho, rt_its, &en, &er, &ec);
-        rt_conv = tc && ec;
-        rlog.tnorm = tn; rlog.trho = tr; rlog.enorm = en; rlog.erho = er;
-        cp(c, "HO_E_avg_RTold2", "HO_E_avg_RTold");   // AL:834-837
-        cp(c, "HO_E_avg_RTold", "HO_E_avg");
-        cp(c, "Temp_RTold2", "Temp_RTold");
-        cp(c, "Temp_RTold", "Temp");
+        convergence(c, c.arr("Temp"), c.arr("Temp_MGQDold"), c.arr("Temp_MGQDold2"), (int)nc, c.p.conv_type,
+                    c.p.conv_lo, mg_its, &tn, &tr, &tc);
+        convergence(c, c.arr("MGQD_E_avg"), c.arr("MGQD_E_avg_MGQDold"), c.arr("MGQD_E_avg_MGQDold2"), (int)nc,
+                    c.p.conv_type, c.p.conv_lo, mg_its, &en, &er, &ec);
+        mg_conv = tc && ec;
+        mlog.tnorm = tn; mlog.trho = tr; mlog.enorm = en; mlog.erho = er;
+        cp(c, "MGQD_E_avg_MGQDold2", "MGQD_E_avg_MGQDold");   // AL:797-800
+        cp(c, "MGQD_E_avg_MGQDold", "MGQD_E_avg");
+        cp(c, "Temp_MGQDold2", "Temp_MGQDold");
+        cp(c, "Temp_MGQDold", "Temp");
     }
+    st.mg_total += mg_its;
+    rlog.mg = mg_its;
+    rlog.kits = c.last_kits;
+    int tc = 0, ec = 0;
+    double tn, tr, en, er;
+    convergence(c, c.arr("Temp"), c.arr("Temp_RTold"), c.arr("Temp_RTold2"), (int)nc, c.p.conv_type, c.p.conv_ho,
+                st.rt_its, &tn, &tr, &tc);
+    convergence(c, c.arr("HO_E_avg"), c.arr("HO_E_avg_RTold"), c.arr("HO_E_avg_RTold2"), (int)nc, c.p.conv_type,
+                c.p.conv_ho, st.rt_its, &en, &er, &ec);
+    st.rt_conv = tc && ec;
+    rlog.tnorm = tn; rlog.trho = tr; rlog.enorm = en; rlog.erho = er;
+    cp(c, "HO_E_avg_RTold2", "HO_E_avg_RTold");   // AL:834-837
+    cp(c, "HO_E_avg_RTold", "HO_E_avg");
+    cp(c, "Temp_RTold2", "Temp_RTold");
+    cp(c, "Temp_RTold", "Temp");
+}
+
+int step_end(Ctx &c, const StepState &st, int its_out[4])
+{
     TRT_CHECK(hipStreamSynchronize(c.stream));
-    if (its_out) { its_out[0] = rt_its; its_out[1] = mg_total; its_out[2] = newt_total; its_out[3] = sweeps; }
+    if (its_out) { its_out[0] = st.rt_its; its_out[1] = st.mg_total; its_out[2] = st.newt_total; its_out[3] = st.sweeps; }
     if (c.verbose)
         fprintf(stderr, "[step] RT %d, MLOQD %d, Newton %d; direct solves so far: %lld factorizations, %lld on reused "
-                "factors (%lld refinement steps)\n", rt_its, mg_total, newt_total, c.nd_factor, c.nd_reused,
+                "factors (%lld refinement steps)\n", st.rt_its, st.mg_total, st.newt_total, c.nd_factor, c.nd_reused,
                 c.nd_reuse_steps);
-    return rt_its;
+    return st.rt_its;
+}
+
+int step(Ctx &c, int its_out[4], int ho_form)
+{
+    StepState st;
+    step_begin(c, st, ho_form);
+    while (!step_done(st)) step_rt_iteration(c, st, false);
+    return step_end(c, st, its_out);
+}
+
+// The low-order state a full step allocates on first use -- the MLOQD and grey QD
+// systems, the MLOQD scratch and the nested-dissection plans and fronts -- allocated
+// now (a request loop admits a solve only when all of it fits).
+void prepare_solves(Ctx &c)
+{
+    if (c.mg.nsys == 0) qd_alloc(c, c.mg, c.ng);
+    if (c.grey.nsys == 0) qd_alloc(c, c.grey, 1);
+    ml_scratch(c);
+    if (c.qd_solver == 0 || c.qd_solver == 3) {
+        qd_nd_prepare(c, c.mg);
+        qd_nd_prepare(c, c.grey);
+    }
+}
+
+// Full time steps of every member of a batch.  The members step in lockstep rounds of
+// RT iterations: each round's sweeps (of the members whose RT iteration starts with one)
+// run as one batched persistent launch, then every member still iterating runs the rest
+// of its RT iteration -- QD factors, the MLOQD and grey Newton iterations, the
+// convergence checks -- on its own stream from a host thread of its own, so the members'
+// low-order solves overlap on the device.  A member's arithmetic is that of step():
+// its results are the same alone and in the batch.
+void batch_step(PkBatch &b, std::vector<int> *its)
+{
+    const std::vector<Ctx *> &cs = pk_batch_ctxs(b);
+    const int n = (int)cs.size();
+    for (Ctx *c : cs)
+        if (c->pk == nullptr) throw Error("batched steps: a member has no tile engine state");
+    std::vector<StepState> st(n);
+    for (int i = 0; i < n; ++i) step_begin(*cs[i], st[i], 0);
+    std::vector<char> sweep(n);
+    std::vector<std::string> errs(n);
+    for (;;) {
+        bool any = false, anysweep = false;
+        for (int i = 0; i < n; ++i) {
+            any = any || !step_done(st[i]);
+            sweep[i] = step_sweeps_next(st[i]);
+            anysweep = anysweep || sweep[i];
+        }
+        if (!any) break;
+        if (anysweep) pk_batch_sweep(b, 3, &sweep);
+        std::vector<std::thread> th;
+        for (int i = 0; i < n; ++i) {
+            if (step_done(st[i])) continue;
+            th.emplace_back([&, i]() {
+                try {
+                    step_rt_iteration(*cs[i], st[i], sweep[i] != 0);
+                } catch (const Error &e) {
+                    errs[i] = e.msg;
+                } catch (const std::exception &e) {
+                    errs[i] = e.what();
+                }
+            });
+        }
+        for (std::thread &t : th) t.join();
+        for (int i = 0; i < n; ++i)
+            if (!errs[i].empty()) throw Error("batched step, member " + std::to_string(i) + ": " + errs[i]);
+        if (anysweep) pk_batch_check(b);
+    }
+    if (its) its->assign((size_t)4 * n, 0);
+    for (int i = 0; i < n; ++i) step_end(*cs[i], st[i], its ? its->data() + 4 * i : nullptr);
 }
 
 // ---------------------------------------------------------------------------

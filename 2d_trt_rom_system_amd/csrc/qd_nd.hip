@@ -997,6 +997,11 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
 // factors of these systems, or when the observed contraction (the last step's ratio of
 // worst backward errors) is above 1/2 or predicts more than max_steps steps in all:
 // refinement stops as soon as it would cost more than it saves.
+void qd_nd_prepare(Ctx &c, QDSystems &s)
+{
+    if (s.nsys > 0) (void)nd_plan_of(c, s);
+}
+
 void qd_nd_forget(QDSystems &s)
 {
     if (s.nd) s.nd->fact_ok = false;

@@ -2,14 +2,19 @@
 // runs one problem per process).
 //
 // A request is a problem (trt_problem), an optional prescribed temperature
-// field and a number of Tgen_QDf steps (ALGORITHMS.f08:1031-1095: one sweep per
-// step with the factors recomputed, no low-order solve).  Requests wait in
-// arrival order.  At each tick the loop
+// field and a number of steps: Tgen_QDf steps (ALGORITHMS.f08:1031-1095: one sweep
+// per step with the factors recomputed, no low-order solve), or in the server's solve
+// mode full time steps of TRT_MLQD_ALGORITHM (a solve: sweeps, QD factors, MLOQD and
+// grey Newton iterations, from the initial state).  Requests wait in arrival order.
+// At each tick the loop
 //   1. admits waiting requests that have arrived, in order, while the device
 //      bytes of the active set stay within the HBM budget and the active count
-//      within its cap (a request larger than the whole budget fails);
-//   2. runs one Tgen_QDf step of every active request, all sweeps in ONE
-//      persistent launch (the batch is rebuilt only when membership changes);
+//      within its cap (a request larger than the whole budget fails; a solve is
+//      admitted with its low-order state allocated);
+//   2. runs one step of every active request, all sweeps of a Tgen_QDf tick, or of
+//      each round of RT iterations of a solve tick, in ONE persistent launch (the
+//      batch is rebuilt only when membership changes), a solve's low-order work on
+//      its own stream from a host thread of its own;
 //   3. retires finished requests: their outputs are copied to host memory and
 //      their device state is freed, so waiting requests can be admitted.
 // Each request keeps its own device state and stream; the batched sweep does
@@ -29,7 +34,7 @@ namespace trt {
 namespace {
 
 // Outputs kept for a finished request (what Tgen_QDf writes per step).
-const char *const kOutputs[] = {"Temp",      "HO_E_avg",   "HO_Eg_avg",  "HO_Eg_edgV", "HO_Eg_edgH",
+const char *const kOutputs[] = {"Temp",      "E_avg",      "HO_E_avg",   "HO_Eg_avg",  "HO_Eg_edgV", "HO_Eg_edgH",
                                 "fg_avg_xx", "fg_avg_yy",  "fg_avg_xy",  "fg_edgV_xx", "fg_edgV_xy",
                                 "fg_edgH_yy", "fg_edgH_xy", "Cg_L",      "Cg_B",       "Cg_R",
                                 "Cg_T"};
@@ -59,6 +64,7 @@ struct Request {
 struct Server {
     size_t budget = 0;
     int max_active = 0;
+    int mode = 0;              // 0: Tgen_QDf steps; 1: full time steps (solves)
     std::vector<Request> reqs;
     // every request's device state lives in one arena of the budget's size,
     // reserved before the first run's clock starts and held until destruction
@@ -211,8 +217,13 @@ void run(trt::Server &s)
             const double ta = now();
             trt_ctx *ctx = nullptr;
             try {
+                if (s.mode == 1 && !r.temp.empty())
+                    throw Error("a solve starts from the initial state (TEMP_INIT): no temperature field");
                 ctx = trt::ctx_create(&r.p, true, arena);
+                if (s.mode == 1) trt::prepare_solves(ctx->c);
             } catch (const Error &e) {
+                if (ctx) trt::ctx_destroy(ctx);
+                ctx = nullptr;
                 const bool nomem = e.msg.find("arena exhausted") != std::string::npos;
                 if (nomem && !(active.empty() && draining.empty())) break;   // wait for memory
                 fail(r, nomem ? std::string("request needs more device memory than the whole budget")
@@ -270,9 +281,14 @@ void run(trt::Server &s)
             }
             const double ts = now();
             const std::vector<size_t> ticked = active;
-            for (size_t i : ticked) trt::tgen_pre(s.reqs[i].ctx->c);
-            trt::pk_batch_sweep(*batch, 3);
-            for (size_t i : ticked) trt::tgen_post(s.reqs[i].ctx->c);
+            std::vector<int> its;
+            if (s.mode == 1) {
+                trt::batch_step(*batch, &its);
+            } else {
+                for (size_t i : ticked) trt::tgen_pre(s.reqs[i].ctx->c);
+                trt::pk_batch_sweep(*batch, 3);
+                for (size_t i : ticked) trt::tgen_post(s.reqs[i].ctx->c);
+            }
             // host work while the tick runs: arrivals join the next tick, finished copies complete
             const double th = now();
             admit();
@@ -292,14 +308,16 @@ void run(trt::Server &s)
             std::vector<size_t> still;
             for (size_t i : active) {
                 Request &r = s.reqs[i];
-                if (std::find(ticked.begin(), ticked.end(), i) == ticked.end()) {
+                const auto pos = std::find(ticked.begin(), ticked.end(), i);
+                if (pos == ticked.end()) {
                     still.push_back(i);        // admitted during this tick
                     continue;
                 }
                 const trt::Ctx &c = r.ctx->c;
                 r.bytes = c.bytes;
                 r.done_steps += 1;
-                const long long u = (long long)c.nx * c.ny * c.nm * c.ng;
+                const long long sweeps = s.mode == 1 ? its[4 * (size_t)(pos - ticked.begin()) + 3] : 1;
+                const long long u = (long long)c.nx * c.ny * c.nm * c.ng * sweeps;
                 r.updates += u;
                 s.updates += u;
                 if (r.done_steps >= r.steps) {
@@ -372,6 +390,14 @@ int trt_server_destroy(trt_server *sv)
 }
 
 const char *trt_server_last_error(const trt_server *sv) { return sv ? sv->err.c_str() : ""; }
+
+int trt_server_set_mode(trt_server *sv, int mode)
+{
+    return sguarded(sv, [&](trt::Server &s) {
+        if (mode != 0 && mode != 1) throw Error("server mode must be 0 (Tgen_QDf steps) or 1 (full time steps)");
+        s.mode = mode;
+    });
+}
 
 int trt_server_submit(trt_server *sv, const trt_problem *p, const double *temp, int steps, double arrival_s,
                       int *id)

@@ -69,6 +69,7 @@ struct PkProb {
     const double *ctab;    // [m][6]: w, w ox^2, w oy^2, w ox oy, w ox, w oy, over the stored intensities' factor
     int tx, ty;            // tile shape
     int refl;              // reflective sides: bit 0 left, 1 bottom, 2 right, 3 top
+    int skip;              // batched sweeps: this problem is not swept this time (its items only advance the queue)
     void *BV, *BH;         // their boundary slices: [left, right][q][NTY][NL][2 TY], [bottom, top][q][NTX][NL][2 TX]
 };
 
@@ -612,6 +613,16 @@ __global__ __launch_bounds__(PK_THREADS, (PkOccupancy<TI, TC, TX, TY>::waves)) v
             if (tid == 0) s_pi = pi;
         }
         const PkProb &P = *reinterpret_cast<const PkProb *>(s_prob);
+        if (P.skip) {                     // a problem left out of this sweep: no wait, no work
+            __syncthreads();              // every wave has read s_item
+            if (tid == 0) {
+                s_next = nxt;
+                s_item = nit;
+            }
+            __syncthreads();
+            cur = __builtin_amdgcn_readfirstlane(s_next);
+            continue;
+        }
         double *xs = dyn;                                           // cross-wave sums
         TI *st = reinterpret_cast<TI *>(dyn + P.xsw);             // staged corner averages
         if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
@@ -704,7 +715,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
         double *xs = dyn;
         TI *st = reinterpret_cast<TI *>(dyn + P.xsw);
         if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
-        const int t0 = sg * P.seg, t1 = min(t0 + P.seg, P.NTY);
+        const int t0 = sg * P.seg, t1 = P.skip ? t0 : min(t0 + P.seg, P.NTY);   // a skipped problem: no tiles
         TI hc[TX][2];
         for (int tyl = t0; tyl < t1; ++tyl) {
             const int fl = fl0 + (tyl - t0) * P.NTX;
@@ -721,6 +732,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
             tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
             __syncthreads();
         }
+        if (P.skip) __syncthreads();      // every wave has read s_item
         if (tid == 0) {
             s_next = nxt;
             s_item = nit;
@@ -754,6 +766,7 @@ template <typename TI>
 __global__ void pk_boundary_moments(const PkProb *probs)
 {
     const PkProb &P = probs[blockIdx.y];
+    if (P.skip) return;
     const int nx = P.nx, ny = P.ny, ng = P.ng, K = P.K;
     const int per = ny + nx;
     const int tot = 4 * ng * per;
@@ -1393,16 +1406,20 @@ static void batch_enqueue(PkBatch &b, const PkProb *table = nullptr)
 // its moments, after the work already enqueued on each problem's stream; each
 // problem's stream waits for the sweep.  Events t0 (mark & 1) and t1 (mark & 2)
 // are recorded on the batch stream before and after the sweep.
-void pk_batch_sweep(PkBatch &b, int mark)
+void pk_batch_sweep(PkBatch &b, int mark, const std::vector<char> *sweep)
 {
     const int n = (int)b.ctxs.size();
+    if (sweep && (int)sweep->size() != n) throw Error("batched sweep: the member mask has the wrong length");
+    auto swept = [&](int i) { return !sweep || (*sweep)[i]; };
     bool dirty = false;
     for (int i = 0; i < n; ++i) {
         Ctx &c = *b.ctxs[i];
-        if (b.up_iold[i] != c.iold) {
+        const int skip = swept(i) ? 0 : 1;
+        if (b.up_iold[i] != c.iold || b.host[i].skip != skip) {
             b.host[i] = c.pk->host[c.iold];
             b.host[i].seg = b.seg_of[i];
             b.host[i].nseg = (b.host[i].NTY + b.seg_of[i] - 1) / b.seg_of[i];
+            b.host[i].skip = skip;
             b.up_iold[i] = c.iold;
             dirty = true;
         }
@@ -1415,6 +1432,7 @@ void pk_batch_sweep(PkBatch &b, int mark)
         b.upload_pending = true;
     }
     for (int i = 0; i < n; ++i) {
+        if (!swept(i)) continue;
         TRT_CHECK(hipEventRecord(b.ready[i], b.ctxs[i]->stream));
         TRT_CHECK(hipStreamWaitEvent(b.stream, b.ready[i], 0));
     }
@@ -1434,6 +1452,7 @@ void pk_batch_sweep(PkBatch &b, int mark)
     TRT_CHECK(hipGetLastError());
     TRT_CHECK(hipEventRecord(b.done, b.stream));
     for (int i = 0; i < n; ++i) {
+        if (!swept(i)) continue;
         Ctx &c = *b.ctxs[i];
         TRT_CHECK(hipStreamWaitEvent(c.stream, b.done, 0));
         c.icur = 1 - c.iold;

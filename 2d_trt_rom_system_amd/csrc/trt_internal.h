@@ -229,6 +229,23 @@ void convergence(Ctx &c, const double *x, const double *xo, const double *xo2, i
                  double eps, int its, double *norm, double *rho, int *conv);
 // ho_form 0: full-order step; 1: the ROM runs' step (QD factors and C_g from the context, no sweep)
 int step(Ctx &c, int its_out[4], int ho_form = 0);
+// the same step in phases (loqd.hip): a batch interleaves them to sweep many problems at once
+struct StepState {
+    int ho_form = 0, maxit_rte = 0;
+    int rt_its = 0, mg_total = 0, newt_total = 0, sweeps = 0;
+    bool rt_conv = false;
+};
+void step_begin(Ctx &c, StepState &st, int ho_form);
+bool step_done(const StepState &st);
+bool step_sweeps_next(const StepState &st);
+void step_rt_iteration(Ctx &c, StepState &st, bool swept);
+int step_end(Ctx &c, const StepState &st, int its_out[4]);
+// allocates the low-order state (QD systems, scratch, nested-dissection plans) of full steps
+void prepare_solves(Ctx &c);
+// full steps of every member of a batch, the sweeps of each round of RT iterations in one
+// launch; its (may be null) receives RT, MLOQD, Newton and sweep counts per member
+struct PkBatch;
+void batch_step(PkBatch &b, std::vector<int> *its);
 void pull_icrn(Ctx &c, int which, double *host);
 void push_icrn(Ctx &c, int which, const double *host);
 // RESTART_OUT / RESTART_IN of the context's state (csrc/restart.hip); restart_in
@@ -262,7 +279,8 @@ struct PkBatch;
 PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled = false, int seg = 0);
 int pk_batch_seg(const PkBatch &b);
 void pk_batch_destroy(PkBatch *b);
-void pk_batch_sweep(PkBatch &b, int mark);
+// sweep (may be null: every member): the members swept this time, the others skipped
+void pk_batch_sweep(PkBatch &b, int mark, const std::vector<char> *sweep = nullptr);
 // mode 0: eager; 1: the batched sweep as a graph; 2: whole Tgen_QDf steps as graphs
 void pk_batch_use_graph(PkBatch &b, int mode);
 bool pk_batch_step_graphs(const PkBatch &b);
@@ -319,6 +337,8 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which);
 int qd_nd_reuse(Ctx &c, QDSystems &s, const std::vector<int> &which, int max_steps);
 // no reuse of the factors s holds now (their next solve factorizes afresh)
 void qd_nd_forget(QDSystems &s);
+// the plan (tree, fronts, factor storage) of s, created now if it does not exist
+void qd_nd_prepare(Ctx &c, QDSystems &s);
 void nd_destroy(Ctx &c);
 // device copy helper
 void dcopy(Ctx &c, double *dst, const double *src, size_t n);

@@ -57,7 +57,7 @@ EXPORTS = [
     "trt_convergence_noalloc", "trt_meb_solve", "trt_outfile_create", "trt_timestep_outs", "trt_tgen_outs",
     "trt_outfile_put", "trt_outfile_close", "trt_outfile_last_error", "trt_nc_open", "trt_nc_dim", "trt_nc_get",
     "trt_nc_last_error", "trt_nc_close", "trt_nc_att", "trt_romgrid_create", "trt_romgrid_tmap", "trt_romgrid_map",
-    "trt_romgrid_maps", "trt_romgrid_destroy", "trt_rt_bc_update",
+    "trt_romgrid_maps", "trt_romgrid_destroy", "trt_rt_bc_update", "trt_batch_step", "trt_server_set_mode",
 ]
 
 _lib = None
@@ -141,6 +141,8 @@ def load(path: str = LIBPATH):
     L.trt_batch_last_error.restype = C.c_char_p
     L.trt_batch_sweep_timed.argtypes = [vp, C.c_int, _dp]
     L.trt_batch_tgen_step.argtypes = [vp, _dp]
+    L.trt_batch_step.argtypes = [vp, C.POINTER(C.c_int)]
+    L.trt_server_set_mode.argtypes = [vp, C.c_int]
     L.trt_batch_use_graph.argtypes = [vp, C.c_int]
     L.trt_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), ip, ip]
     L.trt_server_create.argtypes = [C.POINTER(vp), C.c_size_t, C.c_int]
@@ -452,6 +454,13 @@ class Batch:
         ms = C.c_double()
         self._check(self.L.trt_batch_tgen_step(self.h, C.byref(ms)), "batch_tgen_step")
         return ms.value
+
+    def step(self):
+        """One full time step of every member (trt_batch_step): per member (RT, MLOQD,
+        grey Newton, sweeps)."""
+        its = (C.c_int * (4 * len(self.ctxs)))()
+        self._check(self.L.trt_batch_step(self.h, its), "batch_step")
+        return [tuple(its[4 * i:4 * i + 4]) for i in range(len(self.ctxs))]
 
     def info(self):
         """(updates per batched sweep, persistent grid, work items)."""

@@ -2,9 +2,10 @@
 load generator that reports latency percentiles beside throughput.
 
 A request is a problem, an optional prescribed temperature field and a number
-of Tgen_QDf steps (MainSrc/ALGORITHMS.f08:1031-1095, one sweep per step).  The
-loop itself is native (csrc/serve.hip); this module submits requests, runs the
-loop and summarises it:
+of Tgen_QDf steps (MainSrc/ALGORITHMS.f08:1031-1095, one sweep per step), or, in
+solve mode, a number of full time steps of TRT_MLQD_ALGORITHM from the initial
+state.  The loop itself is native (csrc/serve.hip); this module submits requests,
+runs the loop and summarises it:
 
     python -m 2d_trt_rom_system_amd.serve --rates 2 4 8 --requests 32
 """
@@ -24,7 +25,9 @@ from . import native, problem
 class Server:
     """A trt_server: submit requests, run them, read stats and outputs."""
 
-    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, precision: int = 1, tile=(4, 4)):
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, precision: int = 1, tile=(4, 4),
+                 solves: bool = False):
+        """solves: requests are full time steps (trt_server_set_mode 1), not Tgen_QDf steps."""
         self.L = native.load()
         self.precision = precision
         self.tile = tile
@@ -33,6 +36,8 @@ class Server:
             raise native.TrtError("trt_server_create failed")
         self.h = h
         self.ids = []
+        if solves:
+            self._check(self.L.trt_server_set_mode(self.h, 1), "server_set_mode")
 
     def close(self):
         if getattr(self, "h", None):
@@ -107,12 +112,13 @@ def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
 
 
 def run_load(rate: float, n: int, steps: int = 20, budget_gib: float = 200.0, precision: int = 1, tile=(4, 4),
-             base_seed: int = 20240611, arrival_seed: int = 7, max_active: int = 64, problems=None) -> dict:
+             base_seed: int = 20240611, arrival_seed: int = 7, max_active: int = 64, problems=None,
+             solves: bool = False) -> dict:
     """Serve n requests (problems of the batched distribution unless given)
     arriving as a Poisson process; returns latency and throughput figures."""
     arr = poisson_arrivals(rate, n, arrival_seed)
     probs = problems if problems is not None else [problem.cfg3_problem(base_seed, i, nsteps=steps) for i in range(n)]
-    with Server(int(budget_gib * 2**30), max_active, precision, tile) as srv:
+    with Server(int(budget_gib * 2**30), max_active, precision, tile, solves=solves) as srv:
         ids = [srv.submit(p, steps, float(a)) for p, a in zip(probs, arr)]
         srv.run()
         reqs = [srv.request(i) for i in ids]

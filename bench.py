@@ -179,6 +179,85 @@ def measure_waves(trt, budget_gib: float = 200.0, steps: int = 20, n: int = 64):
     return sv
 
 
+def solve_draws(trt, n: int = 8, nmax: int = 64, base_seed: int = 20240611):
+    """Draws of the batched distribution (in draw order) of at most nmax^2 cells, with
+    uniform material: the distribution's lognormal(0, 1.5) maps stop full MLQD steps of the
+    reference algorithm (negative group energies at the opacity jumps), and 128^2 S8 draws
+    reach the cold-cell 0/0 of fg_Calc within three steps; the 64^2 draws run."""
+    import numpy as np
+    out, i = [], 0
+    while len(out) < n and i < 64:
+        p = trt.problem.cfg3_problem(base_seed, i)
+        if p.nx <= nmax:
+            out.append(p.with_(kmult=np.ones(len(p.kmult))))
+        i += 1
+    return out
+
+
+def measure_batched_solves(trt, steps: int = 5, n: int = 8):
+    """Full MLQD time steps of n draws (64^2, S8..S16, 16..64 groups, uniform material):
+    batched (trt_batch_step: each round of RT iterations' sweeps in one launch, every
+    member's low-order work on its own stream from its own host thread) against the same
+    problems stepped one at a time, FP32 sweep; and one at a time in plain FP64."""
+    probs = solve_draws(trt, n)
+    upd = 0
+    res = {"problems": [f"{p.nx}x{p.ny} S{p.order} G{p.ng}" for p in probs], "steps": steps}
+    for prec, key in ((1, "fp32_one_at_a_time"), (0, "fp64_one_at_a_time")):
+        t = 0.0
+        upd = 0
+        for p in probs:
+            ctx = trt.native.Context(p, precision=prec, tile=(4, 4) if prec else (0, 0))
+            try:
+                ctx.step()                            # warm-up: plans and handles (not timed)
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    upd += p.updates_per_sweep() * ctx.step()[3]
+                t += time.perf_counter() - t0
+            finally:
+                ctx.close()
+        res[key] = {"wall_s": t, "updates_per_s": upd / t, "solves_steps_per_s": len(probs) * steps / t}
+    ctxs = [trt.native.Context(p, precision=1, tile=(4, 4)) for p in probs]
+    try:
+        with trt.native.Batch(ctxs) as b:
+            b.step()                                  # warm-up
+            t0 = time.perf_counter()
+            upd = 0
+            for _ in range(steps):
+                its = b.step()
+                upd += sum(p.updates_per_sweep() * i[3] for p, i in zip(probs, its))
+            t = time.perf_counter() - t0
+    finally:
+        for c in ctxs:
+            c.close()
+    res["fp32_batched"] = {"wall_s": t, "updates_per_s": upd / t, "solves_steps_per_s": len(probs) * steps / t}
+    res["batched_over_one_at_a_time_fp32"] = res["fp32_one_at_a_time"]["wall_s"] / t
+    res["batched_fp32_over_plain_fp64"] = res["fp64_one_at_a_time"]["wall_s"] / t
+    res["note"] = ("full time steps (sweeps, QD factors, MLOQD and grey Newton iterations); updates_per_s = sweep "
+                   "updates over the wall time of the steps, low-order solves included; results of the batched "
+                   "members are bitwise those of the same problems alone (tests/test_gpu_batch_steps.py)")
+    return res
+
+
+def measure_served_solves(trt, n: int = 8, steps: int = 5, budget_gib: float = 64.0):
+    """The request loop in solve mode: n requests (the draws of measure_batched_solves),
+    each `steps` full time steps, at Poisson rates that are fractions of the completion rate
+    of the all-at-once run, FP32 sweep."""
+    probs = solve_draws(trt, n)
+    out = {"rates": []}
+    allat = trt.serve.run_load(1e9, n, steps=steps, budget_gib=budget_gib, problems=probs, solves=True)
+    out["all_at_once"] = allat
+    sat = allat["completed_per_s"]
+    for f in (0.25, 0.5, 1.0):
+        r = round(f * sat, 3)
+        print(f"[bench] served solves, {n} requests at {r:g}/s", file=sys.stderr, flush=True)
+        out["rates"].append(trt.serve.run_load(r, n, steps=steps, budget_gib=budget_gib, problems=probs,
+                                               solves=True))
+    out["note"] = (f"requests = {steps} full MLQD time steps from the initial state (solve mode), the draws of "
+                   "batched_solves, Poisson arrivals at 0.25, 0.5 and 1.0 of the all-at-once completion rate, "
+                   f"HBM budget {budget_gib:g} GiB, FP32 sweep; latency = completion - arrival")
+    return out
+
+
 def measure_serving(trt, rates, n: int, steps: int = 20, budget_gib: float = 200.0):
     """Poisson arrivals at each rate (requests/s) of requests drawn from the batched
     distribution, FP32 continuous batching; then the same requests solved one at a time in
@@ -469,6 +548,12 @@ def main(argv=None):
                       "batching; latency = completion - arrival; rates are fractions 0.25..1.5 of the completion "
                       "rate of the all-at-once run")
         out["serving"] = sv
+    if not args.no_steps and not args.no_batch and not args.only:
+        print("[bench] batched full time steps (solves)", file=sys.stderr, flush=True)
+        out["batched_solves"] = measure_batched_solves(trt)
+    if not args.no_steps and not args.no_serve and not args.only:
+        print("[bench] served solves", file=sys.stderr, flush=True)
+        out["served_solves"] = measure_served_solves(trt)
     if not args.no_steps and not args.only:
         print("[bench] full MLQD time steps (256x256, uniform material)", file=sys.stderr, flush=True)
         out["full_step"] = {"problem": "256x256 cells, S16, 32 groups 1e-3..1e2 keV, uniform material, "

@@ -389,6 +389,14 @@ int trt_batch_tgen_step(trt_batch *b, double *sweep_ms);
  * for a batch stepping together) and replayed; 0 = eager.  Results are bitwise the same
  * in every mode. */
 int trt_batch_use_graph(trt_batch *b, int on);
+/* One full time step of TRT_MLQD_ALGORITHM (as trt_step) of every member.  The members
+ * iterate in lockstep rounds of RT iterations: each round's sweeps, of the members whose
+ * RT iteration starts with one, run as one persistent launch (the others' work items are
+ * skipped); then every member still iterating runs the rest of its RT iteration -- QD
+ * factors, MLOQD and grey Newton iterations, convergence checks -- on its own stream from
+ * a host thread of its own.  its (may be NULL) receives 4 counts per member (RT, MLOQD,
+ * grey Newton, sweeps).  A member's results are bitwise those of trt_step on it alone. */
+int trt_batch_step(trt_batch *b, int *its);
 int trt_batch_info(const trt_batch *b, long long *updates_per_sweep, int *grid, int *nitems);
 
 /* Tile-shape autotuner (no counterpart in the reference).  Creates contexts
@@ -418,7 +426,13 @@ const char *trt_autotune_last_error(void);
  * within max_active, runs one Tgen_QDf step of every active request with all
  * sweeps in one persistent launch, and retires finished requests (outputs
  * copied to host, device state freed).  A request larger than the whole budget
- * fails (status -1).  The problem's vectors are copied at submission. */
+ * fails (status -1).  The problem's vectors are copied at submission.
+ * In solve mode (trt_server_set_mode 1) a request's steps are full time steps of
+ * TRT_MLQD_ALGORITHM from the initial state (as trt_step; no temperature field):
+ * admission allocates its low-order state too, and each tick is trt_batch_step over
+ * the active requests (each round of RT iterations' sweeps in one launch, every
+ * request's low-order work on its own stream).  A request's results are the same
+ * alone and in any mix, in both modes. */
 typedef struct trt_server trt_server;
 typedef struct {
     double arrival_s, admit_s, done_s;  /* seconds from the start of the run; -1 if never */
@@ -440,6 +454,8 @@ typedef struct {
                                            trt_server_destroy) */
 } trt_server_stats;
 int trt_server_create(trt_server **out, size_t hbm_budget, int max_active);
+/* 0 (default): requests are Tgen_QDf steps; 1: requests are solves (full time steps) */
+int trt_server_set_mode(trt_server *s, int mode);
 int trt_server_destroy(trt_server *s);
 const char *trt_server_last_error(const trt_server *s);
 int trt_server_submit(trt_server *s, const trt_problem *p, const double *temp, int steps, double arrival_s,
