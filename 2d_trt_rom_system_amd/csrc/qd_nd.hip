@@ -465,8 +465,35 @@ __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int n
 }
 
 // The forward solve phase on a level of small fronts with the LU factors nd_front_small
-// left (one wavefront per front, lane t holding entry t): z_I = A_II^-1 f_I by the row
-// permutation, the unit lower and the upper triangular solves, then f_B -= F_BI z_I.
+// left (one wavefront per front, lane t holding row t of the front in registers, all of
+// it loaded at once): z_I = A_II^-1 f_I by the row permutation, the unit lower and the
+// upper triangular solves, then f_B -= F_BI z_I, the pivots' values broadcast by lane
+// shuffles.  The template steps keep every register index constant.
+template <int K>
+__device__ __forceinline__ void nd_lu_fwd_step(const double (&r)[ND_SI], double &y, int t, int nIp)
+{
+    if constexpr (K < ND_SI) {
+        if (K < nIp) {
+            const double yk = __shfl(y, K);
+            if (t > K && t < nIp) y = fma(-r[K], yk, y);
+            nd_lu_fwd_step<K + 1>(r, y, t, nIp);
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void nd_lu_bwd_step(const double (&r)[ND_SI], double &y, int t, int nIp)
+{
+    if constexpr (K >= 0) {
+        if (K < nIp) {
+            if (t == K) y = y / r[K];
+            const double yk = __shfl(y, K);
+            if (t < K) y = fma(-r[K], yk, y);
+        }
+        nd_lu_bwd_step<K - 1>(r, y, t, nIp);
+    }
+}
+
 __global__ __launch_bounds__(64) void nd_fwd_lu_small(const double *F, double *f, const int *perm, int nIp, int nBp,
                                                       int nF)
 {
@@ -474,23 +501,24 @@ __global__ __launch_bounds__(64) void nd_fwd_lu_small(const double *F, double *f
     const double *A = F + fr * nF * nF;
     double *v = f + fr * nF;
     const int t = threadIdx.x;
-    double y = t < nIp ? v[perm[fr * nIp + t]] : 0.0;
-    for (int k = 0; k < nIp; ++k) {               // unit lower: column k below the diagonal
-        const double yk = __shfl(y, k);
-        if (t > k && t < nIp) y = fma(-A[t + (size_t)k * nF], yk, y);
+    const bool live = t < nF;
+    double r[ND_SI];                              // row t: L and U (pivot rows) or F_BI (border rows)
+#pragma unroll
+    for (int k = 0; k < ND_SI; ++k) r[k] = (live && k < nIp) ? A[t + (size_t)k * nF] : 0.0;
+    double y = t < nIp ? v[perm[fr * nIp + t]] : (live ? v[t] : 0.0);
+    double z = t < nIp ? y : 0.0;
+    nd_lu_fwd_step<0>(r, z, t, nIp);
+    nd_lu_bwd_step<ND_SI - 1>(r, z, t, nIp);
+    // border rows: f_B -= F_BI z_I
+#pragma unroll
+    for (int k = 0; k < ND_SI; ++k) {
+        if (k < nIp) {
+            const double zk = __shfl(z, k);
+            if (t >= nIp) y = fma(-r[k], zk, y);
+        }
     }
-    for (int k = nIp - 1; k >= 0; --k) {          // upper: x_k = y_k / U_kk, then column k above it
-        if (t == k) y = y / A[(size_t)k + (size_t)k * nF];
-        const double yk = __shfl(y, k);
-        if (t < k) y = fma(-A[t + (size_t)k * nF], yk, y);
-    }
-    if (t < nIp) v[t] = y;
-    double acc = t < nBp ? v[nIp + t] : 0.0;
-    for (int j = 0; j < nIp; ++j) {
-        const double zj = __shfl(y, j);
-        if (t < nBp) acc = fma(-A[nIp + t + (size_t)j * nF], zj, acc);
-    }
-    if (t < nBp) v[nIp + t] = acc;
+    if (t < nIp) v[t] = z;
+    else if (live) v[t] = y;
 }
 
 __global__ void nd_info_max(const rocblas_int *info, size_t n, int *out)
