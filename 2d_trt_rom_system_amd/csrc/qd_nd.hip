@@ -273,16 +273,29 @@ __global__ __launch_bounds__(256) void nd_fwd_inv(const double *F, double *f, in
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) sh[i] = fb[i];
     __syncthreads();
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
-        double acc = 0.0;
-        for (int j = 0; j < nIp; ++j) acc += Fb[i + (size_t)j * nF] * sh[j];
+        double a0 = 0.0, a1 = 0.0;                // two chains: more loads in flight
+        int j = 0;
+#pragma unroll 4
+        for (; j + 1 < nIp; j += 2) {
+            a0 = fma(Fb[i + (size_t)j * nF], sh[j], a0);
+            a1 = fma(Fb[i + (size_t)(j + 1) * nF], sh[j + 1], a1);
+        }
+        if (j < nIp) a0 = fma(Fb[i + (size_t)j * nF], sh[j], a0);
+        const double acc = a0 + a1;
         sh[nIp + i] = acc;
         fb[i] = acc;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nBp; q += blockDim.x) {
-        double acc = fb[nIp + q];
-        for (int j = 0; j < nIp; ++j) acc -= Fb[nIp + q + (size_t)j * nF] * sh[nIp + j];
-        fb[nIp + q] = acc;
+        double a0 = fb[nIp + q], a1 = 0.0;
+        int j = 0;
+#pragma unroll 4
+        for (; j + 1 < nIp; j += 2) {
+            a0 = fma(-Fb[nIp + q + (size_t)j * nF], sh[nIp + j], a0);
+            a1 = fma(-Fb[nIp + q + (size_t)(j + 1) * nF], sh[nIp + j + 1], a1);
+        }
+        if (j < nIp) a0 = fma(-Fb[nIp + q + (size_t)j * nF], sh[nIp + j], a0);
+        fb[nIp + q] = a0 + a1;
     }
 }
 
@@ -294,9 +307,15 @@ __global__ __launch_bounds__(256) void nd_bwd_inv(const double *F, double *f, in
     for (int q = threadIdx.x; q < nBp; q += blockDim.x) sh[q] = fb[nIp + q];
     __syncthreads();
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
-        double acc = fb[i];
-        for (int q = 0; q < nBp; ++q) acc -= Fb[i + (size_t)(nIp + q) * nF] * sh[q];
-        fb[i] = acc;
+        double a0 = fb[i], a1 = 0.0;
+        int q = 0;
+#pragma unroll 4
+        for (; q + 1 < nBp; q += 2) {
+            a0 = fma(-Fb[i + (size_t)(nIp + q) * nF], sh[q], a0);
+            a1 = fma(-Fb[i + (size_t)(nIp + q + 1) * nF], sh[q + 1], a1);
+        }
+        if (q < nBp) a0 = fma(-Fb[i + (size_t)(nIp + q) * nF], sh[q], a0);
+        fb[i] = a0 + a1;
     }
 }
 
