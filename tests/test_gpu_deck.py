@@ -1,13 +1,13 @@
 """Time steps of problems read from input decks, GPU against the oracle: the reference's
 own deck (tests/golden/mainsrc_input.inp: Fleck-Cummings layout, abu36 quadrature, JCP
-groups, 1 keV left boundary, 10x10 cells) run as mlqd, and a BLOCK layout with abu20
-and NE795 groups."""
+groups, 1 keV left boundary, 10x10 cells) run as mlqd, the same deck with reflective bottom
+and top sides, and a BLOCK layout with abu20 and NE795 groups."""
 import importlib
 
 import numpy as np
 import pytest
 
-from conftest import ROOT, rel_err
+from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -28,10 +28,20 @@ def _block_deck():
     return d
 
 
+def _reflective_deck():
+    """The reference's deck with reflective bottom and top sides (bc_type 0 1 0 1)."""
+    text = open(GOLDEN).read()
+    assert "bc_type   0 0 0 0" in text
+    d = D.read_deck(text.replace("bc_type   0 0 0 0", "bc_type   0 1 0 1"))
+    assert tuple(d.bc_type) == (0, 1, 0, 1)
+    return d
+
+
 @pytest.mark.parametrize("which,line_search,tol", [("reference", False, 1e-8), ("reference", True, 1e-5),
-                                                   ("block", False, 1e-8)])
+                                                   ("block", False, 1e-8), ("reflective", False, 1e-8)])
 def test_deck_steps_match_oracle(trt, oracle_mod, which, line_search, tol):
-    d = D.read_deck_file(GOLDEN) if which == "reference" else _block_deck()
+    d = {"reference": lambda: D.read_deck_file(GOLDEN), "block": _block_deck,
+         "reflective": _reflective_deck}[which]()
     d.run_type = "mlqd"
     prob = d.problem(nsteps=3)
     if not line_search:
@@ -45,6 +55,9 @@ def test_deck_steps_match_oracle(trt, oracle_mod, which, line_search, tol):
             err = _l2(ctx.pull(n), o[n].ravel(order="F"))
             assert err < tol, (s, n, err, it_o, it_g)
         for n in ("fg_avg_xx", "fg_avg_yy", "fg_avg_xy"):
-            assert rel_err(ctx.pull(n), o[n].ravel(order="F")) < 10 * tol, (s, n)
+            # Eddington factors are O(1) ratios: absolute error (with reflective bottom and
+            # top the problem is one-dimensional, and f_xy is rounding noise around zero)
+            g, r = ctx.pull(n), o[n].ravel(order="F")
+            assert np.max(np.abs(g - r)) < 10 * tol * max(1.0, np.max(np.abs(r))), (s, n)
     assert ctx.time == pytest.approx(o.time) == pytest.approx(3 * d.delt)
     ctx.close()
