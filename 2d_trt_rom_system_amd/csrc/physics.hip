@@ -69,17 +69,18 @@ void material_update(Ctx &c)
 // COLLAPSE_INTENSITIES: sum the 4 quadrant partials of every moment, and
 // optionally fg_Calc's ratios (MLOQD_SOLVES.f08:646-674) in the same pass.
 //
-// The partials are [location][quadrant][group][moment]; the outputs are the
-// reference's (location, group) arrays with the location fastest.  A block
-// takes 64 locations x 4 groups of one location kind: it reads the partials
-// with 4 adjacent lanes covering 4 adjacent groups (128 contiguous bytes per
-// location and quadrant), stages the results in LDS, and writes each output
-// as 64 consecutive locations per group.
+// The partials are stored by work item of the sweep (PartLayout: a cell's value v of
+// quadrant q sits in q's item of the cell's tile, adjacent groups of a group block in
+// adjacent doubles); the outputs are the reference's (location, group) arrays with the
+// location fastest.  A block takes 64 locations x 4 groups of one location kind: it
+// reads the partials with 4 adjacent lanes covering 4 adjacent groups, stages the
+// results in LDS, and writes each output as 64 consecutive locations per group.
 // ---------------------------------------------------------------------------
 struct CollArgs {
     int nx, ny, ng;
     double cinv, c;
-    const double *Pavg, *PV, *PH;
+    const double *Part, *PVb, *PHb;         // quadrant partials (PartLayout) and incoming boundary faces
+    PartLayout pl;
     double *Hxx, *Hyy, *Hxy, *Ea;           // cell averages
     double *Vxx, *Vxy, *Fx, *Ev;            // V faces
     double *Hyy2, *Hxy2, *Fy, *Eh;          // H faces
@@ -107,16 +108,28 @@ __global__ __launch_bounds__(256) void collapse_kernel(CollArgs a)
         const int g = gb * CL_G + gl;
         if (loc < nloc && g < a.ng) {
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            if (kind < 2) {
-                const double *P = kind == 0 ? a.Pavg : a.PV;
+            if (kind == 0) {
+                const int i = (int)(loc % a.nx), j = (int)(loc / a.nx);
                 for (int q = 0; q < 4; ++q) {
-                    const double4 v = *reinterpret_cast<const double4 *>(P + ((loc * 4 + q) * a.ng + g) * 4);
-                    s0 += v.x; s1 += v.y; s2 += v.z; s3 += v.w;
+                    const double *p = a.Part + a.pl.at(i, j, q, g, 0);
+                    const size_t PP = a.pl.PP;
+                    s0 += p[0]; s1 += p[PP]; s2 += p[2 * PP]; s3 += p[3 * PP];
+                }
+            } else if (kind == 1) {
+                const int f = (int)(loc % (a.nx + 1)), j = (int)(loc / (a.nx + 1));
+                for (int q = 0; q < 4; ++q) {
+                    s0 += part_v(a.pl, a.Part, a.PVb, f, j, q, g, 0);
+                    s1 += part_v(a.pl, a.Part, a.PVb, f, j, q, g, 1);
+                    s2 += part_v(a.pl, a.Part, a.PVb, f, j, q, g, 2);
+                    s3 += part_v(a.pl, a.Part, a.PVb, f, j, q, g, 3);
                 }
             } else {
+                const int i = (int)(loc % a.nx), f = (int)(loc / a.nx);
                 for (int q = 0; q < 4; ++q) {
-                    const double *p = a.PH + ((loc * 4 + q) * a.ng + g) * 5;
-                    s0 += p[0]; s1 += p[1]; s2 += p[2]; s3 += p[4];
+                    s0 += part_h(a.pl, a.Part, a.PHb, i, f, q, g, 0);
+                    s1 += part_h(a.pl, a.Part, a.PHb, i, f, q, g, 1);
+                    s2 += part_h(a.pl, a.Part, a.PHb, i, f, q, g, 2);
+                    s3 += part_h(a.pl, a.Part, a.PHb, i, f, q, g, 4);
                 }
             }
             const double e = s3 * a.cinv;
@@ -195,7 +208,7 @@ static void collapse(Ctx &c, bool with_fg)
     a.nx = c.nx; a.ny = c.ny; a.ng = c.ng;
     a.cinv = 1.0 / C_CONST;     // TRANSPORT_SOLVES.f08:403,475-477 (hard-coded c)
     a.c = c.p.c;                // fg_Calc divides by c*E with the input c (MLOQD_SOLVES.f08:646-674)
-    a.Pavg = c.Pavg; a.PV = c.PV; a.PH = c.PH;
+    a.Part = c.Part; a.PVb = c.PVb; a.PHb = c.PHb; a.pl = c.pl;
     a.Hxx = c.arr("Hg_avg_xx"); a.Hyy = c.arr("Hg_avg_yy"); a.Hxy = c.arr("Hg_avg_xy"); a.Ea = c.arr("HO_Eg_avg");
     a.Vxx = c.arr("Hg_edgV_xx"); a.Vxy = c.arr("Hg_edgV_xy"); a.Fx = c.arr("HO_Fxg_edgV"); a.Ev = c.arr("HO_Eg_edgV");
     a.Hyy2 = c.arr("Hg_edgH_yy"); a.Hxy2 = c.arr("Hg_edgH_xy"); a.Fy = c.arr("HO_Fyg_edgH"); a.Eh = c.arr("HO_Eg_edgH");
@@ -248,7 +261,8 @@ void fg_calc(Ctx &c)
 // boundary faces, read from the quadrant partials (eps already included).
 struct CgArgs {
     int nx, ny, ng, bc[4];
-    const double *PV, *PH;
+    const double *Part, *PVb, *PHb;
+    PartLayout pl;
     double *L, *B, *R, *T;
 };
 
@@ -262,24 +276,22 @@ __global__ void cg_kernel(CgArgs a)
         if (r < 2 * a.ny) {
             const bool left = r < a.ny;
             const int j = left ? r : r - a.ny;
-            const size_t f = (size_t)(left ? 0 : a.nx) + (size_t)(a.nx + 1) * j;
+            const int f = left ? 0 : a.nx;
             // left: outgoing Omega_x < 0 (q2, q3); right: Omega_x > 0 (q1, q4)
             const int qa = left ? 1 : 0, qb = left ? 2 : 3;
-            const double *pa = a.PV + ((f * 4 + qa) * a.ng + g) * 4;
-            const double *pb = a.PV + ((f * 4 + qb) * a.ng + g) * 4;
-            const double v = (pa[2] + pb[2]) / (pa[3] + pb[3]);
+            const double v = (part_v(a.pl, a.Part, a.PVb, f, j, qa, g, 2) + part_v(a.pl, a.Part, a.PVb, f, j, qb, g, 2)) /
+                             (part_v(a.pl, a.Part, a.PVb, f, j, qa, g, 3) + part_v(a.pl, a.Part, a.PVb, f, j, qb, g, 3));
             if (left) a.L[j + (size_t)a.ny * g] = a.bc[0] == 0 ? v : 0.0;
             else a.R[j + (size_t)a.ny * g] = a.bc[2] == 0 ? v : 0.0;
         } else {
             r -= 2 * a.ny;
             const bool bot = r < a.nx;
             const int i = bot ? r : r - a.nx;
-            const size_t f = (size_t)i + (size_t)a.nx * (bot ? 0 : a.ny);
+            const int f = bot ? 0 : a.ny;
             // bottom: outgoing Omega_y < 0 (q3, q4); top: Omega_y > 0 (q1, q2)
             const int qa = bot ? 2 : 0, qb = bot ? 3 : 1;
-            const double *pa = a.PH + ((f * 4 + qa) * a.ng + g) * 5;
-            const double *pb = a.PH + ((f * 4 + qb) * a.ng + g) * 5;
-            const double v = (pa[3] + pb[3]) / (pa[4] + pb[4]);
+            const double v = (part_h(a.pl, a.Part, a.PHb, i, f, qa, g, 3) + part_h(a.pl, a.Part, a.PHb, i, f, qb, g, 3)) /
+                             (part_h(a.pl, a.Part, a.PHb, i, f, qa, g, 4) + part_h(a.pl, a.Part, a.PHb, i, f, qb, g, 4));
             if (bot) a.B[i + (size_t)a.nx * g] = a.bc[1] == 0 ? v : 0.0;
             else a.T[i + (size_t)a.nx * g] = a.bc[3] == 0 ? v : 0.0;
         }
@@ -291,7 +303,7 @@ void cg_calc(Ctx &c)
     CgArgs a{};
     a.nx = c.nx; a.ny = c.ny; a.ng = c.ng;
     for (int k = 0; k < 4; ++k) a.bc[k] = c.p.bc_type[k];
-    a.PV = c.PV; a.PH = c.PH;
+    a.Part = c.Part; a.PVb = c.PVb; a.PHb = c.PHb; a.pl = c.pl;
     a.L = c.arr("Cg_L"); a.B = c.arr("Cg_B"); a.R = c.arr("Cg_R"); a.T = c.arr("Cg_T");
     const int n = (2 * c.nx + 2 * c.ny) * c.ng;
     hipLaunchKernelGGL(cg_kernel, dim3(nblocks_for(n)), dim3(256), 0, c.stream, a);

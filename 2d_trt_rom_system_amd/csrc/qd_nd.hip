@@ -747,10 +747,14 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
         L.terms = static_cast<int *>(c.dalloc(hterms[l].size() * sizeof(int)));
         L.up = static_cast<int *>(c.dalloc(hup[l].size() * sizeof(int)));
         L.kids = static_cast<int *>(c.dalloc(hkids[l].size() * sizeof(int)));
-        TRT_CHECK(hipMemcpy(L.rows, hrows[l].data(), hrows[l].size() * sizeof(int), hipMemcpyHostToDevice));
-        TRT_CHECK(hipMemcpy(L.terms, hterms[l].data(), hterms[l].size() * sizeof(int), hipMemcpyHostToDevice));
-        TRT_CHECK(hipMemcpy(L.up, hup[l].data(), hup[l].size() * sizeof(int), hipMemcpyHostToDevice));
-        TRT_CHECK(hipMemcpy(L.kids, hkids[l].data(), hkids[l].size() * sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpyAsync(L.rows, hrows[l].data(), hrows[l].size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c.stream));
+        TRT_CHECK(hipMemcpyAsync(L.terms, hterms[l].data(), hterms[l].size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c.stream));
+        TRT_CHECK(hipMemcpyAsync(L.up, hup[l].data(), hup[l].size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c.stream));
+        TRT_CHECK(hipMemcpyAsync(L.kids, hkids[l].data(), hkids[l].size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c.stream));
         L.foff = fo; L.voff = vo; L.poff = po;
         fo += (size_t)L.nn * L.nF * L.nF;
         vo += (size_t)L.nn * L.nF;
@@ -773,7 +777,8 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
     {
         std::vector<int> id(std::max(P->cap, 1));
         for (size_t k = 0; k < id.size(); ++k) id[k] = (int)k;
-        TRT_CHECK(hipMemcpy(P->ident, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpyAsync(P->ident, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));   // the plan's tables are uploaded on the stream that uses them
     }
     P->R = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));
     P->D = static_cast<double *>(c.dalloc((size_t)P->cap * s.nrow * sizeof(double)));

@@ -39,7 +39,8 @@ struct SweepArgs {
     const void *Iold;
     void *Inew;
     void *XV, *XH;
-    double *Pavg, *PV, *PH;
+    double *Part, *PVb, *PHb;
+    PartLayout pl;
 };
 
 template <typename T> struct Vec4;
@@ -67,7 +68,6 @@ template <> struct Vec4<float> {
 };
 
 constexpr int SWEEP_THREADS = 256;
-constexpr int NMOM = 13;          // 4 cell-average + 4 V-face + 5 H-face moments
 
 // Block-wide sum over the K directions of each group held by the block.
 // vals[v] is this thread's contribution to moment v; out(v, g) receives the sum.
@@ -219,13 +219,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
             const double vals[NMOM] = {cxx * Ia, cyy * Ia, cxy * Ia, cw * Ia,
                                        cxx * Iv, cxy * Iv, cx * Iv, cw * Iv,
                                        cyy * Ih, cxy * Ih, cx * Ih, cy * Ih, cw * Ih};
-            const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
-            const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
-            double *Pavg = a.Pavg, *PV = a.PV, *PH = a.PH;
             block_reduce<NMOM>(lds, vals, K, a.gblk, g0, ng, [&](int v, int gg, double s) {
-                if (v < 4) Pavg[((c * 4 + q) * ng + gg) * 4 + v] = s;
-                else if (v < 8) PV[((fV * 4 + q) * ng + gg) * 4 + (v - 4)] = s;
-                else PH[((fH * 4 + q) * ng + gg) * 5 + (v - 8)] = s;
+                a.Part[a.pl.at(i, j, q, gg, v)] = s;
             });
             // incoming boundary faces contribute their own moments (their values are the
             // boundary intensities set by RT_BC_UPDATE, UPDATES.f08:12-105)
@@ -234,11 +229,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, double *lds, int 
                 const double Ivb = IVin + eps, Ihb = IHin + eps;
                 const double bv[9] = {cxx * Ivb, cxy * Ivb, cx * Ivb, cw * Ivb,
                                       cyy * Ihb, cxy * Ihb, cx * Ihb, cy * Ihb, cw * Ihb};
-                const size_t fVi = (size_t)(px ? 0 : nx) + (size_t)(nx + 1) * j;
-                const size_t fHi = (size_t)i + (size_t)nx * (py ? 0 : ny);
                 block_reduce<9>(lds, bv, K, a.gblk, g0, ng, [&](int v, int gg, double s) {
-                    if (v < 4) { if (xin) PV[((fVi * 4 + q) * ng + gg) * 4 + v] = s; }
-                    else { if (yin) PH[((fHi * 4 + q) * ng + gg) * 5 + (v - 4)] = s; }
+                    if (v < 4) { if (xin) a.PVb[(((size_t)q * ny + j) * ng + gg) * 4 + v] = s; }
+                    else { if (yin) a.PHb[(((size_t)q * nx + i) * ng + gg) * 5 + (v - 4)] = s; }
                 });
             }
         }
@@ -296,6 +289,16 @@ static void geom_for(const trt_problem &p, SweepGeom &sg)
     sg.NTY = (p.ny + sg.TY - 1) / sg.TY;
 }
 
+static PartLayout part_layout(const trt_problem &p, const SweepGeom &sg)
+{
+    PartLayout L;
+    L.nx = p.nx; L.ny = p.ny; L.ng = p.ng;
+    L.TX = sg.TX; L.TY = sg.TY; L.NTX = sg.NTX; L.NTY = sg.NTY;
+    L.gblk = sg.gblk; L.nblk_g = sg.nblk_g;
+    L.PP = (sg.TX * sg.TY * sg.gblk + 15) / 16 * 16;   // runs of whole 128-byte lines
+    return L;
+}
+
 size_t sweep_bytes(const trt_problem &p, SweepGeom *out)
 {
     SweepGeom sg;
@@ -304,16 +307,17 @@ size_t sweep_bytes(const trt_problem &p, SweepGeom *out)
     const size_t nc = (size_t)p.nx * p.ny, NL = (size_t)p.ng * sg.K;
     const size_t ti = p.precision ? 4 : 8;
     size_t b = 2 * nc * 4 * NL * 4 * ti;                                   // I_crn, I_crn_old
-    b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * sg.NTY * sg.TY * NL * ti;   // XV (whole tiles)
-    b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * sg.NTX * sg.TX * NL * ti;   // XH
+    const size_t NR = (size_t)rec_lanes((int)NL);
+    b += 4 * (size_t)std::max(sg.NTX - 1, 0) * 2 * sg.NTY * sg.TY * NR * ti;   // XV (whole tiles)
+    b += 4 * (size_t)std::max(sg.NTY - 1, 0) * 2 * sg.NTX * sg.TX * NR * ti;   // XH
     bool refl = false;
     for (int k = 0; k < 4; ++k) refl = refl || p.bc_type[k] == 1;
     if (refl)                                                              // boundary slices
-        b += 2 * 4 * ((size_t)sg.NTY * 2 * sg.TY + (size_t)sg.NTX * 2 * sg.TX) * NL * ti;
+        b += 2 * 4 * ((size_t)sg.NTY * 2 * sg.TY + (size_t)sg.NTX * 2 * sg.TX) * NR * ti;
     b += (size_t)p.nm * 6 * 8;                                             // moment weight table
     b += nc * p.ng * 2 * 8;                                                // sc
     if (p.precision) b += nc * p.ng * 2 * 4;                               // FP32 copy of sc
-    b += (nc * 4 + (size_t)(p.nx + 1) * p.ny * 4 + (size_t)p.nx * (p.ny + 1) * 5) * 4 * p.ng * 8;
+    b += (part_layout(p, sg).size() + 4 * (size_t)p.ng * (p.ny * 4 + p.nx * 5)) * 8;   // partial moments
     return b;
 }
 
@@ -335,22 +339,24 @@ void sweep_setup(Ctx &c)
     c.Icrn[1] = c.dalloc(c.icrn_bytes);
     // face exchange: sized for whole tiles (the tile engine's packed layout), which also
     // covers the per-row layout of sweep_mode 1
-    c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * sg.NTY * sg.TY * NL * ti));
-    c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * sg.NTX * sg.TX * NL * ti));
+    const size_t NR = (size_t)rec_lanes((int)NL);      // also covers the per-row layout's NL lanes
+    c.XV = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTX - 1) * 2 * sg.NTY * sg.TY * NR * ti));
+    c.XH = c.dalloc(std::max<size_t>(1, 4 * (size_t)(sg.NTY - 1) * 2 * sg.NTX * sg.TX * NR * ti));
     c.refl = 0;
     for (int k = 0; k < 4; ++k)
         if (c.p.bc_type[k] == 1) c.refl |= 1 << k;
     if (c.refl && c.sweep_mode == 1)
         throw Error("reflective boundaries (BC_Type 1) run on the tile engine (sweep_mode 0, 2, 3 or 4)");
     if (c.refl) {
-        c.BV = c.dalloc(2 * 4 * (size_t)sg.NTY * NL * 2 * sg.TY * ti);
-        c.BH = c.dalloc(2 * 4 * (size_t)sg.NTX * NL * 2 * sg.TX * ti);
+        c.BV = c.dalloc(2 * 4 * (size_t)sg.NTY * NR * 2 * sg.TY * ti);
+        c.BH = c.dalloc(2 * 4 * (size_t)sg.NTX * NR * 2 * sg.TX * ti);
     }
     c.sc = static_cast<double *>(c.dalloc(nc * c.ng * 2 * 8));
     if (c.precision) c.scf = static_cast<float *>(c.dalloc(nc * c.ng * 2 * 4));
-    c.Pavg = static_cast<double *>(c.dalloc(nc * 4 * c.ng * 4 * 8));
-    c.PV = static_cast<double *>(c.dalloc((size_t)(c.nx + 1) * c.ny * 4 * c.ng * 4 * 8));
-    c.PH = static_cast<double *>(c.dalloc((size_t)c.nx * (c.ny + 1) * 4 * c.ng * 5 * 8));
+    c.pl = part_layout(c.p, sg);
+    c.Part = static_cast<double *>(c.dalloc(c.pl.size() * 8));
+    c.PVb = static_cast<double *>(c.dalloc((size_t)4 * c.ny * c.ng * 4 * 8));
+    c.PHb = static_cast<double *>(c.dalloc((size_t)4 * c.nx * c.ng * 5 * 8));
 }
 
 template <typename TI>
@@ -387,7 +393,7 @@ void transport_sweep(Ctx &c, std::vector<hipEvent_t> *ev)
     a.Iold = c.Icrn[c.iold];
     a.Inew = c.Icrn[out];
     a.XV = c.XV; a.XH = c.XH;
-    a.Pavg = c.Pavg; a.PV = c.PV; a.PH = c.PH;
+    a.Part = c.Part; a.PVb = c.PVb; a.PHb = c.PHb; a.pl = c.pl;
     for (int d = 0; d <= sg.NTX + sg.NTY - 2; ++d) {
         a.d = d;
         a.tx_lo = std::max(0, d - (sg.NTY - 1));
@@ -484,7 +490,7 @@ void push_icrn(Ctx &c, int which, const double *host)
 // ---------------------------------------------------------------------------
 struct InflowSlices {
     const void *BV, *BH;
-    int refl, TX, TY, NTX, NTY;
+    int refl, TX, TY, NTX, NTY, NLr;
 };
 
 template <typename TI>
@@ -522,12 +528,12 @@ __global__ void derived_kernel(const TI *I, const double *bcin, InflowSlices bs,
                 const int side = px ? 0 : 2;
                 if (bs.refl & (1 << side)) {
                     const int jf = py ? j : ny - 1 - j, tile = jf / bs.TY, r = jf - tile * bs.TY;
-                    const TI *sl = static_cast<const TI *>(bs.BV) + (size_t)((px ? 0 : 1) * 4 + q) * bs.NTY * NL * 2 * bs.TY;
+                    const TI *sl = static_cast<const TI *>(bs.BV) + (size_t)((px ? 0 : 1) * 4 + q) * bs.NTY * bs.NLr * 2 * bs.TY;
                     const int lane = g * K + k;
                     const double sc = 1.0 / iscale<TI>();
                     // frame-bottom and frame-top corners; the frame is flipped in y when !py
-                    const double fb = (double)sl[rec_elem<TI>(tile, 2 * r, 2 * bs.TY, (int)NL, lane)] * sc;
-                    const double ft = (double)sl[rec_elem<TI>(tile, 2 * r + 1, 2 * bs.TY, (int)NL, lane)] * sc;
+                    const double fb = (double)sl[rec_elem<TI>(tile, 2 * r, 2 * bs.TY, bs.NLr, lane)] * sc;
+                    const double ft = (double)sl[rec_elem<TI>(tile, 2 * r + 1, 2 * bs.TY, bs.NLr, lane)] * sc;
                     const double lo = py ? fb : ft, hi = py ? ft : fb;
                     v = top < 0 ? (lo + hi) * 0.5 : (top ? hi : lo);
                 } else {
@@ -546,12 +552,12 @@ __global__ void derived_kernel(const TI *I, const double *bcin, InflowSlices bs,
                 const int side = py ? 1 : 3;
                 if (bs.refl & (1 << side)) {
                     const int xf = px ? i : nx - 1 - i, tile = xf / bs.TX, ii = xf - tile * bs.TX;
-                    const TI *sl = static_cast<const TI *>(bs.BH) + (size_t)((py ? 0 : 1) * 4 + q) * bs.NTX * NL * 2 * bs.TX;
+                    const TI *sl = static_cast<const TI *>(bs.BH) + (size_t)((py ? 0 : 1) * 4 + q) * bs.NTX * bs.NLr * 2 * bs.TX;
                     const int lane = g * K + k;
                     const double sc = 1.0 / iscale<TI>();
                     // frame-left and frame-right corners; the frame is flipped in x when !px
-                    const double fl = (double)sl[rec_elem<TI>(tile, 2 * ii, 2 * bs.TX, (int)NL, lane)] * sc;
-                    const double fr = (double)sl[rec_elem<TI>(tile, 2 * ii + 1, 2 * bs.TX, (int)NL, lane)] * sc;
+                    const double fl = (double)sl[rec_elem<TI>(tile, 2 * ii, 2 * bs.TX, bs.NLr, lane)] * sc;
+                    const double fr = (double)sl[rec_elem<TI>(tile, 2 * ii + 1, 2 * bs.TX, bs.NLr, lane)] * sc;
                     const double lft = px ? fl : fr, rgt = px ? fr : fl;
                     v = right < 0 ? (lft + rgt) * 0.5 : (right ? rgt : lft);
                 } else {
@@ -587,7 +593,7 @@ void pull_derived(Ctx &c, const char *name, double *host)
     if (kind < 0) throw Error(std::string("not a derived intensity view: ") + name);
     const size_t n = derived_size(c, name);
     double *tmp = static_cast<double *>(c.dalloc(n * 8));
-    const InflowSlices bs{c.BV, c.BH, c.refl, c.sg.TX, c.sg.TY, c.sg.NTX, c.sg.NTY};
+    const InflowSlices bs{c.BV, c.BH, c.refl, c.sg.TX, c.sg.TY, c.sg.NTX, c.sg.NTY, rec_lanes(c.ng * c.sg.K)};
     if (c.precision)
         hipLaunchKernelGGL(derived_kernel<float>, dim3(1024), dim3(256), 0, c.stream,
                            static_cast<const float *>(c.Icrn[c.icur]), c.bcin, bs, tmp, kind, c.nx, c.ny, c.nm, c.ng);

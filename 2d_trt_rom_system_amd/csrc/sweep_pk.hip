@@ -51,7 +51,6 @@ namespace trt {
 namespace {
 
 constexpr int PK_THREADS = 256;
-constexpr int NMOM = 13;
 
 struct PkProb {
     int nx, ny, ng, K, gblk, NTX, NTY;
@@ -65,12 +64,14 @@ struct PkProb {
     const void *Iold;
     void *Inew;
     void *XV, *XH;
-    double *Pavg, *PV, *PH;
+    double *Part, *PVb, *PHb;  // quadrant partial moments (PartLayout) and their incoming boundary faces
+    int PP, nblk_g;            // the layout's run length and group blocks
+    int NLr;                   // lanes of a face-exchange or boundary slice (rec_lanes)
     const double *ctab;    // [m][6]: w, w ox^2, w oy^2, w ox oy, w ox, w oy, over the stored intensities' factor
     int tx, ty;            // tile shape
     int refl;              // reflective sides: bit 0 left, 1 bottom, 2 right, 3 top
     int skip;              // batched sweeps: this problem is not swept this time (its items only advance the queue)
-    void *BV, *BH;         // their boundary slices: [left, right][q][NTY][NL][2 TY], [bottom, top][q][NTX][NL][2 TX]
+    void *BV, *BH;         // their boundary slices: [left, right][q][NTY][NLr][2 TY], [bottom, top][q][NTX][NLr][2 TX]
 };
 
 struct PkArgs {
@@ -309,30 +310,31 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     // incoming faces: the records the left and lower tiles wrote (rows and columns
     // beyond a partial tile's edge are carried but unused)
     constexpr int NV = 2 * TY, NH = 2 * TX;
-    const unsigned vslice = (unsigned)((size_t)P.NTY * NL * NV * sizeof(TI));
-    const unsigned hslice = (unsigned)((size_t)P.NTX * NL * NH * sizeof(TI));
+    const int NR = P.NLr;
+    const unsigned vslice = (unsigned)((size_t)P.NTY * NR * NV * sizeof(TI));
+    const unsigned hslice = (unsigned)((size_t)P.NTX * NR * NH * sizeof(TI));
     TI vbr[TY], vtr[TY], hl[TX], hr[TX];
     {
         TI pv[NV];
 #pragma unroll
         for (int t = 0; t < NV; ++t) pv[t] = bcx;
         if (txl > 0 && act)
-            xload_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl - 1) * P.NTY * NL * NV, vslice, (unsigned)tyl,
-                              (unsigned)NL, (unsigned)L, pv);
+            xload_rec<TI, NV>(XV + (size_t)(q * (P.NTX - 1) + txl - 1) * P.NTY * NR * NV, vslice, (unsigned)tyl,
+                              (unsigned)NR, (unsigned)L, pv);
         else if (act && (P.refl & (px ? 1 : 4)))   // reflective left (right) side: the inflow slice
-            xload_rec<TI, NV>(static_cast<TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * NL * NV, vslice,
-                              (unsigned)tyl, (unsigned)NL, (unsigned)L, pv);
+            xload_rec<TI, NV>(static_cast<TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * NR * NV, vslice,
+                              (unsigned)tyl, (unsigned)NR, (unsigned)L, pv);
 #pragma unroll
         for (int r = 0; r < TY; ++r) { vbr[r] = pv[2 * r]; vtr[r] = pv[2 * r + 1]; }
         TI ph[NH];
 #pragma unroll
         for (int t = 0; t < NH; ++t) ph[t] = bcy;
         if (tyl > 0 && act && !hin_regs)
-            xload_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl - 1) * P.NTX * NL * NH, hslice, (unsigned)txl,
-                              (unsigned)NL, (unsigned)L, ph);
+            xload_rec<TI, NH>(XH + (size_t)(q * (P.NTY - 1) + tyl - 1) * P.NTX * NR * NH, hslice, (unsigned)txl,
+                              (unsigned)NR, (unsigned)L, ph);
         else if (tyl == 0 && act && (P.refl & (py ? 2 : 8)))   // reflective bottom (top) side
-            xload_rec<TI, NH>(static_cast<TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * NL * NH, hslice,
-                              (unsigned)txl, (unsigned)NL, (unsigned)L, ph);
+            xload_rec<TI, NH>(static_cast<TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * NR * NH, hslice,
+                              (unsigned)txl, (unsigned)NR, (unsigned)L, ph);
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             hl[ii] = ph[2 * ii];
@@ -417,18 +419,18 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
         TI pv[NV];
 #pragma unroll
         for (int r = 0; r < TY; ++r) { pv[2 * r] = vbr[r]; pv[2 * r + 1] = vtr[r]; }
-        TI *dst = v_next ? XV + (size_t)(q * (P.NTX - 1) + txl) * P.NTY * NL * NV
-                     : static_cast<TI *>(P.BV) + (size_t)((px ? 1 : 0) * 4 + q) * P.NTY * NL * NV;
-        xstore_rec<TI, NV>(dst, vslice, (unsigned)tyl, (unsigned)NL, (unsigned)L, pv);
+        TI *dst = v_next ? XV + (size_t)(q * (P.NTX - 1) + txl) * P.NTY * NR * NV
+                     : static_cast<TI *>(P.BV) + (size_t)((px ? 1 : 0) * 4 + q) * P.NTY * NR * NV;
+        xstore_rec<TI, NV>(dst, vslice, (unsigned)tyl, (unsigned)NR, (unsigned)L, pv);
     }
     const bool h_next = tyl < P.NTY - 1 && hout_xh, h_bnd = tyl == P.NTY - 1 && (P.refl & (py ? 8 : 2));
     if ((h_next || h_bnd) && act) {
         TI ph[NH];
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) { ph[2 * ii] = (TI)hlc[ii]; ph[2 * ii + 1] = (TI)hrc[ii]; }
-        TI *dst = h_next ? XH + (size_t)(q * (P.NTY - 1) + tyl) * P.NTX * NL * NH
-                     : static_cast<TI *>(P.BH) + (size_t)((py ? 1 : 0) * 4 + q) * P.NTX * NL * NH;
-        xstore_rec<TI, NH>(dst, hslice, (unsigned)txl, (unsigned)NL, (unsigned)L, ph);
+        TI *dst = h_next ? XH + (size_t)(q * (P.NTY - 1) + tyl) * P.NTX * NR * NH
+                     : static_cast<TI *>(P.BH) + (size_t)((py ? 1 : 0) * 4 + q) * P.NTX * NR * NH;
+        xstore_rec<TI, NH>(dst, hslice, (unsigned)txl, (unsigned)NR, (unsigned)L, ph);
     }
 #pragma unroll
     for (int ii = 0; ii < TX; ++ii) {
@@ -492,15 +494,12 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, doubl
         for (int w = 1; w < 4; ++w)
 #pragma unroll
             for (int v = 0; v < NMOM; ++v) m[v] += xp[((w - 1) * NMOM + v) * 64 + ln];
-        const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
-        const int j = py ? j0 + r : ny - 1 - (j0 + r);
-        const size_t c = (size_t)i + (size_t)nx * j;
-        const size_t fV = (size_t)(px ? i + 1 : i) + (size_t)(nx + 1) * j;
-        const size_t fH = (size_t)i + (size_t)nx * (py ? j + 1 : j);
-        store4(gptr(P.Pavg) + ((c * 4 + q) * ng + g) * 4, m[0], m[1], m[2], m[3]);
-        store4(gptr(P.PV) + ((fV * 4 + q) * ng + g) * 4, m[4], m[5], m[6], m[7]);
-        GLOBAL_AS double *o = gptr(P.PH) + ((fH * 4 + q) * ng + g) * 5;
-        o[0] = m[8]; o[1] = m[9]; o[2] = m[10]; o[3] = m[11]; o[4] = m[12];
+        // the item's runs: value v of pair p at v PP + p, so lane ln of the wave writes the
+        // value next to lane ln - 1's
+        GLOBAL_AS double *o = gptr(P.Part) + (size_t)P.PP * NMOM *
+                                                 ((((size_t)q * P.nblk_g + gb) * P.NTY + tyl) * P.NTX + txl) + p;
+#pragma unroll
+        for (int v = 0; v < NMOM; ++v) o[(size_t)v * P.PP] = m[v];
     };
     constexpr int XSP = 3 * NMOM * 64;   // doubles of one pass in xs
     if (P.xsw == 0) {
@@ -779,38 +778,36 @@ __global__ void pk_boundary_moments(const PkProb *probs)
             const int j = f;
             const double b0 = (double)(TI)(P.bcin[(px ? 0 : 2) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
             const bool rf = P.refl & (px ? 1 : 4);
-            const TI *bs = rf ? static_cast<const TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * ng * K * 2 * P.ty
+            const TI *bs = rf ? static_cast<const TI *>(P.BV) + (size_t)((px ? 0 : 1) * 4 + q) * P.NTY * P.NLr * 2 * P.ty
                               : nullptr;
             const int jf = py ? j : ny - 1 - j, tile = jf / P.ty;
             double s[4] = {0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;
-                const double b = rf ? inflow_face(bs, tile, 2 * (jf - tile * P.ty), 2 * P.ty, ng * K, g * K + k) + P.eps
+                const double b = rf ? inflow_face(bs, tile, 2 * (jf - tile * P.ty), 2 * P.ty, P.NLr, g * K + k) + P.eps
                                     : b0;
                 const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
                 s[0] += wm * ox * ox * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * b;
             }
-            const size_t fv = (size_t)(px ? 0 : nx) + (size_t)(nx + 1) * j;
-            double *o = P.PV + ((fv * 4 + q) * ng + g) * 4;
+            double *o = P.PVb + (((size_t)q * ny + j) * ng + g) * 4;
             o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3];
         } else {
             const int i = f - ny;
             const double b0 = (double)(TI)(P.bcin[(py ? 1 : 3) * ng + g] * iscale<TI>()) / iscale<TI>() + P.eps;
             const bool rf = P.refl & (py ? 2 : 8);
-            const TI *bs = rf ? static_cast<const TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * ng * K * 2 * P.tx
+            const TI *bs = rf ? static_cast<const TI *>(P.BH) + (size_t)((py ? 0 : 1) * 4 + q) * P.NTX * P.NLr * 2 * P.tx
                               : nullptr;
             const int xf = px ? i : nx - 1 - i, tile = xf / P.tx;
             double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < K; ++k) {
                 const int m = q * K + k;
-                const double b = rf ? inflow_face(bs, tile, 2 * (xf - tile * P.tx), 2 * P.tx, ng * K, g * K + k) + P.eps
+                const double b = rf ? inflow_face(bs, tile, 2 * (xf - tile * P.tx), 2 * P.tx, P.NLr, g * K + k) + P.eps
                                     : b0;
                 const double wm = P.w[m], ox = P.ox[m], oy = P.oy[m];
                 s[0] += wm * oy * oy * b; s[1] += wm * ox * oy * b; s[2] += wm * ox * b; s[3] += wm * oy * b;
                 s[4] += wm * b;
             }
-            const size_t fh = (size_t)i + (size_t)nx * (py ? 0 : ny);
-            double *o = P.PH + ((fh * 4 + q) * ng + g) * 5;
+            double *o = P.PHb + (((size_t)q * nx + i) * ng + g) * 5;
             o[0] = s[0]; o[1] = s[1]; o[2] = s[2]; o[3] = s[3]; o[4] = s[4];
         }
     }
@@ -854,8 +851,8 @@ __global__ void reflect_kernel(TI *BV, TI *BH, size_t nv, size_t nh, int refl)
 // Every quadrant's slice of each reflective side (blockIdx.y) at Bg_planck(bcT of the
 // side) of the lane's group, as the tile engine rounds the Planckian inflow.
 template <typename TI>
-__global__ void reflect_reset_kernel(TI *BV, TI *BH, const double *bcin, size_t nv, size_t nh, int NL, int K,
-                                     int ng, int refl)
+__global__ void reflect_reset_kernel(TI *BV, TI *BH, const double *bcin, size_t nv, size_t nh, int NL, int NLr,
+                                     int K, int ng, int refl)
 {
     const int side = blockIdx.y;
     if (!(refl & (1 << side))) return;
@@ -864,8 +861,8 @@ __global__ void reflect_reset_kernel(TI *BV, TI *BH, const double *bcin, size_t 
     TI *base = vert ? BV + (size_t)(side == 0 ? 0 : 4) * nv : BH + (size_t)(side == 1 ? 0 : 4) * nh;
     constexpr int E = 16 / (int)sizeof(TI);
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int g = (int)((e / E) % NL) / K;
-        base[e] = (TI)(bcin[side * ng + g] * iscale<TI>());
+        const int lane = (int)((e / E) % NLr);      // lanes from NL on are padding
+        base[e] = lane < NL ? (TI)(bcin[side * ng + lane / K] * iscale<TI>()) : (TI)0;
     }
 }
 
@@ -998,9 +995,11 @@ static size_t staging_bytes(int tx, int ty, int precision)
 static const double *moment_table(Ctx &c)
 {
     std::vector<double> w(c.nm), ox(c.nm), oy(c.nm), tab((size_t)c.nm * 6);
-    TRT_CHECK(hipMemcpy(w.data(), c.w, c.nm * sizeof(double), hipMemcpyDeviceToHost));
-    TRT_CHECK(hipMemcpy(ox.data(), c.ox, c.nm * sizeof(double), hipMemcpyDeviceToHost));
-    TRT_CHECK(hipMemcpy(oy.data(), c.oy, c.nm * sizeof(double), hipMemcpyDeviceToHost));
+    // on the context's stream, after its uploads of the quadrature
+    TRT_CHECK(hipMemcpyAsync(w.data(), c.w, c.nm * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipMemcpyAsync(ox.data(), c.ox, c.nm * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipMemcpyAsync(oy.data(), c.oy, c.nm * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
     const double isc = c.precision ? 1.0 / kIScale32 : 1.0;
     for (int m = 0; m < c.nm; ++m) {
         double *t = &tab[(size_t)m * 6];
@@ -1012,7 +1011,8 @@ static const double *moment_table(Ctx &c)
         t[5] = w[m] * oy[m] * isc;
     }
     double *d = static_cast<double *>(c.dalloc(tab.size() * sizeof(double)));
-    TRT_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    TRT_CHECK(hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));
     return d;
 }
 
@@ -1036,7 +1036,9 @@ static void pk_setup(Ctx &c)
     P.sc = c.precision ? static_cast<const void *>(c.scf) : static_cast<const void *>(c.sc);
     P.bcin = c.bcin;
     P.XV = c.XV; P.XH = c.XH;
-    P.Pavg = c.Pavg; P.PV = c.PV; P.PH = c.PH;
+    P.Part = c.Part; P.PVb = c.PVb; P.PHb = c.PHb;
+    P.PP = c.pl.PP; P.nblk_g = c.pl.nblk_g;
+    P.NLr = rec_lanes(c.ng * sg.K);
     P.tx = sg.TX; P.ty = sg.TY;
     P.refl = c.refl; P.BV = c.BV; P.BH = c.BH;
     pk->lds = (size_t)P.xsw * 8 + stb;
@@ -1060,8 +1062,10 @@ static void pk_setup(Ctx &c)
     tab[1].Iold = c.Icrn[1]; tab[1].Inew = c.Icrn[0];
     pk->host[0] = tab[0];
     pk->host[1] = tab[1];
+    // every setup write goes on the context's stream, ahead of the sweeps that depend on it
+    // (the stream does not wait for work on the null stream)
     pk->probs = static_cast<PkProb *>(c.dalloc(sizeof(tab)));
-    TRT_CHECK(hipMemcpy(pk->probs, tab, sizeof(tab), hipMemcpyHostToDevice));
+    TRT_CHECK(hipMemcpyAsync(pk->probs, tab, sizeof(tab), hipMemcpyHostToDevice, c.stream));
 
     // items in tile-diagonal (or segment-diagonal) order
     std::vector<int4> items;
@@ -1070,11 +1074,12 @@ static void pk_setup(Ctx &c)
     for (int d = 0; d < num_diags(sg, P.nseg, segs); ++d) diag_items(items, 0, 0, sg, P.seg, P.nseg, d, segs);
     pk->nitems = (int)items.size();
     pk->items = static_cast<int4 *>(c.dalloc(items.size() * sizeof(int4)));
-    TRT_CHECK(hipMemcpy(pk->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    TRT_CHECK(hipMemcpyAsync(pk->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
     pk->flags = static_cast<int *>(c.dalloc((size_t)nchain * sg.NTX * sg.NTY * sizeof(int)));
-    TRT_CHECK(hipMemset(pk->flags, 0, (size_t)nchain * sg.NTX * sg.NTY * sizeof(int)));
+    TRT_CHECK(hipMemsetAsync(pk->flags, 0, (size_t)nchain * sg.NTX * sg.NTY * sizeof(int), c.stream));
     pk->queue = static_cast<int *>(c.dalloc(4 * sizeof(int)));
-    TRT_CHECK(hipMemset(pk->queue, 0, 4 * sizeof(int)));
+    TRT_CHECK(hipMemsetAsync(pk->queue, 0, 4 * sizeof(int), c.stream));
+    TRT_CHECK(hipStreamSynchronize(c.stream));      // the host arrays above go out of scope
     pk->kern = segs ? pk->k->seg : pk->k->pk;
     pk->grid = std::min(full_grid, pk->nitems);
 }
@@ -1097,9 +1102,9 @@ static void boundary_moments(Ctx &c, const PkProb *tab)
 // values per boundary slice of a V side and of an H side
 static void slice_sizes(const Ctx &c, size_t &nv, size_t &nh)
 {
-    const size_t NL = (size_t)c.ng * c.sg.K;
-    nv = (size_t)c.sg.NTY * NL * 2 * c.sg.TY;
-    nh = (size_t)c.sg.NTX * NL * 2 * c.sg.TX;
+    const size_t NR = (size_t)rec_lanes(c.ng * c.sg.K);
+    nv = (size_t)c.sg.NTY * NR * 2 * c.sg.TY;
+    nh = (size_t)c.sg.NTX * NR * 2 * c.sg.TX;
 }
 
 void rt_bc_update(Ctx &c)
@@ -1125,10 +1130,10 @@ void reflect_reset(Ctx &c)
     const int NL = c.ng * c.sg.K;
     if (c.precision)
         hipLaunchKernelGGL(reflect_reset_kernel<float>, grid, dim3(256), 0, c.stream, static_cast<float *>(c.BV),
-                           static_cast<float *>(c.BH), c.bcin, nv, nh, NL, c.sg.K, c.ng, c.refl);
+                           static_cast<float *>(c.BH), c.bcin, nv, nh, NL, rec_lanes(NL), c.sg.K, c.ng, c.refl);
     else
         hipLaunchKernelGGL(reflect_reset_kernel<double>, grid, dim3(256), 0, c.stream, static_cast<double *>(c.BV),
-                           static_cast<double *>(c.BH), c.bcin, nv, nh, NL, c.sg.K, c.ng, c.refl);
+                           static_cast<double *>(c.BH), c.bcin, nv, nh, NL, rec_lanes(NL), c.sg.K, c.ng, c.refl);
 }
 
 // Enqueues one sweep: work-list reset, boundary-face moments, persistent kernel.
@@ -1184,7 +1189,8 @@ void pk_check(Ctx &c)
     TRT_CHECK(hipStreamSynchronize(c.stream));
     if (e) {
         const int zero = 0;
-        TRT_CHECK(hipMemcpy(c.pk->queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpyAsync(c.pk->queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice, c.stream));
+        TRT_CHECK(hipStreamSynchronize(c.stream));
         throw Error("persistent sweep: a dependency wait exceeded its spin limit");
     }
 }
@@ -1358,17 +1364,20 @@ PkBatch *pk_batch_create(const std::vector<Ctx *> &ctxs, bool pooled, int seg_re
         b->nitems = (int)items.size();
         if ((size_t)b->nitems != expect) throw Error("batched work list does not cover every tile");
         b->items = static_cast<int4 *>(batch_alloc(*b, items.size() * sizeof(int4)));
-        TRT_CHECK(hipMemcpy(b->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
         b->flags = static_cast<int *>(batch_alloc(*b, (size_t)base[n] * sizeof(int)));
-        TRT_CHECK(hipMemset(b->flags, 0, (size_t)base[n] * sizeof(int)));
         b->queue = static_cast<int *>(batch_alloc(*b, 4 * sizeof(int)));
-        TRT_CHECK(hipMemset(b->queue, 0, 4 * sizeof(int)));
         b->probs = static_cast<PkProb *>(batch_alloc(*b, (size_t)n * sizeof(PkProb)));
         b->lds = (size_t)maxxs * 8 + staging_bytes(b->tx, b->ty, b->precision);
         b->kern = b->segs ? b->k->seg : b->k->pk;
         b->grid = resident_grid(b->kern, b->lds, b->nitems);
         if (pooled) b->stream = cache_stream_get();
         else TRT_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        // the work list, flags and counter on the batch stream, ahead of its sweeps
+        TRT_CHECK(hipMemcpyAsync(b->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                 b->stream));
+        TRT_CHECK(hipMemsetAsync(b->flags, 0, (size_t)base[n] * sizeof(int), b->stream));
+        TRT_CHECK(hipMemsetAsync(b->queue, 0, 4 * sizeof(int), b->stream));
+        TRT_CHECK(hipStreamSynchronize(b->stream));
         TRT_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
         TRT_CHECK(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
         static_assert(sizeof(PkProb) % sizeof(double) == 0, "table staged as doubles");
@@ -1554,7 +1563,8 @@ void pk_batch_check(PkBatch &b)
     TRT_CHECK(hipStreamSynchronize(b.stream));
     if (e) {
         const int zero = 0;
-        TRT_CHECK(hipMemcpy(b.queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice));
+        TRT_CHECK(hipMemcpyAsync(b.queue + 2, &zero, sizeof(int), hipMemcpyHostToDevice, b.stream));
+        TRT_CHECK(hipStreamSynchronize(b.stream));
         throw Error("batched sweep: a dependency wait exceeded its spin limit");
     }
 }

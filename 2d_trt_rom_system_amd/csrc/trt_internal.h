@@ -30,6 +30,11 @@ constexpr double kIScale32 = 1099511627776.0;   // 2^40
 template <typename TI>
 __host__ __device__ constexpr double iscale() { return sizeof(TI) == 4 ? kIScale32 : 1.0; }
 
+// Lanes of a face-exchange or boundary slice: the lane count rounded up to 8, so that each
+// (tile, chunk) run of 16-byte chunks fills whole 128-byte cache lines and no line holds
+// records of two tiles (written by different workgroups at different times).
+__host__ __device__ inline int rec_lanes(int NL) { return (NL + 7) & ~7; }
+
 // Element index of value t of the record of (tile, lane) in a face-exchange or boundary
 // slice whose records hold N values of TI: records are stored in 16-byte chunks,
 // chunk-major ([tile][chunk][lane][16 bytes]).
@@ -64,6 +69,55 @@ struct SweepGeom {
     int nblk_g = 0;   // ceil(ng/gblk)
     int TX = 8, TY = 8, NTX = 0, NTY = 0;
 };
+
+// Quadrant partial moments of a sweep: 13 FP64 values per (cell, quadrant, group), the
+// direction sums of COLLAPSE_INTENSITIES (TRANSPORT_SOLVES.f08:447-470) over one
+// quadrant: the cell average's w Ox^2, w Oy^2, w Ox Oy, w (values 0-3), the downwind V
+// face's w Ox^2, w Ox Oy, w Ox, w (4-7) and the downwind H face's w Oy^2, w Ox Oy, w Ox,
+// w Oy, w (8-12), each times (I + eps).  They are stored by work item of the tile engine:
+// item (quadrant q, group block gb, tile (txl, tyl) of q's swept frame) holds NMOM runs
+// of PP values, run v holding value v of the item's (cell, group) pairs p = (r TX + ii)
+// gblk + gl (r, ii: row and column in the tile; gl: group in the block), so one store
+// instruction of the moment reduction writes 64 consecutive pairs.  Incoming boundary
+// faces (no upwind cell in the quadrant) have arrays of their own: V [q][j][g][4], H
+// [q][i][g][5].
+constexpr int NMOM = 13;
+
+struct PartLayout {
+    int nx = 0, ny = 0, ng = 0, TX = 1, TY = 1, NTX = 0, NTY = 0, gblk = 1, nblk_g = 0, PP = 0;
+    // first value of run 0 of an item
+    __host__ __device__ __forceinline__ size_t item(int q, int gb, int txl, int tyl) const
+    {
+        return ((((size_t)q * nblk_g + gb) * NTY + tyl) * NTX + txl) * NMOM * (size_t)PP;
+    }
+    // value v of cell (i, j), group g, quadrant q
+    __host__ __device__ __forceinline__ size_t at(int i, int j, int q, int g, int v) const
+    {
+        const bool px = (q == 0 || q == 3), py = q < 2;
+        const int il = px ? i : nx - 1 - i, jl = py ? j : ny - 1 - j;
+        const int txl = il / TX, tyl = jl / TY, gb = g / gblk;
+        const int p = ((jl - tyl * TY) * TX + (il - txl * TX)) * gblk + (g - gb * gblk);
+        return item(q, gb, txl, tyl) + (size_t)v * PP + p;
+    }
+    size_t size() const { return (size_t)4 * nblk_g * NTY * NTX * NMOM * PP; }
+};
+
+// value v (0-3) of V face f of row j, quadrant q: the downwind V face of the cell upwind of
+// it, or the incoming boundary face
+__host__ __device__ __forceinline__ double part_v(const PartLayout &L, const double *Part, const double *PVb, int f,
+                                                  int j, int q, int g, int v)
+{
+    const int i = (q == 0 || q == 3) ? f - 1 : f;
+    return (i < 0 || i >= L.nx) ? PVb[(((size_t)q * L.ny + j) * L.ng + g) * 4 + v] : Part[L.at(i, j, q, g, 4 + v)];
+}
+
+// value v (0-4) of H face f of column i, quadrant q
+__host__ __device__ __forceinline__ double part_h(const PartLayout &L, const double *Part, const double *PHb, int i,
+                                                  int f, int q, int g, int v)
+{
+    const int j = q < 2 ? f - 1 : f;
+    return (j < 0 || j >= L.ny) ? PHb[(((size_t)q * L.nx + i) * L.ng + g) * 5 + v] : Part[L.at(i, j, q, g, 8 + v)];
+}
 
 // Batched reduced-QD systems (QD_FV) stored as coefficient arrays on device.
 struct NDPlan;      // nested-dissection plan and fronts of the direct QD solver (qd_nd.hip)
@@ -143,14 +197,16 @@ struct Ctx {
     void *XV = nullptr, *XH = nullptr;     // tile face exchange
     // reflective sides (BC_Type 1): bit 0 left, 1 bottom, 2 right, 3 top; the boundary
     // slices hold the corner intensities of the domain faces, in the exchange record
-    // layout, per side and quadrant: BV [left, right][q][NTY tiles][NL lanes][2 TY],
-    // BH [bottom, top][q][NTX tiles][NL lanes][2 TX].  A side's incoming quadrants
+    // layout, per side and quadrant: BV [left, right][q][NTY tiles][NLr lanes][2 TY],
+    // BH [bottom, top][q][NTX tiles][NLr lanes][2 TX] (NLr = rec_lanes(NL)).  A side's incoming quadrants
     // read their inflow there; its outgoing quadrants write their outflow there.
     int refl = 0;
     void *BV = nullptr, *BH = nullptr;
     double *sc = nullptr;                  // [cell][g][2] = (KapE + 1/(c dt), RT_Src)
     float *scf = nullptr;                  // FP32 copy of sc for the FP32 sweep
-    double *Pavg = nullptr, *PV = nullptr, *PH = nullptr;   // per-quadrant partial moments
+    double *Part = nullptr;                // quadrant partial moments by work item (PartLayout)
+    double *PVb = nullptr, *PHb = nullptr; // their incoming boundary faces
+    PartLayout pl;
     size_t icrn_bytes = 0;
     int sweep_mode = 2;                    // 1, 2, 3 as trt_problem.sweep_mode
     PkState *pk = nullptr;
