@@ -238,9 +238,12 @@ __device__ __forceinline__ void wait_flag(const int *f, int epoch, int limit, in
 // workgroup: with hin_regs the tile's incoming H faces come from hc instead of the
 // exchange buffer, and with hout_xh false its outgoing ones stay in hc only.  The
 // carried values are rounded to TI exactly as the exchange buffer rounds them.
-template <typename TI, typename TC, int TX, int TY, bool px, bool py, typename Sync>
+// publish() runs once the outgoing faces are stored (it makes them visible to the
+// downwind tiles); the tile's new corner intensities are kept in registers until then
+// and stored after it, so their stores are not on the path to the downwind tiles.
+template <typename TI, typename TC, int TX, int TY, bool px, bool py, typename Sync, typename Publish>
 __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync,
-                                          TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
+                                          Publish publish, TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
 {
     const int tid = threadIdx.x;
     const int K = P.K, ng = P.ng, nx = P.nx, ny = P.ny;
@@ -371,8 +374,6 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
 #pragma unroll
         for (int ii = 0; ii < TX; ++ii) {
             if (ii >= iw) continue;
-            const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
-            const size_t c = (size_t)i + (size_t)nx * j;
             const TC bb = bbc[ii];
             const TC A4 = dx4c[ii] * dyr[r];
             const TC gam = aa + bb + (TC)kt[r][ii] * A4;
@@ -396,11 +397,9 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
             const TC x1 = (gam * y1 + aa * y0 - bb * y2) * rD;
             const TC x2 = (gam * y2 + bb * y1 + aa * y3) * rD;
             const TC x3 = (gam * y3 + bb * y0 - aa * y2) * rD;
-            if (act) {
-                V4<TI> o;
-                o.v[perm[0]] = (TI)x0; o.v[perm[1]] = (TI)x1; o.v[perm[2]] = (TI)x2; o.v[perm[3]] = (TI)x3;
-                o.store(Inew + ((c * 4 + q) * NL + L) * 4);
-            }
+            // the old corners of the cell are no longer needed: its new ones take their registers
+            io[r][ii].v[perm[0]] = (TI)x0; io[r][ii].v[perm[1]] = (TI)x1;
+            io[r][ii].v[perm[2]] = (TI)x2; io[r][ii].v[perm[3]] = (TI)x3;
             // outgoing corner edges (:86-89)
             vb = x1; vt = x2; hlc[ii] = x3; hrc[ii] = x2;
             // corner averages for COLLAPSE_INTENSITIES (:447-470)
@@ -436,6 +435,19 @@ __device__ __forceinline__ void tile_body(const PkProb &P, TI *st, int q, int gb
     for (int ii = 0; ii < TX; ++ii) {
         hc[ii][0] = (TI)hlc[ii];
         hc[ii][1] = (TI)hrc[ii];
+    }
+    publish();
+    // the tile's I_crn (TRANSPORT_SOLVES.f08:95-98)
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int j = py ? j0 + r : ny - 1 - (j0 + r);
+#pragma unroll
+        for (int ii = 0; ii < TX; ++ii) {
+            if (r < jh && ii < iw && act) {
+                const int i = px ? i0 + ii : nx - 1 - (i0 + ii);
+                io[r][ii].store(Inew + ((((size_t)i + (size_t)nx * j) * 4 + q) * NL + L) * 4);
+            }
+        }
     }
 }
 
@@ -536,15 +548,17 @@ __device__ __forceinline__ void reduce_tile(const PkProb &P, const TS *st, doubl
     }
 }
 
-template <typename TI, typename TC, int TX, int TY, typename Sync>
+template <typename TI, typename TC, int TX, int TY, typename Sync, typename Publish>
 __device__ __forceinline__ void tile_sweep(const PkProb &P, TI *st, int q, int gb, int txl, int tyl, Sync sync,
-                                           TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
+                                           Publish publish, TI (&hc)[TX][2], bool hin_regs, bool hout_xh)
 {
     switch (q) {
-    case 0: tile_body<TI, TC, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
-    case 1: tile_body<TI, TC, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
-    case 2: tile_body<TI, TC, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
-    default: tile_body<TI, TC, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync, hc, hin_regs, hout_xh); break;
+    case 0: tile_body<TI, TC, TX, TY, true, true>(P, st, q, gb, txl, tyl, sync, publish, hc, hin_regs, hout_xh); break;
+    case 1: tile_body<TI, TC, TX, TY, false, true>(P, st, q, gb, txl, tyl, sync, publish, hc, hin_regs, hout_xh); break;
+    case 2: tile_body<TI, TC, TX, TY, false, false>(P, st, q, gb, txl, tyl, sync, publish, hc, hin_regs, hout_xh);
+        break;
+    default: tile_body<TI, TC, TX, TY, true, false>(P, st, q, gb, txl, tyl, sync, publish, hc, hin_regs, hout_xh);
+        break;
     }
 }
 
@@ -632,14 +646,15 @@ __global__ __launch_bounds__(PK_THREADS, (PkOccupancy<TI, TC, TX, TY>::waves)) v
                 if (tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
             }
             __syncthreads();
+        }, [&]() {
+            wait_vm_stores();             // this wave's face stores have completed
+            __syncthreads();
+            if (tid == 0) {
+                __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_next = nxt;
+                s_item = nit;
+            }
         }, hc, false, true);
-        wait_vm_stores();                 // this wave's face stores have completed
-        __syncthreads();
-        if (tid == 0) {
-            __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_next = nxt;
-            s_item = nit;
-        }
         tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
         __syncthreads();
         cur = __builtin_amdgcn_readfirstlane(s_next);
@@ -661,7 +676,7 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_diag(const PkProb P, int d, 
     TI *st = reinterpret_cast<TI *>(dyn + P.xsw);
     if (P.xsw == 0) xs = reinterpret_cast<double *>(st);
     TI hc[TX][2];
-    tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() { __syncthreads(); }, hc, false, true);
+    tile_sweep<TI, TC, TX, TY>(P, st, q, gb, txl, tyl, [&]() { __syncthreads(); }, [&]() {}, hc, false, true);
     __syncthreads();
     tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
 }
@@ -724,10 +739,11 @@ __global__ __launch_bounds__(PK_THREADS) void sweep_seg(PkArgs a)
                     if (tyl == t0 && tyl > 0) wait_flag(a.flags + fl - P.NTX, epoch, a.spin_limit, err);
                 }
                 __syncthreads();
+            }, [&]() {
+                wait_vm_stores();         // this wave's face stores have completed
+                __syncthreads();
+                if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }, hc, tyl > t0, tyl == t1 - 1);
-            wait_vm_stores();             // this wave's face stores have completed
-            __syncthreads();
-            if (tid == 0) __hip_atomic_store(a.flags + fl, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             tile_reduce<TI, TX, TY>(P, st, xs, q, gb, txl, tyl);
             __syncthreads();
         }
