@@ -274,9 +274,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void sweep_kernel(SweepArgs a)
 static void geom_for(const trt_problem &p, SweepGeom &sg)
 {
     sg.K = p.nm / 4;
-    sg.gblk = SWEEP_THREADS / sg.K;
-    if (sg.gblk > p.ng) sg.gblk = p.ng;
-    sg.nblk_g = (p.ng + sg.gblk - 1) / sg.gblk;
+    // as few group blocks as 256 threads allow, with the groups spread evenly over them
+    // (32 groups at S8: 16 + 16 rather than 25 + 7), so that the blocks' dependency
+    // chains carry equal work and no block is mostly padding
+    const int gmax = std::min(SWEEP_THREADS / sg.K, p.ng);
+    sg.nblk_g = (p.ng + gmax - 1) / gmax;
+    sg.gblk = (p.ng + sg.nblk_g - 1) / sg.nblk_g;
     // default tiles: 4x4 for sweep_mode 1; for the tile engine 4x2 (FP32), 2x4 (FP64) and
     // 2x2 (FP32 storage with FP64 arithmetic), the fastest measured on the 256x256, S16,
     // 32-group problem
