@@ -110,33 +110,53 @@ __global__ void nd_assemble(QDView V, const int *sysmap, int ns, int nIp, int nF
     }
 }
 
-// one block per (node, system): add the children's Schur complements and
-// eliminated right-hand sides, one child after the other
-__global__ void nd_pull(const NDLevDev *lev, int L, int ns, const int *kids)
+// Child k (0 or 1) of every node of level L: its Schur complement and eliminated
+// right-hand side added into the parent's front.  blockIdx.x = (node, system); the
+// blocks of blockIdx.y and their threads stride over the child's border block, whose
+// entries go to distinct parent entries (up is one-to-one), so no two threads of a
+// launch write the same value; the two children are two launches, in a fixed order, so
+// each parent entry is summed in the same order as one child after the other.  Each
+// thread keeps four updates in flight (the front pointers do not alias).
+__global__ __launch_bounds__(256) void nd_pull_child(const NDLevDev *lev, int L, int ns, const int *kids, int k)
 {
     const NDLevDev P = lev[L];
     const int n = blockIdx.x / ns, s = blockIdx.x % ns;
-    double *Fp = P.F + ((size_t)n * ns + s) * P.nF * P.nF;
-    double *fp = P.f + ((size_t)n * ns + s) * P.nF;
-    for (int k = 0; k < 2; ++k) {
-        const int kid = kids[n * 2 + k];
-        if (kid < 0) continue;
-        const NDLevDev C = lev[kid >> 20];
-        const int ci = kid & 0xfffff;
-        const double *Fc = C.F + ((size_t)ci * ns + s) * C.nF * C.nF;
-        const double *fc = C.f + ((size_t)ci * ns + s) * C.nF;
-        const int *up = C.up + (size_t)ci * C.nBp;
-        const int nb = C.nBp;
-        for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-            const int i = e % nb, j = e / nb;
+    const int kid = kids[n * 2 + k];
+    if (kid < 0) return;
+    const NDLevDev C = lev[kid >> 20];
+    const int ci = kid & 0xfffff;
+    double *__restrict__ Fp = P.F + ((size_t)n * ns + s) * P.nF * P.nF;
+    const double *__restrict__ Fc = C.F + ((size_t)ci * ns + s) * C.nF * C.nF + (size_t)C.nIp * C.nF + C.nIp;
+    const int *__restrict__ up = C.up + (size_t)ci * C.nBp;
+    const int nb = C.nBp, tot = nb * nb, nFp = P.nF, nFc = C.nF;
+    const int stride = gridDim.y * blockDim.x;
+    int e = blockIdx.y * blockDim.x + threadIdx.x;
+    for (; e + 3 * stride < tot; e += 4 * stride) {
+        int pos[4];
+        double val[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int eu = e + u * stride, i = eu % nb, j = eu / nb;
             const int pi = up[i], pj = up[j];
-            if (pi >= 0 && pj >= 0) Fp[(size_t)pi + (size_t)pj * P.nF] += Fc[(size_t)(C.nIp + i) + (size_t)(C.nIp + j) * C.nF];
+            pos[u] = (pi >= 0 && pj >= 0) ? pi + pj * nFp : -1;
+            val[u] = Fc[(size_t)i + (size_t)j * nFc];
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (pos[u] >= 0) Fp[pos[u]] += val[u];
+    }
+    for (; e < tot; e += stride) {
+        const int i = e % nb, j = e / nb;
+        const int pi = up[i], pj = up[j];
+        if (pi >= 0 && pj >= 0) Fp[(size_t)pi + (size_t)pj * nFp] += Fc[(size_t)i + (size_t)j * nFc];
+    }
+    if (blockIdx.y == 0) {
+        double *fp = P.f + ((size_t)n * ns + s) * P.nF;
+        const double *fc = C.f + ((size_t)ci * ns + s) * C.nF;
         for (int i = threadIdx.x; i < nb; i += blockDim.x) {
             const int pi = up[i];
             if (pi >= 0) fp[pi] += fc[C.nIp + i];
         }
-        __syncthreads();
     }
 }
 
@@ -956,7 +976,15 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
             const int nb = L.nn * ns;
             hipLaunchKernelGGL(nd_assemble, dim3(nb), dim3(128), 0, c.stream, V, P.sysmap, ns, L.nIp, L.nF, L.rows,
                                L.terms, P.W, F, f);
-            if (l > 0) hipLaunchKernelGGL(nd_pull, dim3(nb), dim3(256), 0, c.stream, P.levdev, l, ns, L.kids);
+            if (l > 0) {
+                // children's border blocks are at most the largest of the levels below
+                int mb = 0;
+                for (int q = 0; q < l; ++q) mb = std::max(mb, P.lev[q].nBp);
+                const int chunks = (int)std::min<long long>(64, std::max<long long>(1, ((long long)mb * mb + 1023) / 1024));
+                for (int k = 0; k < 2; ++k)
+                    hipLaunchKernelGGL(nd_pull_child, dim3(nb, chunks), dim3(256), 0, c.stream, P.levdev, l, ns,
+                                       L.kids, k);
+            }
             TRT_CHECK(hipGetLastError());
             const rocblas_stride sF = (rocblas_stride)L.nF * L.nF, sf = L.nF, sP = L.nIp;
             rocblas_int *piv = P.ipiv + L.poff * ns;
