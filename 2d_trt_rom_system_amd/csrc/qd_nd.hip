@@ -279,17 +279,56 @@ __global__ void nd_pull_vec(const NDLevDev *lev, int L, int ns, const int *kids)
     }
 }
 
+// Right-hand side of a front in a refinement solve, as nd_rhs and nd_pull_vec leave
+// it, inside the level's forward kernel: the pivot part of W R, then the children's
+// eliminated right-hand sides, one child after the other (the same additions in the
+// same order).  The block is the front of (node, system) blockIdx.x.
+struct NDFwdIn {
+    const double *R, *W;
+    size_t nrow;
+    const int *rows, *kids;
+    const NDLevDev *lev;
+    int L, ns;
+};
+
+__device__ __forceinline__ void nd_fwd_in(const NDFwdIn &a, int nIp, int nF, double *fb)
+{
+    const int n = blockIdx.x / a.ns, s = blockIdx.x % a.ns;
+    for (int p = threadIdx.x; p < nIp; p += blockDim.x) {
+        const int r = a.rows[(size_t)n * nF + p];
+        fb[p] = r >= 0 ? a.W[(size_t)s * a.nrow + r] * a.R[(size_t)s * a.nrow + r] : 0.0;
+    }
+    __syncthreads();
+    if (a.L == 0) return;
+    for (int k = 0; k < 2; ++k) {
+        const int kid = a.kids[n * 2 + k];
+        if (kid >= 0) {
+            const NDLevDev C = a.lev[kid >> 20];
+            const int ci = kid & 0xfffff;
+            const double *fc = C.f + ((size_t)ci * a.ns + s) * C.nF;
+            const int *up = C.up + (size_t)ci * C.nBp;
+            for (int i = threadIdx.x; i < C.nBp; i += blockDim.x) {
+                const int pi = up[i];
+                if (pi >= 0) fb[pi] += fc[C.nIp + i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // The solve phase on fronts whose pivot block holds its explicit inverse G = A_II^-1
 // (the factor-reuse mode, levels of fronts too large for nd_front_small), one block
-// per (node, system) of a level.  Forward:
-// z_I = G f_I in place of f_I, then f_B -= F_BI z_I.  Backward, with x_B gathered
-// into f_B: x_I = z_I - (A_II^-1 F_IB) x_B in place of z_I.  Matrix columns are read
-// across the threads (column-major fronts): every load instruction is coalesced.
-__global__ __launch_bounds__(256) void nd_fwd_inv(const double *F, double *f, int nIp, int nBp, int nF)
+// per (node, system) of a level.  Forward (after nd_fwd_in):
+// z_I = G f_I in place of f_I, then f_B -= F_BI z_I.  Backward, with x_B read from the
+// ancestors' solution: x_I = z_I - (A_II^-1 F_IB) x_B, written to the solution.  Matrix
+// columns are read across the threads (column-major fronts): every load instruction is
+// coalesced.
+__global__ __launch_bounds__(256) void nd_fwd_inv(const double *F, double *f, int nIp, int nBp, int nF, NDFwdIn in)
 {
     extern __shared__ double sh[];            // f_I, then z
     const double *Fb = F + (size_t)blockIdx.x * nF * nF;
     double *fb = f + (size_t)blockIdx.x * nF;
+    nd_fwd_in(in, nIp, nF, fb);
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) sh[i] = fb[i];
     __syncthreads();
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
@@ -319,12 +358,21 @@ __global__ __launch_bounds__(256) void nd_fwd_inv(const double *F, double *f, in
     }
 }
 
-__global__ __launch_bounds__(256) void nd_bwd_inv(const double *F, double *f, int nIp, int nBp, int nF)
+// backward, the fused form of nd_gather, the back substitution and nd_scatter: x_B from
+// the solution D ([system][nrow], written by the ancestors' launches), x_I into D
+__global__ __launch_bounds__(256) void nd_bwd_inv(const double *F, const double *f, int nIp, int nBp, int nF,
+                                                  double *D, size_t nrow, int ns, const int *rows)
 {
     extern __shared__ double sh[];            // x_B
+    const int n = blockIdx.x / ns, s = blockIdx.x % ns;
     const double *Fb = F + (size_t)blockIdx.x * nF * nF;
-    double *fb = f + (size_t)blockIdx.x * nF;
-    for (int q = threadIdx.x; q < nBp; q += blockDim.x) sh[q] = fb[nIp + q];
+    const double *fb = f + (size_t)blockIdx.x * nF;
+    const int *rw = rows + (size_t)n * nF;
+    double *Ds = D + (size_t)s * nrow;
+    for (int q = threadIdx.x; q < nBp; q += blockDim.x) {
+        const int r = rw[nIp + q];
+        sh[q] = r >= 0 ? Ds[r] : 0.0;
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < nIp; i += blockDim.x) {
         double a0 = fb[i], a1 = 0.0;
@@ -335,7 +383,8 @@ __global__ __launch_bounds__(256) void nd_bwd_inv(const double *F, double *f, in
             a1 = fma(-Fb[i + (size_t)(nIp + q + 1) * nF], sh[q + 1], a1);
         }
         if (q < nBp) a0 = fma(-Fb[i + (size_t)(nIp + q) * nF], sh[q], a0);
-        fb[i] = a0 + a1;
+        const int r = rw[i];
+        if (r >= 0) Ds[r] = a0 + a1;
     }
 }
 
@@ -534,12 +583,13 @@ __device__ __forceinline__ void nd_lu_bwd_step(const double (&r)[ND_SI], double 
 }
 
 __global__ __launch_bounds__(64) void nd_fwd_lu_small(const double *F, double *f, const int *perm, int nIp, int nBp,
-                                                      int nF)
+                                                      int nF, NDFwdIn in)
 {
     const size_t fr = blockIdx.x;
     const double *A = F + fr * nF * nF;
     double *v = f + fr * nF;
     const int t = threadIdx.x;
+    nd_fwd_in(in, nIp, nF, v);
     const bool live = t < nF;
     double r[ND_SI];                              // row t: L and U (pivot rows) or F_BI (border rows)
 #pragma unroll
@@ -864,30 +914,24 @@ static void nd_apply(Ctx &c, NDPlan &P, const std::vector<NDLevDev> &tab, int ns
     const double one = 1.0, mone = -1.0;
     hipLaunchKernelGGL(nd_zero, dim3(nblocks(P.vsz * ns)), dim3(256), 0, c.stream, P.f, P.vsz * ns);
     if (P.ginv) {
+        // one launch per level and direction: the right-hand side, the children's parts and
+        // the forward elimination; x_B, the back substitution and x_I into D
         for (int l = 0; l <= H; ++l) {
             const NDLevel &L = P.lev[l];
             const int nb = L.nn * ns, bs = L.nF <= 64 ? 64 : 256;
-            hipLaunchKernelGGL(nd_rhs, dim3(nb), dim3(128), 0, c.stream, P.R, P.W, nrow, ns, L.nIp, L.nF, L.rows,
-                               tab[l].f);
-            if (l > 0) hipLaunchKernelGGL(nd_pull_vec, dim3(nb), dim3(128), 0, c.stream, P.levdev, l, ns, L.kids);
+            const NDFwdIn in{P.R, P.W, nrow, L.rows, L.kids, P.levdev, l, ns};
             if (nd_small(L))
                 hipLaunchKernelGGL(nd_fwd_lu_small, dim3(nb), dim3(64), 0, c.stream, tab[l].F, tab[l].f,
-                                   P.perm + L.poff * ns, L.nIp, L.nBp, L.nF);
+                                   P.perm + L.poff * ns, L.nIp, L.nBp, L.nF, in);
             else
                 hipLaunchKernelGGL(nd_fwd_inv, dim3(nb), dim3(bs), 2 * L.nIp * sizeof(double), c.stream, tab[l].F,
-                                   tab[l].f, L.nIp, L.nBp, L.nF);
+                                   tab[l].f, L.nIp, L.nBp, L.nF, in);
         }
         for (int l = H; l >= 0; --l) {
             const NDLevel &L = P.lev[l];
             const int nb = L.nn * ns, bs = L.nF <= 64 ? 64 : 256;
-            if (L.nBp > 0) {
-                hipLaunchKernelGGL(nd_gather, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nBp,
-                                   L.nF, L.rows, tab[l].f);
-                hipLaunchKernelGGL(nd_bwd_inv, dim3(nb), dim3(bs), std::max(L.nBp, 1) * sizeof(double), c.stream,
-                                   tab[l].F, tab[l].f, L.nIp, L.nBp, L.nF);
-            }
-            hipLaunchKernelGGL(nd_scatter, dim3(nb), dim3(128), 0, c.stream, P.D, nrow, P.ident, ns, L.nIp, L.nF,
-                               L.rows, tab[l].f);
+            hipLaunchKernelGGL(nd_bwd_inv, dim3(nb), dim3(bs), std::max(L.nBp, 1) * sizeof(double), c.stream,
+                               tab[l].F, tab[l].f, L.nIp, L.nBp, L.nF, P.D, nrow, ns, L.rows);
         }
         hipLaunchKernelGGL(nd_add, dim3(nblocks((size_t)ns * nrow)), dim3(256), 0, c.stream, x, P.D, P.sysmap, ns,
                            nrow);
