@@ -831,6 +831,11 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
         po += (size_t)L.nn * L.nIp;
     }
     P->fsz = fo; P->vsz = vo; P->psz = po;
+    if (c.verbose > 1)
+        for (size_t l = 0; l < P->lev.size(); ++l)
+            fprintf(stderr, "[qd_nd plan] level %zu: %d fronts, %d pivots + %d border = %d%s\n", l, P->lev[l].nn,
+                    P->lev[l].nIp, P->lev[l].nBp, P->lev[l].nF,
+                    P->lev[l].nF <= ND_SF && P->lev[l].nIp <= ND_SI ? " (one wavefront each)" : "");
     // systems per chunk: fronts of all levels within 8 GiB
     const double budget = 8.0 * (1u << 30);
     P->cap = (int)std::max(1.0, std::min<double>(s.nsys, std::floor(budget / (8.0 * (double)fo))));
