@@ -416,12 +416,29 @@ struct NDSmall {
     bool live, pivot_row, chosen;
 };
 
+// NIP > 0 fixes the front's pivot count at compile time: the columns past nF are
+// loaded as zeros and eliminated like the others (they stay zero and are never
+// stored), so no column needs a runtime guard and a kernel is about a tenth of the
+// code of the NIP = 0 form, which guards every column by the runtime nIp and nF
+// (that form's 426 KB of code streams through the 64 KB instruction cache once per
+// wave).  Both forms do the same operations on every stored entry, bitwise.
+template <int NIP>
+__device__ __forceinline__ bool nd_col(const NDSmall &w, int j)
+{
+    return NIP > 0 || j < w.nF;
+}
+template <int NIP>
+__device__ __forceinline__ int nd_nip(const NDSmall &w)
+{
+    return NIP > 0 ? NIP : w.nIp;
+}
+
 // pivot step K (the template argument keeps every register index constant)
-template <int K>
+template <int K, int NIP>
 __device__ __forceinline__ void nd_small_elim(NDSmall &w)
 {
-    if constexpr (K < ND_SI) {
-        if (K < w.nIp) {
+    if constexpr (K < (NIP > 0 ? NIP : ND_SI)) {
+        if (NIP > 0 || K < w.nIp) {
             // the largest |entry| of column K among the pivot rows not chosen yet
             double key = (w.pivot_row && !w.chosen) ? fabs(w.r[K]) : -1.0;
             int who = w.t;
@@ -440,7 +457,7 @@ __device__ __forceinline__ void nd_small_elim(NDSmall &w)
                 w.s_orig[K] = w.t;
 #pragma unroll
                 for (int j = K; j < ND_SF; ++j)
-                    if (j < w.nF) w.prow[j] = w.r[j];
+                    if (nd_col<NIP>(w, j)) w.prow[j] = w.r[j];
                 w.prow[ND_SF] = w.rf;
             }
             __syncthreads();
@@ -451,27 +468,27 @@ __device__ __forceinline__ void nd_small_elim(NDSmall &w)
                 w.r[K] = l;
 #pragma unroll
                 for (int j = K + 1; j < ND_SF; ++j)
-                    if (j < w.nF) w.r[j] = fma(-l, w.prow[j], w.r[j]);
+                    if (nd_col<NIP>(w, j)) w.r[j] = fma(-l, w.prow[j], w.r[j]);
                 w.rf = fma(-l, w.prow[ND_SF], w.rf);
             }
             __syncthreads();
-            nd_small_elim<K + 1>(w);
+            nd_small_elim<K + 1, NIP>(w);
         }
     }
 }
 
 // back substitution step K (last pivot first): x_K = z_K / U_KK, then the
 // pivot rows chosen before K remove U_{row,K} x_K
-template <int K>
+template <int K, int NIP>
 __device__ __forceinline__ void nd_small_back(NDSmall &w)
 {
     if constexpr (K >= 0) {
-        if (K < w.nIp) {
+        if (NIP > 0 || K < w.nIp) {
             if (w.pos == K) {
                 const double inv = w.r[K] != 0.0 ? 1.0 / w.r[K] : 0.0;
 #pragma unroll
-                for (int j = 0; j < ND_SF; ++j)
-                    if (j >= w.nIp && j < w.nF) {
+                for (int j = NIP; j < ND_SF; ++j)
+                    if (j >= nd_nip<NIP>(w) && nd_col<NIP>(w, j)) {
                         w.r[j] *= inv;
                         w.prow[j] = w.r[j];
                     }
@@ -482,16 +499,17 @@ __device__ __forceinline__ void nd_small_back(NDSmall &w)
             if (w.pivot_row && w.pos < K) {
                 const double u = w.r[K];
 #pragma unroll
-                for (int j = 0; j < ND_SF; ++j)
-                    if (j >= w.nIp && j < w.nF) w.r[j] = fma(-u, w.prow[j], w.r[j]);
+                for (int j = NIP; j < ND_SF; ++j)
+                    if (j >= nd_nip<NIP>(w) && nd_col<NIP>(w, j)) w.r[j] = fma(-u, w.prow[j], w.r[j]);
                 w.rf = fma(-u, w.prow[ND_SF], w.rf);
             }
             __syncthreads();
         }
-        nd_small_back<K - 1>(w);
+        nd_small_back<K - 1, NIP>(w);
     }
 }
 
+template <int NIP>
 __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int nF, int nIp, size_t sF, size_t sf,
                                                      rocblas_int *ipiv, rocblas_int *info, int *perm, int nb)
 {
@@ -516,8 +534,8 @@ __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int n
 #pragma unroll
     for (int j = 0; j < ND_SF; ++j) w.r[j] = (w.live && j < nF) ? A[(size_t)j * nF + w.t] : 0.0;
     w.rf = w.live ? v[w.t] : 0.0;
-    nd_small_elim<0>(w);
-    nd_small_back<ND_SI - 1>(w);
+    nd_small_elim<0, NIP>(w);
+    nd_small_back<(NIP > 0 ? NIP : ND_SI) - 1, NIP>(w);
     if (w.pivot_row) {
         // LU row and A_II^-1 F_IB row at the row's pivot position
 #pragma unroll
@@ -869,6 +887,22 @@ static NDPlan *nd_plan(Ctx &c, const QDSystems &s)
 // fronts that nd_front_small factors (one wavefront each)
 static bool nd_small(const NDLevel &L) { return L.nF <= ND_SF && L.nIp <= ND_SI; }
 
+// The small-front kernel for a level: the form compiled for its pivot count where
+// there is one (the leaves of 4x4 cells and the separator levels of 4 and 8 pivots
+// above them), else the form with runtime sizes; TRT_ND_SMALL_GENERIC=1 takes the latter
+// for every level (the two give bitwise equal fronts).
+static const void *nd_front_small_kernel(int nIp)
+{
+    if (const char *v = getenv("TRT_ND_SMALL_GENERIC"); v && atoi(v) != 0)
+        return reinterpret_cast<const void *>(&nd_front_small<0>);
+    switch (nIp) {
+    case 4: return reinterpret_cast<const void *>(&nd_front_small<4>);
+    case 8: return reinterpret_cast<const void *>(&nd_front_small<8>);
+    case 48: return reinterpret_cast<const void *>(&nd_front_small<48>);
+    default: return reinterpret_cast<const void *>(&nd_front_small<0>);
+    }
+}
+
 static QDView nd_view(const QDSystems &s)
 {
     QDView V{};
@@ -1039,9 +1073,13 @@ void qd_nd(Ctx &c, QDSystems &s, const std::vector<int> &which)
             rocblas_int *piv = P.ipiv + L.poff * ns;
             if (nd_small(L)) {
                 // small fronts (the leaves and the levels above them): one kernel, one wavefront each
-                hipLaunchKernelGGL(nd_front_small, dim3(nb), dim3(64), 0, c.stream, F, f, L.nF, L.nIp, (size_t)sF,
-                                   (size_t)sf, piv, P.info + ioff, P.perm + L.poff * ns, nb);
-                TRT_CHECK(hipGetLastError());
+                double *Fa = F, *fa = f;
+                int nFa = L.nF, nIpa = L.nIp, nba = nb;
+                size_t sFa = (size_t)sF, sfa = (size_t)sf;
+                rocblas_int *pa = piv, *ia = P.info + ioff;
+                int *perma = P.perm + L.poff * ns;
+                void *args[] = {&Fa, &fa, &nFa, &nIpa, &sFa, &sfa, &pa, &ia, &perma, &nba};
+                TRT_CHECK(hipLaunchKernel(nd_front_small_kernel(L.nIp), dim3(nb), dim3(64), args, 0, c.stream));
                 ioff += (size_t)nb;
                 continue;
             }

@@ -209,3 +209,24 @@ def test_reused_factors_equal_fresh_factorizations(trt):
     assert b.qd_stats()[1] == 0
     a.close()
     b.close()
+
+
+def test_small_front_forms_bitwise(trt, monkeypatch):
+    """The small-front factorization compiled for a level's pivot count (48 at the 4x4-cell
+    leaves of a 32x32 grid, 4 and 8 at the separator levels above them) and the form with
+    runtime sizes (TRT_ND_SMALL_GENERIC=1) do the same operations on every stored entry:
+    two steps that factorize every solve end in bitwise equal states."""
+    prob = trt.problem.mapped(nx=32, order=4, ng=6, sigma=1.0)
+    prob.use_line_search = 0
+    names = ("Temp", "E_avg", "HO_E_avg", "MGQD_E_avg", "Eg_avg", "Eg_edgV", "Eg_edgH")
+    out = []
+    for generic in ("0", "1"):
+        monkeypatch.setenv("TRT_ND_SMALL_GENERIC", generic)
+        c = trt.native.Context(prob, qd_solver=3)
+        for _ in range(2):
+            c.step()
+        out.append({n: c.pull(n).copy() for n in names})
+        c.close()
+    for n in names:
+        assert np.all(np.isfinite(out[0][n])), n
+        assert np.array_equal(out[0][n], out[1][n]), n
