@@ -433,6 +433,34 @@ __device__ __forceinline__ int nd_nip(const NDSmall &w)
     return NIP > 0 ? NIP : w.nIp;
 }
 
+// One step of the pivot search's all-reduce: the partner lane's candidate by a DPP move
+// within a row of 16 lanes (CTRL: 0xB1 and 0x4E swap neighbours and pairs in a quad,
+// 0x141 mirrors 8 lanes, 0x140 mirrors 16), or by a lane shuffle across rows (XOR 16, 32).
+// Every step pairs each lane with one of another group, so after the six steps every lane
+// holds the largest candidate; the order (largest |entry|, then lowest lane) is total, so
+// the choice does not depend on the network.
+template <int CTRL>
+__device__ __forceinline__ void nd_pivot_step(double &key, int &who)
+{
+    double ok;
+    int ow;
+    if constexpr (CTRL < 0x10) {
+        // CTRL 1 and 2: lanes t ^ 16 and t ^ 32
+        ok = __shfl_xor(key, CTRL << 4);
+        ow = __shfl_xor(who, CTRL << 4);
+    } else {
+        const long long kb = __double_as_longlong(key);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)kb, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(kb >> 32), CTRL, 0xF, 0xF, false);
+        ok = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+        ow = __builtin_amdgcn_update_dpp(0, who, CTRL, 0xF, 0xF, false);
+    }
+    if (ok > key || (ok == key && ow < who)) {
+        key = ok;
+        who = ow;
+    }
+}
+
 // pivot step K (the template argument keeps every register index constant)
 template <int K, int NIP>
 __device__ __forceinline__ void nd_small_elim(NDSmall &w)
@@ -442,12 +470,21 @@ __device__ __forceinline__ void nd_small_elim(NDSmall &w)
             // the largest |entry| of column K among the pivot rows not chosen yet
             double key = (w.pivot_row && !w.chosen) ? fabs(w.r[K]) : -1.0;
             int who = w.t;
-            for (int o = 1; o < 64; o <<= 1) {
-                const double ok = __shfl_xor(key, o);
-                const int ow = __shfl_xor(who, o);
-                if (ok > key || (ok == key && ow < who)) {
-                    key = ok;
-                    who = ow;
+            if constexpr (NIP > 0) {
+                nd_pivot_step<0xB1>(key, who);
+                nd_pivot_step<0x4E>(key, who);
+                nd_pivot_step<0x141>(key, who);
+                nd_pivot_step<0x140>(key, who);
+                nd_pivot_step<1>(key, who);
+                nd_pivot_step<2>(key, who);
+            } else {
+                for (int o = 1; o < 64; o <<= 1) {
+                    const double ok = __shfl_xor(key, o);
+                    const int ow = __shfl_xor(who, o);
+                    if (ok > key || (ok == key && ow < who)) {
+                        key = ok;
+                        who = ow;
+                    }
                 }
             }
             const int p = __builtin_amdgcn_readfirstlane(who);
