@@ -610,14 +610,22 @@ __global__ __launch_bounds__(64) void nd_front_small(double *F, double *f, int n
 // The forward solve phase on a level of small fronts with the LU factors nd_front_small
 // left (one wavefront per front, lane t holding row t of the front in registers, all of
 // it loaded at once): z_I = A_II^-1 f_I by the row permutation, the unit lower and the
-// upper triangular solves, then f_B -= F_BI z_I, the pivots' values broadcast by lane
-// shuffles.  The template steps keep every register index constant.
+// upper triangular solves, then f_B -= F_BI z_I, the pivots' values broadcast by
+// v_readlane (the source lane is uniform, so no LDS crossbar round trip).  The template
+// steps keep every register index constant.
+__device__ __forceinline__ double nd_lane(double x, int lane)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <int K>
 __device__ __forceinline__ void nd_lu_fwd_step(const double (&r)[ND_SI], double &y, int t, int nIp)
 {
     if constexpr (K < ND_SI) {
         if (K < nIp) {
-            const double yk = __shfl(y, K);
+            const double yk = nd_lane(y, K);
             if (t > K && t < nIp) y = fma(-r[K], yk, y);
             nd_lu_fwd_step<K + 1>(r, y, t, nIp);
         }
@@ -630,7 +638,7 @@ __device__ __forceinline__ void nd_lu_bwd_step(const double (&r)[ND_SI], double 
     if constexpr (K >= 0) {
         if (K < nIp) {
             if (t == K) y = y / r[K];
-            const double yk = __shfl(y, K);
+            const double yk = nd_lane(y, K);
             if (t < K) y = fma(-r[K], yk, y);
         }
         nd_lu_bwd_step<K - 1>(r, y, t, nIp);
@@ -657,7 +665,7 @@ __global__ __launch_bounds__(64) void nd_fwd_lu_small(const double *F, double *f
 #pragma unroll
     for (int k = 0; k < ND_SI; ++k) {
         if (k < nIp) {
-            const double zk = __shfl(z, k);
+            const double zk = nd_lane(z, k);
             if (t >= nIp) y = fma(-r[k], zk, y);
         }
     }
